@@ -1,0 +1,116 @@
+/*
+ * posit_gpu_amd.h -- C ABI of the MI355X (gfx950) posit32 dense linear algebra engine.
+ *
+ * Plain C types only: a posit32 value is its 32-bit pattern (uint32_t), matrices are
+ * column-major with leading dimensions (as in the reference's libRpack), and device
+ * memory / streams are opaque pointers.  Every function returns a pga_status; an
+ * illegal argument gives PGA_ERROR_ILLEGAL_ARGUMENT and pga_illegal_argument() then
+ * reports the 1-based parameter number the reference's Mxerbla would have printed.
+ *
+ * The reference interface each entry point replaces is cited beside it (paths relative
+ * to the reference tree).  The C++ drop-in of mpblas_posit.h / mplapack_posit.h
+ * (include/mpblas_posit.h, include/mplapack_posit.h) is built on these functions.
+ */
+#ifndef POSIT_GPU_AMD_H
+#define POSIT_GPU_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    PGA_SUCCESS = 0,
+    PGA_ERROR_ILLEGAL_ARGUMENT = 1, /* see pga_illegal_argument() */
+    PGA_ERROR_NOT_INITIALIZED = 2,
+    PGA_ERROR_HIP = 3,              /* a HIP runtime call failed */
+    PGA_ERROR_OUT_OF_MEMORY = 4,
+    PGA_ERROR_NO_DEVICE = 5,
+    PGA_ERROR_UNSUPPORTED = 6
+} pga_status;
+
+typedef struct {
+    char name[64];
+    char arch[32];
+    int compute_units;        /* hipDeviceProp_t.multiProcessorCount */
+    int clock_khz;            /* hipDeviceProp_t.clockRate */
+    int simd_per_cu;          /* 4 on CDNA */
+    int lanes_per_simd;       /* 16 on CDNA: a wave64 VALU op takes 4 cycles */
+    double int32_valu_peak;   /* CU * SIMD * lanes * clock, integer lane-ops per second */
+    size_t hbm_bytes;
+} pga_device_info;
+
+/* ---- runtime ------------------------------------------------------------------- */
+
+/* Select and initialise a device.  Replaces initOpenCL (libRpack/Rgemm_opencl.cpp:59-70)
+ * and its one-time set-up (init_gpu, :22-57; OclHelper.hpp:43-285).  Idempotent.
+ * device < 0 picks PGA_DEVICE from the environment, else device 0. */
+pga_status pga_init(int device);
+pga_status pga_finalize(void);
+int pga_is_initialized(void);
+const char *pga_status_string(pga_status s);
+/* 1-based parameter number of the last illegal argument on this thread (0 if none). */
+int pga_illegal_argument(void);
+pga_status pga_get_device_info(pga_device_info *info);
+
+/* GEMM implementation: 0 = tiled decoded kernel (default), 1 = exact one-element-per-
+ * thread kernel (validation).  Also set by PGA_GEMM_IMPL=tiled|exact. */
+pga_status pga_set_gemm_impl(int impl);
+
+/* ---- device memory and streams (plumbing for resident data) ------------------- */
+pga_status pga_malloc(void **dptr, size_t bytes);
+pga_status pga_free(void *dptr);
+pga_status pga_memcpy_h2d(void *dst, const void *src, size_t bytes);
+pga_status pga_memcpy_d2h(void *dst, const void *src, size_t bytes);
+pga_status pga_memcpy_d2d(void *dst, const void *src, size_t bytes);
+pga_status pga_memset(void *dptr, int value, size_t bytes);
+pga_status pga_stream_create(void **stream);
+pga_status pga_stream_destroy(void *stream);
+pga_status pga_stream_synchronize(void *stream); /* NULL: the library's default stream */
+pga_status pga_device_synchronize(void);
+pga_status pga_event_create(void **event);
+pga_status pga_event_destroy(void *event);
+pga_status pga_event_record(void *event, void *stream);
+pga_status pga_event_synchronize(void *event);
+pga_status pga_event_elapsed_ms(float *ms, void *start, void *stop);
+
+/* ---- posit32 GEMM ----------------------------------------------------------------
+ * C = alpha*op(A)*op(B) + beta*C with the per-element rounding order of the CPU Rgemm
+ * (libRpack/Rgemm.cpp:62-251): NN/NT accumulate (alpha*b)*a into beta*C in ascending
+ * k; TN/TT sum a*b from zero, then alpha*t + beta*C.
+ *
+ * pga_rgemm: host pointers, synchronous.  Replaces RgemmG (libRpack/Rgemm_opencl.cpp:
+ * 199-250) and Rgemm_internal (:72-197); copies only the logical sub-matrices.  transa /
+ * transb follow Rgemm's argument check ('N','T','C' in either case; 'C' means 'T').
+ * pga_rgemm_device: the same on device pointers, asynchronous on `stream`. */
+pga_status pga_rgemm(char transa, char transb, int m, int n, int k, uint32_t alpha,
+                     const uint32_t *a, int lda, const uint32_t *b, int ldb, uint32_t beta,
+                     uint32_t *c, int ldc);
+pga_status pga_rgemm_device(char transa, char transb, int m, int n, int k, uint32_t alpha,
+                            const uint32_t *a, int lda, const uint32_t *b, int ldb,
+                            uint32_t beta, uint32_t *c, int ldc, void *stream);
+
+/* ---- conversions (device codecs) -------------------------------------------------
+ * double <-> posit32 on device arrays: round to nearest, ties to the even pattern,
+ * saturating at minpos/maxpos; NaN and infinities become NaR.  (The reference converts
+ * with SoftPosit's posit32(double) constructor and toDouble(), e.g. main_gemm.cpp:99-104.) */
+pga_status pga_p32_from_f64_device(const double *x, uint32_t *p, size_t n, void *stream);
+pga_status pga_p32_to_f64_device(const uint32_t *p, double *x, size_t n, void *stream);
+
+/* Synthetic inputs: p[i] = posit32(lo + (hi-lo)*u_i) with u_i = SplitMix64(seed, i)
+ * mapped to [0,1) with 53 bits.  (The reference draws from std::random_device,
+ * util.cpp:166-233; the configurations here fix seeded uniform inputs instead.) */
+pga_status pga_fill_uniform_p32_device(uint32_t *p, size_t n, uint64_t seed, double lo, double hi,
+                                       void *stream);
+
+/* ---- elementwise scalar arithmetic on device arrays (validation of the core) ---- */
+/* op: 0 add, 1 sub, 2 mul, 3 div, 4 sqrt (b ignored) */
+pga_status pga_p32_elementwise_device(int op, const uint32_t *a, const uint32_t *b,
+                                      uint32_t *out, size_t n, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* POSIT_GPU_AMD_H */

@@ -1,0 +1,308 @@
+"""posit_gpu_amd -- MI355X (gfx950) posit32 dense linear algebra.
+
+Python access to the C ABI of posit_gpu_amd/lib/libposit_gpu_amd.so (declared in
+include/posit_gpu_amd.h).  Every computation runs in the HIP library on the GPU; this
+module only moves arrays and calls entry points.  Importing it without the built library
+raises ImportError: there is no host fallback.
+
+Posit32 values are numpy uint32 arrays holding the bit patterns; matrices are
+column-major (Fortran order), as in the reference's libRpack.
+
+Names that mirror the reference API (RgemmG, Rgemm, ...) take the reference's argument
+lists (libRpack/mpblas_posit.h:37-55, libRpack/mplapack_posit.h:38-57) with numpy arrays
+in place of posit32 pointers.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libposit_gpu_amd.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(
+        f"posit_gpu_amd: HIP library not built ({LIB_PATH}); run __graft_entry__.build() "
+        "or make -C posit_gpu_amd/csrc"
+    )
+
+_lib = ctypes.CDLL(LIB_PATH)
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_U32 = ctypes.c_uint32
+_C = ctypes.c_char
+_SZ = ctypes.c_size_t
+
+P32_ZERO = 0x00000000
+P32_ONE = 0x40000000
+P32_MONE = 0xC0000000
+P32_NAR = 0x80000000
+
+
+class PositError(RuntimeError):
+    """A call into the HIP library failed."""
+
+    def __init__(self, status, where, argument=0):
+        self.status = status
+        self.argument = argument
+        msg = _lib.pga_status_string(status).decode()
+        if argument:
+            msg += f" (parameter {argument})"
+        super().__init__(f"{where}: {msg}")
+
+
+class DeviceInfo(ctypes.Structure):
+    _fields_ = [
+        ("name", ctypes.c_char * 64),
+        ("arch", ctypes.c_char * 32),
+        ("compute_units", ctypes.c_int),
+        ("clock_khz", ctypes.c_int),
+        ("simd_per_cu", ctypes.c_int),
+        ("lanes_per_simd", ctypes.c_int),
+        ("int32_valu_peak", ctypes.c_double),
+        ("hbm_bytes", ctypes.c_size_t),
+    ]
+
+
+def _sig(name, args, res=_I):
+    f = getattr(_lib, name)
+    f.argtypes = args
+    f.restype = res
+    return f
+
+
+_sig("pga_init", [_I])
+_sig("pga_finalize", [])
+_sig("pga_is_initialized", [])
+_sig("pga_status_string", [_I], ctypes.c_char_p)
+_sig("pga_illegal_argument", [])
+_sig("pga_get_device_info", [ctypes.POINTER(DeviceInfo)])
+_sig("pga_set_gemm_impl", [_I])
+_sig("pga_malloc", [ctypes.POINTER(_P), _SZ])
+_sig("pga_free", [_P])
+_sig("pga_memcpy_h2d", [_P, _P, _SZ])
+_sig("pga_memcpy_d2h", [_P, _P, _SZ])
+_sig("pga_memcpy_d2d", [_P, _P, _SZ])
+_sig("pga_memset", [_P, _I, _SZ])
+_sig("pga_stream_create", [ctypes.POINTER(_P)])
+_sig("pga_stream_destroy", [_P])
+_sig("pga_stream_synchronize", [_P])
+_sig("pga_device_synchronize", [])
+_sig("pga_event_create", [ctypes.POINTER(_P)])
+_sig("pga_event_destroy", [_P])
+_sig("pga_event_record", [_P, _P])
+_sig("pga_event_synchronize", [_P])
+_sig("pga_event_elapsed_ms", [ctypes.POINTER(ctypes.c_float), _P, _P])
+_sig("pga_rgemm", [_C, _C, _I, _I, _I, _U32, _P, _I, _P, _I, _U32, _P, _I])
+_sig("pga_rgemm_device", [_C, _C, _I, _I, _I, _U32, _P, _I, _P, _I, _U32, _P, _I, _P])
+_sig("pga_p32_from_f64_device", [_P, _P, _SZ, _P])
+_sig("pga_p32_to_f64_device", [_P, _P, _SZ, _P])
+_sig("pga_fill_uniform_p32_device", [_P, _SZ, ctypes.c_uint64, ctypes.c_double, ctypes.c_double, _P])
+_sig("pga_p32_elementwise_device", [_I, _P, _P, _P, _SZ, _P])
+
+
+def _check(status, where):
+    if status != 0:
+        raise PositError(status, where, _lib.pga_illegal_argument() if status == 1 else 0)
+
+
+def init(device=-1):
+    """Select the GPU (initOpenCL in the reference, libRpack/Rgemm_opencl.cpp:59)."""
+    _check(_lib.pga_init(device), "pga_init")
+
+
+def initOpenCL():  # noqa: N802 -- reference name (libRpack/def.h:63)
+    init(-1)
+
+
+def finalize():
+    _check(_lib.pga_finalize(), "pga_finalize")
+
+
+def device_info():
+    init()
+    info = DeviceInfo()
+    _check(_lib.pga_get_device_info(ctypes.byref(info)), "pga_get_device_info")
+    return {
+        "name": info.name.decode(),
+        "arch": info.arch.decode(),
+        "compute_units": info.compute_units,
+        "clock_khz": info.clock_khz,
+        "simd_per_cu": info.simd_per_cu,
+        "lanes_per_simd": info.lanes_per_simd,
+        "int32_valu_peak": info.int32_valu_peak,
+        "hbm_bytes": info.hbm_bytes,
+    }
+
+
+def set_gemm_impl(impl):
+    """'tiled' (default) or 'exact' (one element per thread, pattern-level ops)."""
+    init()
+    _check(_lib.pga_set_gemm_impl({"tiled": 0, "exact": 1}[impl]), "pga_set_gemm_impl")
+
+
+# ---------------------------------------------------------------------------------
+# device memory
+# ---------------------------------------------------------------------------------
+class DeviceArray:
+    """A device allocation holding a numpy-shaped array (column-major for matrices)."""
+
+    def __init__(self, shape, dtype=np.uint32, order="F"):
+        init()
+        self.shape = tuple(shape) if isinstance(shape, (tuple, list)) else (int(shape),)
+        self.dtype = np.dtype(dtype)
+        self.order = order
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = _P()
+        _check(_lib.pga_malloc(ctypes.byref(p), max(self.nbytes, 4)), "pga_malloc")
+        self.ptr = p.value
+
+    @classmethod
+    def from_host(cls, a):
+        a = np.asarray(a)
+        order = "F" if a.ndim > 1 else "C"
+        d = cls(a.shape, a.dtype, order)
+        h = np.asfortranarray(a) if order == "F" else np.ascontiguousarray(a)
+        if d.nbytes:
+            _check(_lib.pga_memcpy_h2d(d.ptr, h.ctypes.data, d.nbytes), "pga_memcpy_h2d")
+        return d
+
+    def to_host(self):
+        h = np.empty(self.shape, dtype=self.dtype, order=self.order)
+        if self.nbytes:
+            _check(_lib.pga_memcpy_d2h(h.ctypes.data, self.ptr, self.nbytes), "pga_memcpy_d2h")
+        return h
+
+    def copy_from(self, other):
+        assert other.nbytes == self.nbytes
+        _check(_lib.pga_memcpy_d2d(self.ptr, other.ptr, self.nbytes), "pga_memcpy_d2d")
+
+    def offset(self, elements):
+        """Raw device pointer `elements` items into the buffer."""
+        return self.ptr + int(elements) * self.dtype.itemsize
+
+    def free(self):
+        if self.ptr:
+            _lib.pga_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Event:
+    def __init__(self):
+        init()
+        p = _P()
+        _check(_lib.pga_event_create(ctypes.byref(p)), "pga_event_create")
+        self.ptr = p.value
+
+    def record(self, stream=None):
+        _check(_lib.pga_event_record(self.ptr, stream), "pga_event_record")
+
+    def synchronize(self):
+        _check(_lib.pga_event_synchronize(self.ptr), "pga_event_synchronize")
+
+    def elapsed_ms(self, later):
+        ms = ctypes.c_float()
+        _check(_lib.pga_event_elapsed_ms(ctypes.byref(ms), self.ptr, later.ptr), "pga_event_elapsed_ms")
+        return float(ms.value)
+
+    def __del__(self):
+        try:
+            _lib.pga_event_destroy(self.ptr)
+        except Exception:
+            pass
+
+
+def synchronize(stream=None):
+    _check(_lib.pga_stream_synchronize(stream), "pga_stream_synchronize")
+
+
+# ---------------------------------------------------------------------------------
+# codecs and elementwise arithmetic (device)
+# ---------------------------------------------------------------------------------
+def from_double(x):
+    """float64 array -> posit32 patterns, converted on the GPU."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    dx = DeviceArray.from_host(x.ravel())
+    dp = DeviceArray(x.size, np.uint32, "C")
+    _check(_lib.pga_p32_from_f64_device(dx.ptr, dp.ptr, x.size, None), "pga_p32_from_f64_device")
+    synchronize()
+    return dp.to_host().reshape(x.shape)
+
+
+def to_double(p):
+    p = np.ascontiguousarray(p, dtype=np.uint32)
+    dp = DeviceArray.from_host(p.ravel())
+    dx = DeviceArray(p.size, np.float64, "C")
+    _check(_lib.pga_p32_to_f64_device(dp.ptr, dx.ptr, p.size, None), "pga_p32_to_f64_device")
+    synchronize()
+    return dx.to_host().reshape(p.shape)
+
+
+_OPS = {"add": 0, "sub": 1, "mul": 2, "div": 3, "sqrt": 4}
+
+
+def elementwise(op, a, b=None):
+    a = np.ascontiguousarray(a, dtype=np.uint32).ravel()
+    b = a if b is None else np.ascontiguousarray(b, dtype=np.uint32).ravel()
+    da, db = DeviceArray.from_host(a), DeviceArray.from_host(b)
+    do = DeviceArray(a.size, np.uint32, "C")
+    _check(_lib.pga_p32_elementwise_device(_OPS[op], da.ptr, db.ptr, do.ptr, a.size, None),
+           "pga_p32_elementwise_device")
+    synchronize()
+    return do.to_host()
+
+
+def fill_uniform(d, seed, lo=-1.0, hi=1.0, stream=None):
+    """Fill a DeviceArray of uint32 with posit32(lo + (hi-lo)*u), u from SplitMix64(seed, i)."""
+    n = int(np.prod(d.shape))
+    _check(_lib.pga_fill_uniform_p32_device(d.ptr, n, seed, lo, hi, stream), "pga_fill_uniform_p32_device")
+
+
+# ---------------------------------------------------------------------------------
+# GEMM
+# ---------------------------------------------------------------------------------
+def _ch(c):
+    return c.encode()[:1] if isinstance(c, str) else bytes(c)[:1]
+
+
+def rgemm_device(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream=None):
+    """C = alpha*op(A)*op(B) + beta*C on device pointers (ints) or DeviceArrays."""
+    pa = a.ptr if isinstance(a, DeviceArray) else a
+    pb = b.ptr if isinstance(b, DeviceArray) else b
+    pc = c.ptr if isinstance(c, DeviceArray) else c
+    _check(_lib.pga_rgemm_device(_ch(transa), _ch(transb), m, n, k, alpha, pa, lda, pb, ldb, beta, pc, ldc,
+                                 stream), "pga_rgemm_device")
+
+
+def RgemmG(transa, transb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc):  # noqa: N802
+    """libRpack/mpblas_posit.h:49-55 on host numpy uint32 buffers (C updated in place).
+
+    A, B and C are flat (or Fortran-ordered) buffers addressed with the leading
+    dimensions, exactly as the reference's posit32 pointers.
+    """
+    for x in (A, B, C):
+        assert x.dtype == np.uint32
+    assert C.flags.writeable and (C.flags.f_contiguous or C.flags.c_contiguous)
+    _check(_lib.pga_rgemm(_ch(transa), _ch(transb), m, n, k, alpha, A.ctypes.data, lda, B.ctypes.data, ldb,
+                          beta, C.ctypes.data, ldc), "pga_rgemm")
+
+
+Rgemm = RgemmG  # both run on the GPU with Rgemm's per-element order
+
+
+def rgemm(transa, transb, alpha, A, B, beta, C):
+    """Convenience form on Fortran-ordered uint32 matrices; returns the new C."""
+    A = np.asfortranarray(A, dtype=np.uint32)
+    B = np.asfortranarray(B, dtype=np.uint32)
+    C = np.array(C, dtype=np.uint32, order="F", copy=True)
+    m, n = C.shape
+    k = A.shape[0] if transa.upper() in "TC" else A.shape[1]
+    RgemmG(transa, transb, m, n, k, alpha, A, max(1, A.shape[0]), B, max(1, B.shape[0]), beta, C, max(1, m))
+    return C

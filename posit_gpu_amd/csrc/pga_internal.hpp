@@ -1,0 +1,47 @@
+// pga_internal.hpp -- shared declarations of the HIP runtime layer (not part of the ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/posit_gpu_amd.h"
+
+namespace pga {
+
+enum class GemmImpl { Tiled = 0, Exact = 1 };
+
+// Device-pointer GEMM launch (rgemm.hip); arguments already validated.
+hipError_t rgemm_launch(bool ta, bool tb, int m, int n, int k, uint32_t alpha, const uint32_t* A,
+                        int lda, const uint32_t* B, int ldb, uint32_t beta, uint32_t* C, int ldc,
+                        hipStream_t s, GemmImpl impl);
+
+// Runtime state (runtime.hip)
+struct Runtime {
+  bool initialized = false;
+  int device = -1;
+  hipStream_t stream = nullptr;
+  GemmImpl gemm_impl = GemmImpl::Tiled;
+};
+Runtime& runtime();
+
+void set_illegal_argument(int index);
+pga_status from_hip(hipError_t e);
+
+// Growable device scratch buffers used by the host-pointer entry points.
+void* scratch(int slot, size_t bytes, pga_status* st);
+
+}  // namespace pga
+
+#define PGA_CHECK_INIT()                                         \
+  do {                                                           \
+    if (!::pga::runtime().initialized) {                         \
+      pga_status _s = pga_init(-1);                              \
+      if (_s != PGA_SUCCESS) return _s;                          \
+    }                                                            \
+  } while (0)
+
+#define PGA_HIP(call)                                            \
+  do {                                                           \
+    hipError_t _e = (call);                                      \
+    if (_e != hipSuccess) return ::pga::from_hip(_e);            \
+  } while (0)

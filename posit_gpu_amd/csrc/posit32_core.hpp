@@ -1,0 +1,346 @@
+// posit32_core.hpp -- posit32 (es = 2) arithmetic for gfx950 kernels and host code.
+//
+// Two layers:
+//
+//  * Pattern-level operations (pga::add, sub, mul, div, sqrt, from_double, ...):
+//    bit-exact with the SoftPosit routines the reference uses.  add and mul follow the
+//    reference's OpenCL port of SoftPosit (libRpack/kernel_entry.cl:88-538): the same
+//    64-bit fraction frames, the same truncating alignment shift and the same rounding
+//    tail, including its treatment of regimes 29 and 30.  Regimes are decoded with a
+//    count-leading-zeros instead of the reference's bit-by-bit loops.  div, sqrt and the
+//    double conversion are correctly rounded (round to nearest, ties to the even
+//    pattern, saturating at minpos / maxpos).
+//
+//  * Decoded-domain operations (pga::dec): the representation the GEMM inner loop works
+//    in.  A value is (m, sc, sg): mantissa with its hidden bit at bit 30 (the fraction
+//    below it, the three lowest bits clear), scale sc = 4k + e, and sign sg (0 or bit
+//    31).  For results whose regime is at most 27 bits long the SoftPosit routines are
+//    correctly rounded on the posit bit string, so in that range a fused
+//    multiply-then-add can be computed with 32-bit integer arithmetic carrying a guard,
+//    round and sticky bit.  Callers must route every other case to the pattern-level
+//    operations (the GEMM does so with range flags).
+#pragma once
+
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define PGA_HD __host__ __device__ __forceinline__
+#define PGA_D __device__ __forceinline__
+#else
+#define PGA_HD inline
+#define PGA_D inline
+#endif
+
+namespace pga {
+
+constexpr uint32_t P_ZERO = 0x00000000u;
+constexpr uint32_t P_ONE = 0x40000000u;
+constexpr uint32_t P_MONE = 0xC0000000u;
+constexpr uint32_t P_NAR = 0x80000000u;
+constexpr uint32_t P_MAXPOS = 0x7FFFFFFFu;
+constexpr uint32_t P_MINPOS = 0x00000001u;
+
+PGA_HD int clz32(uint32_t x) { return x ? __builtin_clz(x) : 32; }
+PGA_HD int clz64(uint64_t x) { return x ? __builtin_clzll(x) : 64; }
+PGA_HD uint32_t neg(uint32_t a) { return 0u - a; }
+PGA_HD bool lt(uint32_t a, uint32_t b) { return (int32_t)a < (int32_t)b; }
+PGA_HD bool le(uint32_t a, uint32_t b) { return (int32_t)a <= (int32_t)b; }
+PGA_HD uint32_t abs_p(uint32_t a) { return ((int32_t)a < 0) ? 0u - a : a; }
+
+// Regime, exponent and fraction of a positive, nonzero, non-NaR pattern.  `frac31` is
+// the fraction with its hidden bit at bit 30, the layout SoftPosit uses.
+struct Parts {
+  int scale;        // 4k + e
+  uint32_t frac31;  // hidden bit at 30, fraction bits from 29 downwards
+};
+
+PGA_HD Parts parts(uint32_t u) {
+  uint32_t x = u << 1;                            // regime starts at bit 31
+  uint32_t ones = (uint32_t)((int32_t)x >> 31);   // all ones for a run of ones
+  int run = clz32(x ^ ones);                      // 1 .. 31
+  uint32_t after = (x << run) << 1;               // drop the run and its terminator
+  int k = ones ? run - 1 : -run;
+  Parts p;
+  p.scale = 4 * k + (int)(after >> 30);
+  p.frac31 = 0x40000000u | (after & 0x3FFFFFFFu);
+  return p;
+}
+
+// The rounding tail shared by add and mul (kernel_entry.cl:167-213 / 487-533): `f`
+// carries the magnitude with its hidden bit at 62; returns the positive pattern.
+PGA_HD uint32_t round_tail(int scale, uint64_t f) {
+  int k = scale >> 2;  // arithmetic shift: floor(scale / 4)
+  uint32_t e = (uint32_t)(scale & 3);
+  int reg = (k < 0) ? -k : k + 1;
+  if (reg > 30) return (k >= 0) ? P_MAXPOS : P_MINPOS;
+  uint32_t regime = (k >= 0) ? (0x7FFFFFFFu - (0x7FFFFFFFu >> reg)) : (0x40000000u >> reg);
+  uint64_t w = (f & 0x3FFFFFFFFFFFFFFFull) >> (reg + 2);
+  uint32_t frac = (uint32_t)(w >> 32);
+  uint32_t guard;
+  if (reg <= 28) {
+    guard = (uint32_t)(w >> 31) & 1u;
+    e <<= (28 - reg);
+  } else {
+    // regime 29 keeps one exponent bit, regime 30 none; the fraction bits are all
+    // below the last kept bit
+    guard = (reg == 30) ? (e >> 1) : (e & 1u);
+    e = (reg == 30) ? 0u : (e >> 1);
+    frac = 0;
+  }
+  uint32_t ui = regime + e + frac;
+  if (guard) {
+    // as in the reference, the sticky bit is taken from the fraction window only
+    uint32_t sticky = (w & 0x7FFFFFFFull) != 0;
+    ui += (ui & 1u) | sticky;
+  }
+  return ui;
+}
+
+// posit32 addition, bit-exact with kernel_entry.cl:370-397.
+PGA_HD uint32_t add(uint32_t a, uint32_t b) {
+  if (a == 0 || b == 0) return a | b;
+  if (a == P_NAR || b == P_NAR) return P_NAR;
+  bool negz = (a >> 31) != 0;
+  bool subtract = ((a ^ b) >> 31) != 0;
+  uint32_t ua = negz ? 0u - a : a;
+  uint32_t ub = (negz != subtract) ? 0u - b : b;  // |b|
+  if (subtract && ua == ub) return 0;
+  if ((int32_t)ua < (int32_t)ub) {
+    uint32_t t = ua;
+    ua = ub;
+    ub = t;
+    if (subtract) negz = !negz;
+  }
+  Parts pa = parts(ua), pb = parts(ub);
+  int shift = pa.scale - pb.scale;  // == 4(kA-kB) + eA - eB >= 0
+  uint64_t fa = (uint64_t)pa.frac31 << 32;
+  uint64_t fb = (uint64_t)pb.frac31 << 32;
+  int scale = pa.scale;
+  uint64_t f;
+  if (!subtract) {
+    f = fa + ((shift > 63) ? 0ull : (fb >> shift));
+    if (f >> 63) {
+      f >>= 1;
+      scale += 1;
+    }
+  } else {
+    if (shift > 63) return negz ? 0u - ua : ua;
+    f = fa - (fb >> shift);
+    int z = clz64(f) - 1;  // f != 0 here
+    f <<= z;
+    scale -= z;
+  }
+  uint32_t ui = round_tail(scale, f);
+  return negz ? 0u - ui : ui;
+}
+
+PGA_HD uint32_t sub(uint32_t a, uint32_t b) {
+  if (a == P_NAR || b == P_NAR) return P_NAR;
+  return add(a, 0u - b);
+}
+
+// posit32 multiplication, bit-exact with kernel_entry.cl:399-538.
+PGA_HD uint32_t mul(uint32_t a, uint32_t b) {
+  if (a == P_NAR || b == P_NAR) return P_NAR;
+  if (a == 0 || b == 0) return 0;
+  bool negz = ((a ^ b) >> 31) != 0;
+  Parts pa = parts((a >> 31) ? 0u - a : a);
+  Parts pb = parts((b >> 31) ? 0u - b : b);
+  int scale = pa.scale + pb.scale;
+  uint64_t p = (uint64_t)pa.frac31 * (uint64_t)pb.frac31;  // hidden at 60 or 61
+  if (p >> 61) {
+    p >>= 1;
+    scale += 1;
+  }
+  uint32_t ui = round_tail(scale, p << 2);
+  return negz ? 0u - ui : ui;
+}
+
+// ---- correctly rounded encoder (div, sqrt, conversions) ----
+
+// Round (-1)^negz * m * 2^(scale - hb) to the nearest posit32 on the bit string (ties to
+// the even pattern); m has its leading one at bit hb (29 <= hb <= 63) and `sticky` says
+// whether nonzero bits lie below m.  Saturates at minpos / maxpos.
+PGA_HD uint32_t encode_rne(bool negz, int scale, uint64_t m, int hb, bool sticky) {
+  int k = scale >> 2;
+  uint32_t e = (uint32_t)(scale & 3);
+  int reg = (k < 0) ? -k : k + 1;
+  uint32_t ui;
+  if (reg > 30) {
+    ui = (k >= 0) ? P_MAXPOS : P_MINPOS;
+  } else {
+    // normalise m to the leading one at bit 63, then the bit string after the sign is
+    // [regime: reg+1 bits][exponent: 2][fraction: 63 bits]; keep 31 bits.
+    uint64_t f = m << (63 - hb);
+    uint64_t fr = f << 1;  // fraction bits, top-aligned
+    int head = reg + 1 + 2;  // 4 .. 33 bits of regime and exponent
+    uint32_t regime = (k >= 0) ? (0x7FFFFFFFu - (0x7FFFFFFFu >> reg)) : (0x40000000u >> reg);
+    uint32_t kept, guard;
+    bool rest;
+    if (head <= 31) {
+      int nf = 31 - head;  // fraction bits that fit
+      uint32_t fkeep = nf ? (uint32_t)(fr >> (64 - nf)) : 0u;
+      kept = regime + (e << nf) + fkeep;
+      uint64_t below = nf ? (fr << nf) : fr;
+      guard = (uint32_t)(below >> 63);
+      rest = ((below << 1) != 0) || sticky;
+    } else if (head == 32) {  // reg == 29: one exponent bit kept
+      kept = regime + (e >> 1);
+      guard = e & 1u;
+      rest = (fr != 0) || sticky;
+    } else {  // reg == 30: no exponent bit kept
+      kept = regime;
+      guard = e >> 1;
+      rest = (e & 1u) || (fr != 0) || sticky;
+    }
+    if (guard && (rest || (kept & 1u))) kept++;
+    ui = kept;
+  }
+  return negz ? 0u - ui : ui;
+}
+
+// Exact value of a finite nonzero pattern: (-1)^negz * m27 * 2^(scale - 27) with the
+// leading one of m27 at bit 27.
+PGA_HD void unpack27(uint32_t a, bool& negz, int& scale, uint32_t& m27) {
+  negz = (a >> 31) != 0;
+  Parts p = parts(negz ? 0u - a : a);
+  scale = p.scale;
+  m27 = p.frac31 >> 3;
+}
+
+PGA_HD uint32_t div(uint32_t a, uint32_t b) {
+  if (a == P_NAR || b == P_NAR || b == 0) return P_NAR;
+  if (a == 0) return 0;
+  bool na, nb;
+  int sa, sb;
+  uint32_t ma, mb;
+  unpack27(a, na, sa, ma);
+  unpack27(b, nb, sb, mb);
+  uint64_t num = (uint64_t)ma << 36;
+  uint64_t q = num / mb;
+  uint64_t r = num - q * mb;
+  int hb = (q >> 36) ? 36 : 35;
+  return encode_rne(na != nb, sa - sb - 36 + hb, q, hb, r != 0);
+}
+
+// floor(sqrt(x)) for x < 2^62, plus exactness
+PGA_HD uint64_t isqrt64(uint64_t x, bool& exact) {
+  uint64_t res = 0, bit = 1ull << 62;
+  while (bit > x) bit >>= 2;
+  while (bit) {
+    if (x >= res + bit) {
+      x -= res + bit;
+      res = (res >> 1) + bit;
+    } else {
+      res >>= 1;
+    }
+    bit >>= 2;
+  }
+  exact = (x == 0);
+  return res;
+}
+
+PGA_HD uint32_t sqrt(uint32_t a) {
+  if (a == 0) return 0;
+  if (a >> 31) return P_NAR;  // NaR and negative values
+  bool n_;
+  int s;
+  uint32_t m;
+  unpack27(a, n_, s, m);
+  int ex = s - 27;
+  uint64_t mm = m;
+  if (ex & 1) {
+    mm <<= 1;
+    ex -= 1;
+  }
+  // sqrt(mm * 2^32) has 30-31 bits; take it with the remainder as the sticky bit
+  bool exact;
+  uint64_t r = isqrt64(mm << 32, exact);
+  int hb = 63 - clz64(r);
+  return encode_rne(false, ex / 2 - 16 + hb, r, hb, !exact);
+}
+
+PGA_HD uint32_t from_double(double x) {
+  if (x == 0.0) return 0;
+  uint64_t bits;
+  __builtin_memcpy(&bits, &x, sizeof bits);
+  int bexp = (int)((bits >> 52) & 0x7FF);
+  bool negz = (bits >> 63) != 0;
+  if (bexp == 0x7FF) return P_NAR;  // inf, nan
+  if (bexp == 0) return negz ? 0u - 1u : 1u;  // subnormal: far below minpos
+  uint64_t m = (bits & ((1ull << 52) - 1)) | (1ull << 52);
+  return encode_rne(negz, bexp - 1023, m, 52, false);
+}
+
+PGA_HD double to_double(uint32_t a) {
+  if (a == 0) return 0.0;
+  if (a == P_NAR) return __builtin_nan("");
+  bool negz;
+  int s;
+  uint32_t m;
+  unpack27(a, negz, s, m);
+  // m * 2^(s-27), exact in double for |s| <= 120
+  uint64_t bits = ((uint64_t)(s + 1023) << 52) | (((uint64_t)m << 25) & ((1ull << 52) - 1));
+  double v;
+  __builtin_memcpy(&v, &bits, sizeof v);
+  return negz ? -v : v;
+}
+
+// ---------------------------------------------------------------------------------
+// Decoded-domain representation used by the GEMM inner loop.
+// ---------------------------------------------------------------------------------
+namespace dec {
+
+// scale of zero: far below every finite scale, so that zero loses every magnitude
+// comparison and a product with a zero factor stays far below every finite product
+constexpr int ZERO_SC = -(1 << 20);
+// scale of NaR: far above every finite scale; range checks send it to the exact path
+constexpr int NAR_SC = (1 << 24);
+// any scale below this is a zero (a zero's scale drifts downwards by design)
+constexpr int ZERO_LIMIT = -(1 << 19);
+// scales for which a result is computed here: regime at most 27 bits long
+constexpr int FAST_MIN = -108;
+constexpr int FAST_MAX = 107;
+
+struct Val {
+  uint32_t m;  // hidden bit at 30 (or exactly 2^31 after a rounding carry); 0 for zero
+  int sc;      // 4k + e
+  uint32_t sg; // 0 or 0x80000000
+};
+
+PGA_HD Val decode(uint32_t a) {
+  Val v;
+  if (a == 0) {
+    v.m = 0;
+    v.sc = ZERO_SC;
+    v.sg = 0;
+  } else if (a == P_NAR) {
+    v.m = 0x40000000u;
+    v.sc = NAR_SC;
+    v.sg = 0;
+  } else {
+    v.sg = a & 0x80000000u;
+    Parts p = parts(v.sg ? 0u - a : a);
+    v.m = p.frac31;
+    v.sc = p.scale;
+  }
+  return v;
+}
+
+// Exact pattern of a decoded value (the value must be a posit: no rounding happens).
+PGA_HD uint32_t encode(Val v) {
+  if (v.sc < ZERO_LIMIT) return 0;
+  if (v.sc >= NAR_SC / 2) return P_NAR;
+  uint32_t m = v.m;
+  int sc = v.sc;
+  if (m >> 31) {  // 2^31: a rounding carry that was not renormalised
+    m >>= 1;
+    sc += 1;
+  }
+  uint32_t ui = round_tail(sc, (uint64_t)m << 32);
+  return v.sg ? 0u - ui : ui;
+}
+
+}  // namespace dec
+}  // namespace pga

@@ -1,0 +1,326 @@
+// rgemm.hip -- posit32 GEMM kernels and their launchers (gfx950).
+//
+// Replaces the OpenCL kernels gemm_{NN,TN,NT,TT}_X of libRpack/kernel_entry.cl:598-792,
+// with the per-element rounding order of the CPU Rgemm (libRpack/Rgemm.cpp:62-251),
+// which is what every reference caller of RgemmG expects bit for bit.
+#include <hip/hip_runtime.h>
+
+#include "pga_internal.hpp"
+#include "rgemm_kernels.hpp"
+
+namespace pga {
+namespace gemm {
+
+// ---------------------------------------------------------------------------------
+// Tiled kernel
+// ---------------------------------------------------------------------------------
+template <int BM, int BN, int BK, int TM, int TN, bool TA, bool TB>
+__global__ __launch_bounds__((BM / TM) * (BN / TN)) void rgemm_tiled(
+    int m, int n, int k, uint32_t alpha, const uint32_t* __restrict__ A, int lda,
+    const uint32_t* __restrict__ B, int ldb, uint32_t beta, uint32_t* __restrict__ C, int ldc,
+    int tiles_m) {
+  constexpr int NX = BM / TM;
+  constexpr int NY = BN / TN;
+  constexpr int NT = NX * NY;
+  constexpr int NW = NT / 64;
+  constexpr bool SUM_FIRST = TA;  // TN and TT sum a*b from zero, then scale
+  constexpr int U = dec::FAST_MAX - growth_bits(BK);
+  static_assert((BM * BK) % NT == 0 && (BN * BK) % NT == 0, "tile loads");
+  static_assert(NT % 64 == 0, "whole waves");
+
+  __shared__ uint32_t sAm[BK][BM];
+  __shared__ int sAs[BK][BM];
+  __shared__ uint32_t sAg[BK][BM];
+  __shared__ uint32_t sTm[BK][BN];
+  __shared__ int sTs[BK][BN];
+  __shared__ uint32_t sTg[BK][BN];
+  __shared__ int sBound[NW][4];
+
+  const int tid = threadIdx.x;
+  const int tx = tid % NX;
+  const int ty = tid / NX;
+  const int wave = tid / 64;
+  // tiles in column-major order of the tile grid: consecutive workgroups share B slices
+  const int bid = blockIdx.x;
+  const int i0 = (bid % tiles_m) * BM;
+  const int j0 = (bid / tiles_m) * BN;
+
+  // ---- accumulators ----
+  Val acc[TM][TN];
+#pragma unroll
+  for (int r = 0; r < TM; ++r) {
+#pragma unroll
+    for (int c = 0; c < TN; ++c) {
+      int i = i0 + tx + r * NX, j = j0 + ty + c * NY;
+      uint32_t init = 0;
+      if (!SUM_FIRST && beta != 0 && i < m && j < n) {
+        uint32_t cij = C[i + (size_t)j * ldc];
+        init = (beta == P_ONE) ? cij : pga::mul(beta, cij);
+      }
+      acc[r][c] = dec::decode(init);
+    }
+  }
+
+  const bool alpha_one = (alpha == P_ONE);
+  const bool alpha_mone = (alpha == P_MONE);
+
+  for (int kt = 0; kt < k; kt += BK) {
+    // ---- stage the A slice (BM x BK) and the B-side slice (BK x BN), decoded ----
+    int amax = INT32_MIN, amin = INT32_MAX, tmax = INT32_MIN, tmin = INT32_MAX;
+#pragma unroll
+    for (int q = 0; q < (BM * BK) / NT; ++q) {
+      int e = tid + q * NT;
+      int ii, ll;
+      if (!TA) {  // A is m x k: walk rows fastest (coalesced)
+        ii = e % BM;
+        ll = e / BM;
+      } else {  // A is k x m: walk l fastest
+        ll = e % BK;
+        ii = e / BK;
+      }
+      int i = i0 + ii, l = kt + ll;
+      uint32_t v = 0;
+      if (i < m && l < k) v = TA ? A[l + (size_t)i * lda] : A[i + (size_t)l * lda];
+      Val d = dec::decode(v);
+      sAm[ll][ii] = d.m;
+      sAs[ll][ii] = d.sc;
+      sAg[ll][ii] = d.sg;
+      amax = max(amax, d.sc);
+      if (v != 0) amin = min(amin, d.sc);
+    }
+#pragma unroll
+    for (int q = 0; q < (BN * BK) / NT; ++q) {
+      int e = tid + q * NT;
+      int jj, ll;
+      if (!TB) {  // B is k x n: walk l fastest
+        ll = e % BK;
+        jj = e / BK;
+      } else {  // B is n x k: walk columns fastest
+        jj = e % BN;
+        ll = e / BN;
+      }
+      int j = j0 + jj, l = kt + ll;
+      uint32_t v = 0;
+      if (j < n && l < k) v = TB ? B[j + (size_t)l * ldb] : B[l + (size_t)j * ldb];
+      if (!SUM_FIRST) v = alpha_one ? v : (alpha_mone ? pga::neg(v) : pga::mul(alpha, v));
+      Val d = dec::decode(v);
+      sTm[ll][jj] = d.m;
+      sTs[ll][jj] = d.sc;
+      sTg[ll][jj] = d.sg;
+      tmax = max(tmax, d.sc);
+      if (v != 0) tmin = min(tmin, d.sc);
+    }
+    // wave reductions of the slice bounds
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      amax = max(amax, __shfl_xor(amax, off));
+      amin = min(amin, __shfl_xor(amin, off));
+      tmax = max(tmax, __shfl_xor(tmax, off));
+      tmin = min(tmin, __shfl_xor(tmin, off));
+    }
+    if ((tid & 63) == 0) {
+      sBound[wave][0] = amax;
+      sBound[wave][1] = amin;
+      sBound[wave][2] = tmax;
+      sBound[wave][3] = tmin;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+      amax = max(amax, sBound[w][0]);
+      amin = min(amin, sBound[w][1]);
+      tmax = max(tmax, sBound[w][2]);
+      tmin = min(tmin, sBound[w][3]);
+    }
+    // products of nonzero operands lie in [amin + tmin, amax + tmax + 1]
+    bool fast = (amax + tmax + 1 <= U);
+    if (amin != INT32_MAX && tmin != INT32_MAX) fast = fast && (amin + tmin >= dec::FAST_MIN);
+    // accumulators of this lane (zeros have scales far below every bound)
+    int cmax = INT32_MIN;
+#pragma unroll
+    for (int r = 0; r < TM; ++r)
+#pragma unroll
+      for (int c = 0; c < TN; ++c) cmax = max(cmax, acc[r][c].sc + 1);
+    fast = fast && (cmax <= U);
+    const bool wave_fast = __all(fast);
+
+    const int lmax = min(BK, k - kt);
+    bool redo = !wave_fast;
+    if (wave_fast) {
+      Val saved[TM][TN];
+#pragma unroll
+      for (int r = 0; r < TM; ++r)
+#pragma unroll
+        for (int c = 0; c < TN; ++c) saved[r][c] = acc[r][c];
+      bool low = false;
+      for (int l = 0; l < lmax; ++l) {
+        uint32_t am[TM], ag[TM], tm[TN], tg[TN];
+        int as_[TM], ts[TN];
+#pragma unroll
+        for (int r = 0; r < TM; ++r) {
+          am[r] = sAm[l][tx + r * NX];
+          as_[r] = sAs[l][tx + r * NX];
+          ag[r] = sAg[l][tx + r * NX];
+        }
+#pragma unroll
+        for (int c = 0; c < TN; ++c) {
+          tm[c] = sTm[l][ty + c * NY];
+          ts[c] = sTs[l][ty + c * NY];
+          tg[c] = sTg[l][ty + c * NY];
+        }
+#pragma unroll
+        for (int r = 0; r < TM; ++r)
+#pragma unroll
+          for (int c = 0; c < TN; ++c) low |= mac(acc[r][c], tm[c], ts[c], tg[c], am[r], as_[r], ag[r]);
+      }
+      if (__any(low)) {
+        redo = true;
+#pragma unroll
+        for (int r = 0; r < TM; ++r)
+#pragma unroll
+          for (int c = 0; c < TN; ++c) acc[r][c] = saved[r][c];
+      }
+    }
+    if (redo) {
+      // exact path for this slice: pattern-level SoftPosit operations
+#pragma unroll
+      for (int r = 0; r < TM; ++r) {
+#pragma unroll
+        for (int c = 0; c < TN; ++c) {
+          uint32_t pc = dec::encode(acc[r][c]);
+          for (int l = 0; l < lmax; ++l) {
+            Val a{sAm[l][tx + r * NX], sAs[l][tx + r * NX], sAg[l][tx + r * NX]};
+            Val t{sTm[l][ty + c * NY], sTs[l][ty + c * NY], sTg[l][ty + c * NY]};
+            pc = pga::add(pc, pga::mul(dec::encode(t), dec::encode(a)));
+          }
+          acc[r][c] = dec::decode(pc);
+        }
+      }
+    }
+    // keep zero accumulators at the canonical zero scale (sums of zeros drift downwards)
+#pragma unroll
+    for (int r = 0; r < TM; ++r)
+#pragma unroll
+      for (int c = 0; c < TN; ++c)
+        if (acc[r][c].sc < dec::ZERO_LIMIT) acc[r][c].sc = dec::ZERO_SC;
+    __syncthreads();
+  }
+
+  // ---- epilogue and store ----
+#pragma unroll
+  for (int r = 0; r < TM; ++r) {
+#pragma unroll
+    for (int c = 0; c < TN; ++c) {
+      int i = i0 + tx + r * NX, j = j0 + ty + c * NY;
+      if (i < m && j < n) {
+        uint32_t v = dec::encode(acc[r][c]);
+        if (SUM_FIRST) {
+          uint32_t at = pga::mul(alpha, v);
+          v = (beta == 0) ? at : pga::add(at, pga::mul(beta, C[i + (size_t)j * ldc]));
+        }
+        C[i + (size_t)j * ldc] = v;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// Exact kernel: one element per thread, pattern-level operations only.  Used for
+// validation of the tiled kernel and selectable at run time.
+// ---------------------------------------------------------------------------------
+template <bool TA, bool TB>
+__global__ __launch_bounds__(256) void rgemm_exact(int m, int n, int k, uint32_t alpha,
+                                                   const uint32_t* __restrict__ A, int lda,
+                                                   const uint32_t* __restrict__ B, int ldb,
+                                                   uint32_t beta, uint32_t* __restrict__ C,
+                                                   int ldc) {
+  int i = blockIdx.x * 16 + (threadIdx.x % 16);
+  int j = blockIdx.y * 16 + (threadIdx.x / 16);
+  if (i >= m || j >= n) return;
+  uint32_t* cij = C + i + (size_t)j * ldc;
+  if (!TA) {
+    uint32_t c = (beta == 0) ? 0u : (beta == P_ONE ? *cij : pga::mul(beta, *cij));
+    for (int l = 0; l < k; ++l) {
+      uint32_t b = TB ? B[j + (size_t)l * ldb] : B[l + (size_t)j * ldb];
+      c = pga::add(c, pga::mul(pga::mul(alpha, b), A[i + (size_t)l * lda]));
+    }
+    *cij = c;
+  } else {
+    uint32_t t = 0;
+    for (int l = 0; l < k; ++l) {
+      uint32_t b = TB ? B[j + (size_t)l * ldb] : B[l + (size_t)j * ldb];
+      t = pga::add(t, pga::mul(A[l + (size_t)i * lda], b));
+    }
+    uint32_t at = pga::mul(alpha, t);
+    *cij = (beta == 0) ? at : pga::add(at, pga::mul(beta, *cij));
+  }
+}
+
+// alpha == 0 (Rgemm.cpp:142-157): C = beta==0 ? 0 : beta*C
+__global__ __launch_bounds__(256) void scale_c(int m, int n, uint32_t beta, uint32_t* C, int ldc) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  int j = blockIdx.y;
+  if (i >= m) return;
+  uint32_t* p = C + i + (size_t)j * ldc;
+  *p = (beta == 0) ? 0u : pga::mul(beta, *p);
+}
+
+constexpr int BM = 64, BN = 64, BK = 16, TM = 4, TN = 4;
+constexpr int NTHREADS = (BM / TM) * (BN / TN);
+
+template <bool TA, bool TB>
+static hipError_t launch_tiled(int m, int n, int k, uint32_t alpha, const uint32_t* A, int lda,
+                               const uint32_t* B, int ldb, uint32_t beta, uint32_t* C, int ldc,
+                               hipStream_t s) {
+  int tm = (m + BM - 1) / BM, tn = (n + BN - 1) / BN;
+  long long blocks = (long long)tm * tn;
+  if (blocks > 0x7FFFFFFF) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((rgemm_tiled<BM, BN, BK, TM, TN, TA, TB>), dim3((unsigned)blocks),
+                     dim3(NTHREADS), 0, s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tm);
+  return hipGetLastError();
+}
+
+template <bool TA, bool TB>
+static hipError_t launch_exact(int m, int n, int k, uint32_t alpha, const uint32_t* A, int lda,
+                               const uint32_t* B, int ldb, uint32_t beta, uint32_t* C, int ldc,
+                               hipStream_t s) {
+  dim3 grid((m + 15) / 16, (n + 15) / 16);
+  if (grid.y > 65535) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((rgemm_exact<TA, TB>), grid, dim3(256), 0, s, m, n, k, alpha, A, lda, B, ldb,
+                     beta, C, ldc);
+  return hipGetLastError();
+}
+
+}  // namespace gemm
+
+// Device-pointer GEMM with Rgemm's quick returns; the caller has validated arguments.
+hipError_t rgemm_launch(bool ta, bool tb, int m, int n, int k, uint32_t alpha, const uint32_t* A,
+                        int lda, const uint32_t* B, int ldb, uint32_t beta, uint32_t* C, int ldc,
+                        hipStream_t s, GemmImpl impl) {
+  using namespace gemm;
+  if (m == 0 || n == 0 || ((alpha == 0 || k == 0) && beta == P_ONE)) return hipSuccess;
+  if (alpha == 0) {
+    if (n > 65535) {
+      for (int j0 = 0; j0 < n; j0 += 65535) {
+        int nn = (n - j0 < 65535) ? n - j0 : 65535;
+        hipLaunchKernelGGL(scale_c, dim3((m + 255) / 256, nn), dim3(256), 0, s, m, nn, beta,
+                           C + (size_t)j0 * ldc, ldc);
+      }
+    } else {
+      hipLaunchKernelGGL(scale_c, dim3((m + 255) / 256, n), dim3(256), 0, s, m, n, beta, C, ldc);
+    }
+    return hipGetLastError();
+  }
+  if (impl == GemmImpl::Exact) {
+    if (!ta && !tb) return launch_exact<false, false>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s);
+    if (!ta && tb) return launch_exact<false, true>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s);
+    if (ta && !tb) return launch_exact<true, false>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s);
+    return launch_exact<true, true>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s);
+  }
+  if (!ta && !tb) return launch_tiled<false, false>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s);
+  if (!ta && tb) return launch_tiled<false, true>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s);
+  if (ta && !tb) return launch_tiled<true, false>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s);
+  return launch_tiled<true, true>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s);
+}
+
+}  // namespace pga

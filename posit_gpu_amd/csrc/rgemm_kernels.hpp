@@ -1,0 +1,96 @@
+// rgemm_kernels.hpp -- posit32 GEMM kernels for gfx950 (integer VALU, no MFMA).
+//
+// Semantics (per C element, matching libRpack/Rgemm.cpp:62-251):
+//   NN / NT (Rgemm.cpp:163-185, 206-227):
+//     c = beta==0 ? 0 : (beta==1 ? C : beta*C);  for l ascending: c = c + (alpha*b_lj)*a_il
+//   TN / TT (Rgemm.cpp:186-204, 228-246):
+//     t = 0;  for l ascending: t = t + a_li*b_lj;  C = beta==0 ? alpha*t : alpha*t + beta*C
+// with each posit operation rounded.  The K loop of every element runs in ascending
+// order in one thread (no split-K), so any tiling gives the reference's bits.
+//
+// Tiling: a workgroup owns a BM x BN tile of C; each lane owns TM x TN elements
+// (rows tx + r*BM/TM, columns ty + c*BN/TN).  For each BK-slice of K the A-slice and
+// the B-side slice (alpha*B for NN/NT, B for TN/TT) are decoded once into LDS in the
+// (mantissa, scale, sign) form, then every lane runs TM*TN independent multiply-add
+// chains out of LDS.  The decoded multiply-add is exact for results whose regime is at
+// most 27 bits long; a slice whose operands or accumulators could leave that range,
+// and a wave in which some sum did leave it, is computed with the pattern-level
+// SoftPosit operations instead.
+#pragma once
+
+#include "posit32_core.hpp"
+
+namespace pga {
+namespace gemm {
+
+using dec::Val;
+
+// Rounds y (hidden bit at 30; bits >= 2 exact, bits 1..0 sticky) to the precision of
+// scale `sc`: 28 - reg fraction bits, round to nearest, ties to even.  Valid for
+// FAST_MIN <= sc <= FAST_MAX; for other scales the result is unspecified (but
+// computed without undefined behaviour).  May return exactly 2^31 on a carry.
+PGA_D uint32_t round_to_scale(uint32_t y, int sc) {
+  int k = sc >> 2;
+  uint32_t sh = (uint32_t)((k ^ (k >> 31)) + 3) & 31u;  // dropped bits: reg + 2
+  uint32_t mask = (1u << sh) - 1u;
+  uint32_t lsb = (y >> sh) & 1u;
+  return (y + (mask >> 1) + lsb) & ~mask;
+}
+
+// c <- round(c + round(t * a)) in the decoded domain.  Returns true when the sum fell
+// below the fast range (a nonzero result with scale < FAST_MIN); the caller must then
+// recompute.  Preconditions (guaranteed by the slice checks): t and a are finite,
+// every nonzero product has scale in [FAST_MIN, FAST_MAX - 1] and the accumulator
+// stays at or below FAST_MAX.
+PGA_D bool mac(Val& c, uint32_t tm, int ts, uint32_t tg, uint32_t am, int as, uint32_t ag) {
+  // ---- product t * a ----
+  uint32_t hi = __umulhi(tm, am);  // mantissas in [2^30, 2^31): product in [2^60, 2^62)
+  uint32_t lo = tm * am;
+  uint32_t cy = hi >> 29;
+  int ps = ts + as + (int)cy;
+  uint32_t pm = (hi << (2u - cy)) | (lo != 0u ? 1u : 0u);  // hidden at 30, sticky in bit 0
+  uint32_t pg = tg ^ ag;
+  pm = round_to_scale(pm, ps);
+  uint32_t ov = pm >> 31;  // renormalise a rounding carry
+  pm >>= ov;
+  ps += (int)ov;
+
+  // ---- sum c + p ----
+  // order by magnitude with one 64-bit compare of (scale, mantissa); scales are offset
+  // so that every scale that can occur (zeros included) is non-negative
+  uint64_t kp = ((uint64_t)(uint32_t)(ps + (1 << 28)) << 32) | pm;
+  uint64_t kc = ((uint64_t)(uint32_t)(c.sc + (1 << 28)) << 32) | c.m;
+  bool pbig = kp > kc;
+  uint32_t mb = pbig ? pm : c.m;
+  uint32_t ms = pbig ? c.m : pm;
+  int sb = pbig ? ps : c.sc;
+  int ss = pbig ? c.sc : ps;
+  uint32_t gb = pbig ? pg : c.sg;
+  uint32_t d = (uint32_t)(sb - ss);
+  d = d < 31u ? d : 31u;
+  uint32_t lost = ms & ((1u << d) - 1u);
+  uint32_t al = (ms >> d) | (lost != 0u ? 1u : 0u);
+  uint32_t x = ((pg ^ c.sg) != 0u) ? mb - al : mb + al;
+  // leading zeros; a zero sum gets a huge count so its scale lands far below zero
+  uint32_t lz = x ? (uint32_t)__builtin_clz(x) : (1u << 21);
+  int sn = sb + 1 - (int)lz;
+  uint32_t z = x << (lz & 31u);
+  uint32_t y = (z >> 1) | (z & 1u);
+  c.m = round_to_scale(y, sn);
+  c.sc = sn;
+  c.sg = gb;
+  // a nonzero result below the fast range (zeros are below ZERO_LIMIT)
+  return (uint32_t)(sn - dec::ZERO_LIMIT) < (uint32_t)(dec::FAST_MIN - dec::ZERO_LIMIT);
+}
+
+// Largest accumulator / product scale for which BK more steps cannot pass FAST_MAX:
+// each step at most adds a product and rounds up by a factor <= 1.25 (a result in the
+// fast range keeps at least one fraction bit), so after BK steps
+// |c| < 1.25^BK * (BK + 1) * 2^(U+1).
+constexpr int growth_bits(int bk) {
+  // ceil(log2(1.25^bk * (bk + 1))) + 1, evaluated for the slice depths used here
+  return bk <= 8 ? 7 : (bk <= 16 ? 11 : 18);
+}
+
+}  // namespace gemm
+}  // namespace pga

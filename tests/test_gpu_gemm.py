@@ -1,0 +1,188 @@
+"""GPU parity of the posit32 GEMM (through the C ABI) against the oracle's restatement of
+the reference CPU Rgemm (libRpack/Rgemm.cpp:62-251).  Bit-exact: every output pattern
+must match."""
+import os
+
+import numpy as np
+import pytest
+
+from datagen import posit_matrix, random_patterns
+
+pytestmark = pytest.mark.gpu
+
+ONE, MONE, NAR = 0x40000000, 0xC0000000, 0x80000000
+THREADS = min(16, os.cpu_count() or 1)
+
+
+def _both(oracle, pga, ta, tb, alpha, beta, A, B, C, impl="tiled"):
+    m, n = C.shape
+    k = A.shape[0] if ta == "T" else A.shape[1]
+    want = C.copy(order="F")
+    assert oracle.rgemm(ta, tb, m, n, k, alpha, A, B, beta, want, nthreads=THREADS) == 0
+    pga.set_gemm_impl(impl)
+    try:
+        got = pga.rgemm(ta, tb, alpha, A, B, beta, C)
+    finally:
+        pga.set_gemm_impl("tiled")
+    return got, want
+
+
+def _assert_same(got, want):
+    bad = np.argwhere(got != want)
+    assert bad.size == 0, f"{len(bad)} mismatches, first {[(tuple(i), hex(got[tuple(i)]), hex(want[tuple(i)])) for i in bad[:4]]}"
+
+
+def test_config1_512_nn(pga, oracle):
+    """Plain Rgemm C = A*B, M=N=K=512, alpha=1, beta=0, uniform[-1,1) inputs."""
+    n = 512
+    A = posit_matrix(oracle, 101, n, n)
+    B = posit_matrix(oracle, 102, n, n)
+    C = posit_matrix(oracle, 103, n, n)
+    got, want = _both(oracle, pga, "N", "N", ONE, 0, A, B, C)
+    _assert_same(got, want)
+
+
+@pytest.mark.parametrize("impl", ["tiled", "exact"])
+@pytest.mark.parametrize("ta", ["N", "T"])
+@pytest.mark.parametrize("tb", ["N", "T"])
+@pytest.mark.parametrize("alpha,beta", [(ONE, 0), (MONE, ONE), (0x3A000000, 0xC8000000), (ONE, ONE), (0, 0x44000000)])
+def test_variants_ragged(pga, oracle, impl, ta, tb, alpha, beta):
+    m, n, k = 77, 45, 133
+    A = posit_matrix(oracle, 1, k if ta == "T" else m, m if ta == "T" else k)
+    B = posit_matrix(oracle, 2, n if tb == "T" else k, k if tb == "T" else n)
+    C = posit_matrix(oracle, 3, m, n)
+    got, want = _both(oracle, pga, ta, tb, alpha, beta, A, B, C, impl)
+    _assert_same(got, want)
+
+
+@pytest.mark.parametrize("m,n,k", [(1, 1, 1), (31, 200, 17), (200, 31, 16), (64, 64, 0), (64, 64, 1), (65, 129, 257),
+                                   (3, 300, 5), (130, 2, 64)])
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("N", "T"), ("T", "N")])
+def test_shapes(pga, oracle, m, n, k, ta, tb):
+    # operands stored with at least one row/column so that k == 0 still has valid buffers
+    A = posit_matrix(oracle, 11, max(1, k if ta == "T" else m), max(1, m if ta == "T" else k))
+    B = posit_matrix(oracle, 12, max(1, n if tb == "T" else k), max(1, k if tb == "T" else n))
+    C = posit_matrix(oracle, 13, m, n)
+    for alpha, beta in [(MONE, ONE), (0x3C000000, 0x38000000)]:
+        want = C.copy(order="F")
+        got = C.copy(order="F")
+        assert oracle.rgemm(ta, tb, m, n, k, alpha, A, B, beta, want, nthreads=THREADS) == 0
+        pga.RgemmG(ta, tb, m, n, k, alpha, A, max(1, A.shape[0]), B, max(1, B.shape[0]), beta, got, max(1, m))
+        _assert_same(got, want)
+
+
+@pytest.mark.parametrize("kind", ["patterns", "long_regimes", "sparse", "nar", "tiny", "huge", "cancel"])
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("T", "N"), ("N", "T")])
+def test_value_classes(pga, oracle, kind, ta, tb):
+    """Inputs that push the decoded fast path to its limits or out of it."""
+    rng = np.random.default_rng(hash(kind) % 1000 + ord(ta) + 3 * ord(tb))
+    m, n, k = 70, 66, 96
+
+    def mat(r, c, seed):
+        if kind == "patterns":
+            v = random_patterns(rng, r * c)
+        elif kind == "long_regimes":
+            v = random_patterns(rng, r * c, True)
+        elif kind == "sparse":
+            v = posit_matrix(oracle, seed, r, c).ravel(order="F").copy()
+            v[rng.random(v.size) < 0.7] = 0
+        elif kind == "nar":
+            v = posit_matrix(oracle, seed, r, c).ravel(order="F").copy()
+            v[rng.integers(0, v.size, 3)] = NAR
+        elif kind == "tiny":  # products near 2^-100, sums can cancel below 2^-108
+            v = oracle.from_double_v(rng.uniform(-1, 1, r * c) * 2.0 ** -50)
+        elif kind == "huge":  # products near 2^100: accumulators leave the fast range
+            v = oracle.from_double_v(rng.uniform(-1, 1, r * c) * 2.0 ** 52)
+        else:  # cancel: few distinct values so that many sums cancel exactly
+            v = oracle.from_double_v(rng.choice([-1.0, -0.5, 0.5, 1.0, 0.25], r * c))
+        return np.asfortranarray(np.asarray(v, np.uint32).reshape(c, r).T)
+
+    A = mat(k if ta == "T" else m, m if ta == "T" else k, 1)
+    B = mat(n if tb == "T" else k, k if tb == "T" else n, 2)
+    C = mat(m, n, 3)
+    for alpha, beta in [(ONE, 0), (MONE, ONE)]:
+        got, want = _both(oracle, pga, ta, tb, alpha, beta, A, B, C)
+        _assert_same(got, want)
+
+
+def test_tiled_equals_exact_kernel(pga, oracle):
+    rng = np.random.default_rng(77)
+    m, n, k = 300, 260, 200
+    A = posit_matrix(oracle, 31, m, k, -4, 4)
+    B = posit_matrix(oracle, 32, k, n, -4, 4)
+    C = posit_matrix(oracle, 33, m, n, -4, 4)
+    pga.set_gemm_impl("exact")
+    e = pga.rgemm("N", "N", MONE, A, B, ONE, C)
+    pga.set_gemm_impl("tiled")
+    t = pga.rgemm("N", "N", MONE, A, B, ONE, C)
+    _assert_same(t, e)
+    del rng
+
+
+def test_device_submatrix_update(pga, oracle):
+    """Trailing update on a sub-block of a resident matrix: C22 -= A21 * A12, ld = n."""
+    n, j, jb = 700, 128, 64
+    M = posit_matrix(oracle, 41, n, n)
+    d = pga.DeviceArray.from_host(M)
+    r0 = j + jb
+    mm = nn = n - r0
+    pga.rgemm_device("N", "N", mm, nn, jb, MONE, d.offset(r0 + j * n), n, d.offset(j + r0 * n), n, ONE,
+                     d.offset(r0 + r0 * n), n)
+    pga.synchronize()
+    got = d.to_host()
+    want = M.copy(order="F")
+    sub = np.asfortranarray(want[r0:, r0:])
+    oracle.rgemm("N", "N", mm, nn, jb, MONE, np.asfortranarray(want[r0:, j:j + jb]),
+                 np.asfortranarray(want[j:j + jb, r0:]), ONE, sub, nthreads=THREADS)
+    want[r0:, r0:] = sub
+    _assert_same(got, want)
+
+
+def test_illegal_arguments(pga):
+    A = np.zeros((4, 4), np.uint32, order="F")
+    with pytest.raises(pga.PositError) as e:
+        pga.RgemmG("X", "N", 4, 4, 4, ONE, A, 4, A, 4, 0, A.copy(order="F"), 4)
+    assert e.value.argument == 1
+    with pytest.raises(pga.PositError) as e:
+        pga.RgemmG("N", "N", 4, 4, 4, ONE, A, 3, A, 4, 0, A.copy(order="F"), 4)
+    assert e.value.argument == 8
+
+
+def test_trailing_shape_4096_full(pga, oracle):
+    """Config-2 shape M=N=4096, K=256, alpha=-1, beta=1, device-resident, whole result."""
+    M = N = 4096
+    K = 256
+    dA = pga.DeviceArray((M, K))
+    dB = pga.DeviceArray((K, N))
+    dC = pga.DeviceArray((M, N))
+    pga.fill_uniform(dA, 201)
+    pga.fill_uniform(dB, 202)
+    pga.fill_uniform(dC, 203)
+    A, B, C = dA.to_host(), dB.to_host(), dC.to_host()
+    pga.rgemm_device("N", "N", M, N, K, MONE, dA, M, dB, K, ONE, dC, M)
+    pga.synchronize()
+    got = dC.to_host()
+    oracle.rgemm("N", "N", M, N, K, MONE, A, B, ONE, C, nthreads=THREADS)
+    _assert_same(got, C)
+
+
+def test_trailing_shape_16384_sampled_columns(pga, oracle):
+    """Config-2 shape M=N=16384, K=256: full-size launch, every row of 48 random columns
+    checked against the oracle (each column of C depends only on its own column of B)."""
+    M = N = 16384
+    K = 256
+    dA = pga.DeviceArray((M, K))
+    dB = pga.DeviceArray((K, N))
+    dC = pga.DeviceArray((M, N))
+    pga.fill_uniform(dA, 301)
+    pga.fill_uniform(dB, 302)
+    pga.fill_uniform(dC, 303)
+    A, B, C0 = dA.to_host(), dB.to_host(), dC.to_host()
+    pga.rgemm_device("N", "N", M, N, K, MONE, dA, M, dB, K, ONE, dC, M)
+    pga.synchronize()
+    got = dC.to_host()
+    cols = np.sort(np.random.default_rng(9).choice(N, 48, replace=False))
+    Bs = np.asfortranarray(B[:, cols])
+    Cs = np.asfortranarray(C0[:, cols])
+    oracle.rgemm("N", "N", M, cols.size, K, MONE, A, Bs, ONE, Cs, nthreads=THREADS)
+    _assert_same(np.asfortranarray(got[:, cols]), Cs)
