@@ -76,7 +76,12 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--only-shape", type=int, default=0,
+                    help="run only the M=N=<value> GEMM (used for per-dispatch profiling)")
     args = ap.parse_args()
+    global SHAPES
+    if args.only_shape:
+        SHAPES = [args.only_shape]
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the posit32 GEMM path is "
               "benchmarked on a single MI355X.", file=sys.stderr)

@@ -92,6 +92,69 @@ pga_status pga_rgemm_device(char transa, char transb, int m, int n, int k, uint3
                             const uint32_t *a, int lda, const uint32_t *b, int ldb,
                             uint32_t beta, uint32_t *c, int ldc, void *stream);
 
+/* ---- BLAS-2/3 (libRpack/mpblas_posit.h:37-45) -------------------------------------
+ * Host-pointer forms are synchronous drop-ins for the CPU routines of the reference; the
+ * *_device forms take device pointers and run asynchronously on `stream`.  Argument
+ * checks and Mxerbla parameter numbers are those of the reference routine. */
+/* Rtrsm (libRpack/Rtrsm.cpp:31-291): all side/uplo/transa/diag combinations */
+pga_status pga_rtrsm(char side, char uplo, char transa, char diag, int m, int n, uint32_t alpha,
+                     const uint32_t *a, int lda, uint32_t *b, int ldb);
+pga_status pga_rtrsm_device(char side, char uplo, char transa, char diag, int m, int n,
+                            uint32_t alpha, const uint32_t *a, int lda, uint32_t *b, int ldb,
+                            void *stream);
+/* Rsyrk (libRpack/Rsyrk.cpp:31-215) */
+pga_status pga_rsyrk(char uplo, char trans, int n, int k, uint32_t alpha, const uint32_t *a,
+                     int lda, uint32_t beta, uint32_t *c, int ldc);
+pga_status pga_rsyrk_device(char uplo, char trans, int n, int k, uint32_t alpha, const uint32_t *a,
+                            int lda, uint32_t beta, uint32_t *c, int ldc, void *stream);
+/* Rgemv (libRpack/Rgemv.cpp:31-204) */
+pga_status pga_rgemv(char trans, int m, int n, uint32_t alpha, const uint32_t *a, int lda,
+                     const uint32_t *x, int incx, uint32_t beta, uint32_t *y, int incy);
+pga_status pga_rgemv_device(char trans, int m, int n, uint32_t alpha, const uint32_t *a, int lda,
+                            const uint32_t *x, int incx, uint32_t beta, uint32_t *y, int incy,
+                            void *stream);
+/* Rscal (libRpack/Rscal.cpp:62-119) */
+pga_status pga_rscal(int n, uint32_t da, uint32_t *x, int incx);
+pga_status pga_rscal_device(int n, uint32_t da, uint32_t *x, int incx, void *stream);
+/* iRamax (libRpack/iRamax.cpp:31-88): *result = 1-based index (0 if n < 1 or incx <= 0);
+ * the device form writes it to device memory */
+pga_status pga_iramax(int n, const uint32_t *x, int incx, int *result);
+pga_status pga_iramax_device(int n, const uint32_t *x, int incx, int *result, void *stream);
+/* Rlaswp (libRpack/Rlaswp.cpp:32-111); ipiv is 1-based */
+pga_status pga_rlaswp(int n, uint32_t *a, int lda, int k1, int k2, const int *ipiv, int incx);
+pga_status pga_rlaswp_device(int n, uint32_t *a, int lda, int k1, int k2, const int *ipiv,
+                             int incx, void *stream);
+
+/* ---- LAPACK (libRpack/mplapack_posit.h:38-57) --------------------------------------
+ * ipiv 1-based, info as in LAPACK: 0 ok, >0 first zero pivot (LU, factorization
+ * continues) or order of the first non-positive leading minor (Cholesky, stops).
+ * nb <= 0 selects iMlaenv's 64.  Device forms keep ipiv and info in device memory. */
+/* Rgetrf_opencl (libRpack/Rgetrf_opencl.cpp:32-154, block size `nbb`) / Rgetrf */
+pga_status pga_rgetrf(int m, int n, uint32_t *a, int lda, int *ipiv, int *info, int nb);
+pga_status pga_rgetrf_device(int m, int n, uint32_t *a, int lda, int *ipiv, int *info, int nb,
+                             void *stream);
+/* Rgetrf2 / Rgetrf2_opencl (libRpack/Rgetrf2.cpp:32-185): recursive, unblocked */
+pga_status pga_rgetrf2(int m, int n, uint32_t *a, int lda, int *ipiv, int *info);
+pga_status pga_rgetrf2_device(int m, int n, uint32_t *a, int lda, int *ipiv, int *info,
+                              void *stream);
+/* Rgetrs (libRpack/Rgetrs.cpp:32-118) */
+pga_status pga_rgetrs(char trans, int n, int nrhs, const uint32_t *a, int lda, const int *ipiv,
+                      uint32_t *b, int ldb, int *info);
+pga_status pga_rgetrs_device(char trans, int n, int nrhs, const uint32_t *a, int lda,
+                             const int *ipiv, uint32_t *b, int ldb, void *stream);
+/* Rpotrf_opencl / Rpotrf (libRpack/Rpotrf_opencl.cpp:32-166) */
+pga_status pga_rpotrf(char uplo, int n, uint32_t *a, int lda, int *info, int nb);
+pga_status pga_rpotrf_device(char uplo, int n, uint32_t *a, int lda, int *info, int nb,
+                             void *stream);
+/* Rpotrf2 (libRpack/Rpotrf2.cpp:32-151): recursive, unblocked */
+pga_status pga_rpotrf2(char uplo, int n, uint32_t *a, int lda, int *info);
+pga_status pga_rpotrf2_device(char uplo, int n, uint32_t *a, int lda, int *info, void *stream);
+/* Rpotrs (libRpack/Rpotrs.cpp:32-110) */
+pga_status pga_rpotrs(char uplo, int n, int nrhs, const uint32_t *a, int lda, uint32_t *b, int ldb,
+                      int *info);
+pga_status pga_rpotrs_device(char uplo, int n, int nrhs, const uint32_t *a, int lda, uint32_t *b,
+                             int ldb, void *stream);
+
 /* ---- conversions (device codecs) -------------------------------------------------
  * double <-> posit32 on device arrays: round to nearest, ties to the even pattern,
  * saturating at minpos/maxpos; NaN and infinities become NaR.  (The reference converts
@@ -105,10 +168,30 @@ pga_status pga_p32_to_f64_device(const uint32_t *p, double *x, size_t n, void *s
 pga_status pga_fill_uniform_p32_device(uint32_t *p, size_t n, uint64_t seed, double lo, double hi,
                                        void *stream);
 
+/* FP64 backward error of a solution x of A x = b (device arrays, n x n, one right-hand
+ * side): ||A x - b||_inf / (||A||_inf ||x||_inf) with A, x, b decoded exactly and FP64
+ * sums; synchronous, result on the host.  Reported beside LU / Cholesky results as the
+ * reference's check programs report residuals (main_lu_check.cpp:234-257). */
+pga_status pga_backward_error_device(int n, const uint32_t *a, int lda, const uint32_t *x,
+                                     const uint32_t *b, double *result, void *stream);
+
 /* ---- elementwise scalar arithmetic on device arrays (validation of the core) ---- */
 /* op: 0 add, 1 sub, 2 mul, 3 div, 4 sqrt (b ignored) */
 pga_status pga_p32_elementwise_device(int op, const uint32_t *a, const uint32_t *b,
                                       uint32_t *out, size_t n, void *stream);
+
+/* ---- host scalar arithmetic for the posit32 value type ------------------------
+ * Scalar operations for the drop-in posit32 type (include/softposit_cpp.h), which the
+ * reference takes from the SoftPosit library (operators of posit32, e.g. used at
+ * Rgetrf2.cpp:122 and main_gemm.cpp:99-104).  Same bits as the device arithmetic.
+ * Matrix routines never use these: they run on the GPU. */
+uint32_t pga_p32_add(uint32_t a, uint32_t b);
+uint32_t pga_p32_sub(uint32_t a, uint32_t b);
+uint32_t pga_p32_mul(uint32_t a, uint32_t b);
+uint32_t pga_p32_div(uint32_t a, uint32_t b);
+uint32_t pga_p32_sqrt(uint32_t a);
+uint32_t pga_p32_from_double(double x);
+double pga_p32_to_double(uint32_t a);
 
 #ifdef __cplusplus
 }

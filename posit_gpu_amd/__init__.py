@@ -306,3 +306,166 @@ def rgemm(transa, transb, alpha, A, B, beta, C):
     k = A.shape[0] if transa.upper() in "TC" else A.shape[1]
     RgemmG(transa, transb, m, n, k, alpha, A, max(1, A.shape[0]), B, max(1, B.shape[0]), beta, C, max(1, m))
     return C
+
+
+# ---------------------------------------------------------------------------------
+# BLAS-2/3 and LAPACK (host-pointer drop-ins and device forms)
+# ---------------------------------------------------------------------------------
+_IP = ctypes.POINTER(ctypes.c_int)
+_sig("pga_rtrsm", [_C, _C, _C, _C, _I, _I, _U32, _P, _I, _P, _I])
+_sig("pga_rtrsm_device", [_C, _C, _C, _C, _I, _I, _U32, _P, _I, _P, _I, _P])
+_sig("pga_rsyrk", [_C, _C, _I, _I, _U32, _P, _I, _U32, _P, _I])
+_sig("pga_rsyrk_device", [_C, _C, _I, _I, _U32, _P, _I, _U32, _P, _I, _P])
+_sig("pga_rgemv", [_C, _I, _I, _U32, _P, _I, _P, _I, _U32, _P, _I])
+_sig("pga_rgemv_device", [_C, _I, _I, _U32, _P, _I, _P, _I, _U32, _P, _I, _P])
+_sig("pga_rscal", [_I, _U32, _P, _I])
+_sig("pga_rscal_device", [_I, _U32, _P, _I, _P])
+_sig("pga_iramax", [_I, _P, _I, _IP])
+_sig("pga_iramax_device", [_I, _P, _I, _P, _P])
+_sig("pga_rlaswp", [_I, _P, _I, _I, _I, _P, _I])
+_sig("pga_rlaswp_device", [_I, _P, _I, _I, _I, _P, _I, _P])
+_sig("pga_rgetrf", [_I, _I, _P, _I, _P, _IP, _I])
+_sig("pga_rgetrf_device", [_I, _I, _P, _I, _P, _P, _I, _P])
+_sig("pga_rgetrf2", [_I, _I, _P, _I, _P, _IP])
+_sig("pga_rgetrf2_device", [_I, _I, _P, _I, _P, _P, _P])
+_sig("pga_rgetrs", [_C, _I, _I, _P, _I, _P, _P, _I, _IP])
+_sig("pga_rgetrs_device", [_C, _I, _I, _P, _I, _P, _P, _I, _P])
+_sig("pga_rpotrf", [_C, _I, _P, _I, _IP, _I])
+_sig("pga_rpotrf_device", [_C, _I, _P, _I, _P, _I, _P])
+_sig("pga_rpotrf2", [_C, _I, _P, _I, _IP])
+_sig("pga_rpotrf2_device", [_C, _I, _P, _I, _P, _P])
+_sig("pga_rpotrs", [_C, _I, _I, _P, _I, _P, _I, _IP])
+_sig("pga_rpotrs_device", [_C, _I, _I, _P, _I, _P, _I, _P])
+
+
+def _buf(x, dtype=np.uint32):
+    assert x.dtype == dtype, x.dtype
+    assert x.flags.f_contiguous or x.flags.c_contiguous
+    return x.ctypes.data
+
+
+def _ld(x):
+    return max(1, x.shape[0]) if x.ndim > 1 else max(1, x.size)
+
+
+def Rtrsm(side, uplo, transa, diag, m, n, alpha, a, lda, b, ldb):  # noqa: N802
+    """libRpack/mpblas_posit.h:38 (Rtrsm.cpp:31-291), on the GPU; b updated in place."""
+    _check(_lib.pga_rtrsm(_ch(side), _ch(uplo), _ch(transa), _ch(diag), m, n, alpha, _buf(a), lda, _buf(b), ldb),
+           "pga_rtrsm")
+
+
+def Rsyrk(uplo, trans, n, k, alpha, a, lda, beta, c, ldc):  # noqa: N802
+    """libRpack/mpblas_posit.h:39 (Rsyrk.cpp:31-215)."""
+    _check(_lib.pga_rsyrk(_ch(uplo), _ch(trans), n, k, alpha, _buf(a), lda, beta, _buf(c), ldc), "pga_rsyrk")
+
+
+def Rgemv(trans, m, n, alpha, a, lda, x, incx, beta, y, incy):  # noqa: N802
+    """libRpack/mpblas_posit.h:40 (Rgemv.cpp:31-204)."""
+    _check(_lib.pga_rgemv(_ch(trans), m, n, alpha, _buf(a), lda, _buf(x), incx, beta, _buf(y), incy), "pga_rgemv")
+
+
+def Rscal(n, da, dx, incx):  # noqa: N802
+    """libRpack/mpblas_posit.h:43 (Rscal.cpp:62-119)."""
+    _check(_lib.pga_rscal(n, da, _buf(dx), incx), "pga_rscal")
+
+
+def iRamax(n, dx, incx):  # noqa: N802
+    """libRpack/mpblas_posit.h:44 (iRamax.cpp:31-88): 1-based index."""
+    r = ctypes.c_int()
+    _check(_lib.pga_iramax(n, _buf(dx), incx, ctypes.byref(r)), "pga_iramax")
+    return r.value
+
+
+def Rlaswp(n, a, lda, k1, k2, ipiv, incx):  # noqa: N802
+    """libRpack/mplapack_posit.h:54 (Rlaswp.cpp:32-111); ipiv int32, 1-based."""
+    _check(_lib.pga_rlaswp(n, _buf(a), lda, k1, k2, _buf(ipiv, np.int32), incx), "pga_rlaswp")
+
+
+def Rgetrf_opencl(m, n, a, lda, ipiv, nbb=0):  # noqa: N802
+    """libRpack/mplapack_posit.h:49 (Rgetrf_opencl.cpp:32-154); returns info."""
+    info = ctypes.c_int()
+    _check(_lib.pga_rgetrf(m, n, _buf(a), lda, _buf(ipiv, np.int32), ctypes.byref(info), nbb), "pga_rgetrf")
+    return info.value
+
+
+def Rgetrf(m, n, a, lda, ipiv):  # noqa: N802
+    """libRpack/mplapack_posit.h:48 (Rgetrf.cpp): nb from iMlaenv (64)."""
+    return Rgetrf_opencl(m, n, a, lda, ipiv, 0)
+
+
+def Rgetrf2(m, n, a, lda, ipiv):  # noqa: N802
+    """libRpack/mplapack_posit.h:51 (Rgetrf2.cpp:32-185)."""
+    info = ctypes.c_int()
+    _check(_lib.pga_rgetrf2(m, n, _buf(a), lda, _buf(ipiv, np.int32), ctypes.byref(info)), "pga_rgetrf2")
+    return info.value
+
+
+Rgetrf2_opencl = Rgetrf2
+
+
+def Rgetrs(trans, n, nrhs, a, lda, ipiv, b, ldb):  # noqa: N802
+    """libRpack/mplapack_posit.h:46 (Rgetrs.cpp:32-118); returns info."""
+    info = ctypes.c_int()
+    _check(_lib.pga_rgetrs(_ch(trans), n, nrhs, _buf(a), lda, _buf(ipiv, np.int32), _buf(b), ldb, ctypes.byref(info)),
+           "pga_rgetrs")
+    return info.value
+
+
+def Rpotrf_opencl(uplo, n, a, lda, nb=0):  # noqa: N802
+    """libRpack/mplapack_posit.h:43 (Rpotrf_opencl.cpp:32-166); returns info."""
+    info = ctypes.c_int()
+    _check(_lib.pga_rpotrf(_ch(uplo), n, _buf(a), lda, ctypes.byref(info), nb), "pga_rpotrf")
+    return info.value
+
+
+def Rpotrf(uplo, n, a, lda):  # noqa: N802
+    return Rpotrf_opencl(uplo, n, a, lda, 0)
+
+
+def Rpotrf2(uplo, n, a, lda):  # noqa: N802
+    info = ctypes.c_int()
+    _check(_lib.pga_rpotrf2(_ch(uplo), n, _buf(a), lda, ctypes.byref(info)), "pga_rpotrf2")
+    return info.value
+
+
+def Rpotrs(uplo, n, nrhs, a, lda, b, ldb):  # noqa: N802
+    info = ctypes.c_int()
+    _check(_lib.pga_rpotrs(_ch(uplo), n, nrhs, _buf(a), lda, _buf(b), ldb, ctypes.byref(info)), "pga_rpotrs")
+    return info.value
+
+
+def _dp(x):
+    return x.ptr if isinstance(x, DeviceArray) else x
+
+
+def getrf_device(m, n, a, lda, ipiv, info, nb=0, stream=None):
+    _check(_lib.pga_rgetrf_device(m, n, _dp(a), lda, _dp(ipiv), _dp(info), nb, stream), "pga_rgetrf_device")
+
+
+def getrs_device(trans, n, nrhs, a, lda, ipiv, b, ldb, stream=None):
+    _check(_lib.pga_rgetrs_device(_ch(trans), n, nrhs, _dp(a), lda, _dp(ipiv), _dp(b), ldb, stream),
+           "pga_rgetrs_device")
+
+
+def potrf_device(uplo, n, a, lda, info, nb=0, stream=None):
+    _check(_lib.pga_rpotrf_device(_ch(uplo), n, _dp(a), lda, _dp(info), nb, stream), "pga_rpotrf_device")
+
+
+def potrs_device(uplo, n, nrhs, a, lda, b, ldb, stream=None):
+    _check(_lib.pga_rpotrs_device(_ch(uplo), n, nrhs, _dp(a), lda, _dp(b), ldb, stream), "pga_rpotrs_device")
+
+
+def gemv_device(trans, m, n, alpha, a, lda, x, incx, beta, y, incy, stream=None):
+    _check(_lib.pga_rgemv_device(_ch(trans), m, n, alpha, _dp(a), lda, _dp(x), incx, beta, _dp(y), incy, stream),
+           "pga_rgemv_device")
+
+
+_sig("pga_backward_error_device", [_I, _P, _I, _P, _P, ctypes.POINTER(ctypes.c_double), _P])
+
+
+def backward_error_device(n, a, lda, x, b, stream=None):
+    """||A x - b||_inf / (||A||_inf ||x||_inf) in FP64 on the GPU (posit32 device arrays)."""
+    r = ctypes.c_double()
+    _check(_lib.pga_backward_error_device(n, _dp(a), lda, _dp(x), _dp(b), ctypes.byref(r), stream),
+           "pga_backward_error_device")
+    return r.value

@@ -1,0 +1,605 @@
+// lapack.hip -- posit32 BLAS-2/3 kernels and LU / Cholesky drivers on the GPU.
+//
+// Every routine keeps the per-element operation order of the reference routine cited
+// beside it, so results are bit-identical; all arithmetic runs on the device.  The
+// factorization drivers follow the reference's blocking and recursion exactly
+// (Rgetrf_opencl.cpp, Rgetrf2.cpp, Rpotrf_opencl.cpp, Rpotrf2.cpp), which also keeps
+// the reference's zero-multiplier skips of Rtrsm / Rsyrk in the same places; the
+// recursion is driven from the host, the work is launched on one stream.
+#include <hip/hip_runtime.h>
+
+#include "pga_internal.hpp"
+#include "posit32_core.hpp"
+#include "rgemm_kernels.hpp"
+
+namespace pga {
+namespace la {
+
+constexpr int WG = 256;
+
+__device__ __forceinline__ bool stopped(const int* stop) { return stop && *stop; }
+
+// ---------------------------------------------------------------------------------
+// TRSM, left side: one workgroup per column of B (Rtrsm.cpp:115-188)
+// ---------------------------------------------------------------------------------
+template <bool UPPER, bool TRANS>
+__global__ __launch_bounds__(WG) void k_trsm_left(int m, int n, uint32_t alpha, int nounit,
+                                                  const uint32_t* __restrict__ A, int lda,
+                                                  uint32_t* __restrict__ B, int ldb,
+                                                  uint32_t* __restrict__ work, const int* stop) {
+  const int j = blockIdx.x;
+  if (j >= n || stopped(stop)) return;
+  uint32_t* b = B + (size_t)j * ldb;
+  const int tid = threadIdx.x;
+  auto a = [&](int r, int c) { return A[r + (size_t)c * lda]; };
+
+  if (!TRANS) {
+    // B := alpha*B, then column sweep; a zero b_k skips step k (Rtrsm.cpp:128,146)
+    if (alpha != P_ONE) {
+      for (int i = tid; i < m; i += WG) b[i] = mul(alpha, b[i]);
+      __syncthreads();
+    }
+    for (int s = 0; s < m; ++s) {
+      const int k = UPPER ? m - 1 - s : s;
+      uint32_t bk = b[k];
+      bool active = (bk != 0);
+      if (active) {
+        if (nounit) bk = div(bk, a(k, k));
+        if (UPPER) {
+          for (int i = tid; i < k; i += WG) b[i] = sub(b[i], mul(bk, a(i, k)));
+        } else {
+          for (int i = k + 1 + tid; i < m; i += WG) b[i] = sub(b[i], mul(bk, a(i, k)));
+        }
+      }
+      __syncthreads();
+      if (active && nounit && tid == 0) b[k] = bk;
+    }
+    __syncthreads();
+  } else if (UPPER) {
+    // temp_i = alpha*b_i - sum_{k<i} a_ki*b_k in ascending k (Rtrsm.cpp:161-173); b_k is
+    // final before any term that needs it, so the sums advance as a sweep over k
+    for (int i = tid; i < m; i += WG) b[i] = mul(alpha, b[i]);
+    __syncthreads();
+    for (int k = 0; k < m; ++k) {
+      uint32_t bk = b[k];
+      if (nounit) bk = div(bk, a(k, k));
+      for (int i = k + 1 + tid; i < m; i += WG) b[i] = sub(b[i], mul(a(k, i), bk));
+      __syncthreads();
+      if (nounit && tid == 0) b[k] = bk;
+    }
+    __syncthreads();
+  } else {
+    // temp_i = alpha*b_i - sum_{k>i} a_ki*b_k in ascending k, i descending
+    // (Rtrsm.cpp:174-186): the first term of row i needs b_{i+1}, so the sums form one
+    // chain.  The negated products of a row are formed and decoded in parallel; one lane
+    // runs the chain of additions (exact in every case, see gemm::add_checked).
+    uint32_t* pm = work + (size_t)j * 3 * m;
+    int* ps = (int*)(pm + m);
+    uint32_t* pg = pm + 2 * (size_t)m;
+    for (int i = m - 1; i >= 0; --i) {
+      for (int k = i + 1 + tid; k < m; k += WG) {
+        dec::Val v = dec::decode(neg(mul(a(k, i), b[k])));
+        pm[k] = v.m;
+        ps[k] = v.sc;
+        pg[k] = v.sg;
+      }
+      __syncthreads();
+      if (tid == 0) {
+        dec::Val t = dec::decode(mul(alpha, b[i]));
+        int k = i + 1;
+        for (; k + 4 <= m; k += 4) {
+          dec::Val q[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) q[u] = dec::Val{pm[k + u], ps[k + u], pg[k + u]};
+#pragma unroll
+          for (int u = 0; u < 4; ++u) gemm::add_checked(t, q[u]);
+        }
+        for (; k < m; ++k) gemm::add_checked(t, dec::Val{pm[k], ps[k], pg[k]});
+        uint32_t tp = dec::encode(t);
+        if (nounit) tp = div(tp, a(i, i));
+        b[i] = tp;
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// TRSM, right side: rows of B are independent; one lane per row (Rtrsm.cpp:189-286)
+// ---------------------------------------------------------------------------------
+template <bool UPPER, bool TRANS>
+__global__ __launch_bounds__(WG) void k_trsm_right(int m, int n, uint32_t alpha, int nounit,
+                                                   const uint32_t* __restrict__ A, int lda,
+                                                   uint32_t* __restrict__ B, int ldb,
+                                                   const int* stop) {
+  const int i = blockIdx.x * WG + threadIdx.x;
+  if (i >= m || stopped(stop)) return;
+  auto a = [&](int r, int c) { return A[r + (size_t)c * lda]; };
+  auto b = [&](int c) -> uint32_t& { return B[i + (size_t)c * ldb]; };
+  if (!TRANS) {
+    if (UPPER) {  // :194-214
+      for (int j = 0; j < n; ++j) {
+        if (alpha != P_ONE) b(j) = mul(alpha, b(j));
+        for (int k = 0; k < j; ++k)
+          if (a(k, j) != 0) b(j) = sub(b(j), mul(a(k, j), b(k)));
+        if (nounit) {
+          uint32_t t = div(P_ONE, a(j, j));
+          b(j) = mul(t, b(j));
+        }
+      }
+    } else {  // :215-236
+      for (int j = n - 1; j >= 0; --j) {
+        if (alpha != P_ONE) b(j) = mul(alpha, b(j));
+        for (int k = j + 1; k < n; ++k)
+          if (a(k, j) != 0) b(j) = sub(b(j), mul(a(k, j), b(k)));
+        if (nounit) {
+          uint32_t t = div(P_ONE, a(j, j));
+          b(j) = mul(t, b(j));
+        }
+      }
+    }
+  } else {
+    if (UPPER) {  // :241-262
+      for (int k = n - 1; k >= 0; --k) {
+        if (nounit) {
+          uint32_t t = div(P_ONE, a(k, k));
+          b(k) = mul(t, b(k));
+        }
+        for (int j = 0; j < k; ++j) {
+          uint32_t t = a(j, k);
+          if (t != 0) b(j) = sub(b(j), mul(t, b(k)));
+        }
+        if (alpha != P_ONE) b(k) = mul(alpha, b(k));
+      }
+    } else {  // :263-285
+      for (int k = 0; k < n; ++k) {
+        if (nounit) {
+          uint32_t t = div(P_ONE, a(k, k));
+          b(k) = mul(t, b(k));
+        }
+        for (int j = k + 1; j < n; ++j) {
+          uint32_t t = a(j, k);
+          if (t != 0) b(j) = sub(b(j), mul(t, b(k)));
+        }
+        if (alpha != P_ONE) b(k) = mul(alpha, b(k));
+      }
+    }
+  }
+}
+
+// B = 0 (alpha == 0, Rtrsm.cpp:101-108)
+__global__ __launch_bounds__(WG) void k_zero(int m, int n, uint32_t* B, int ldb, const int* stop) {
+  const int i = blockIdx.x * WG + threadIdx.x;
+  if (i >= m || stopped(stop)) return;
+  for (int j = 0; j < n; ++j) B[i + (size_t)j * ldb] = 0;
+}
+
+// ---------------------------------------------------------------------------------
+// GEMV (Rgemv.cpp:31-204), one lane per element of y
+// ---------------------------------------------------------------------------------
+__global__ __launch_bounds__(WG) void k_gemv(int trans, int m, int n, uint32_t alpha,
+                                             const uint32_t* __restrict__ A, int lda,
+                                             const uint32_t* __restrict__ x, int incx, uint32_t beta,
+                                             uint32_t* __restrict__ y, int incy) {
+  const int leny = trans ? n : m, lenx = trans ? m : n;
+  const int r = blockIdx.x * WG + threadIdx.x;
+  if (r >= leny) return;
+  const long kx = incx > 0 ? 0 : -(long)(lenx - 1) * incx;
+  const long ky = incy > 0 ? 0 : -(long)(leny - 1) * incy;
+  uint32_t& yr = y[ky + (long)r * incy];
+  uint32_t v = yr;
+  if (beta != P_ONE) v = (beta == 0) ? 0u : mul(beta, v);
+  if (alpha != 0) {
+    if (!trans) {  // y_i += (alpha*x_j)*a_ij, ascending j (:150-173)
+      for (int j = 0; j < n; ++j) v = add(v, mul(mul(alpha, x[kx + (long)j * incx]), A[r + (size_t)j * lda]));
+    } else {  // y_j += alpha*(sum_i a_ij*x_i) (:174-201)
+      uint32_t t = 0;
+      for (int i = 0; i < m; ++i) t = add(t, mul(A[i + (size_t)r * lda], x[kx + (long)i * incx]));
+      v = add(v, mul(alpha, t));
+    }
+  }
+  yr = v;
+}
+
+// SCAL (Rscal.cpp:62-119)
+__global__ __launch_bounds__(WG) void k_scal(int n, uint32_t da, uint32_t* x, int incx) {
+  for (long i = blockIdx.x * (long)WG + threadIdx.x; i < n; i += (long)gridDim.x * WG)
+    x[i * incx] = mul(da, x[i * incx]);
+}
+
+// IAMAX (iRamax.cpp:31-88): first index of the largest |x| in signed pattern order
+__device__ void block_iamax(int n, const uint32_t* x, long incx, int* out_idx) {
+  __shared__ uint32_t sv[WG];
+  __shared__ int si[WG];
+  const int tid = threadIdx.x;
+  uint32_t best = 0;
+  int bi = -1;
+  for (int i = tid; i < n; i += WG) {
+    uint32_t v = abs_p(x[(long)i * incx]);
+    if (bi < 0 || (int32_t)v > (int32_t)best) {
+      best = v;
+      bi = i;
+    }
+  }
+  sv[tid] = best;
+  si[tid] = bi;
+  __syncthreads();
+  for (int s = WG / 2; s > 0; s >>= 1) {
+    if (tid < s) {
+      int oi = si[tid + s];
+      uint32_t ov = sv[tid + s];
+      int mi = si[tid];
+      if (oi >= 0 && (mi < 0 || (int32_t)ov > (int32_t)sv[tid] ||
+                      ((int32_t)ov == (int32_t)sv[tid] && oi < mi))) {
+        sv[tid] = ov;
+        si[tid] = oi;
+      }
+    }
+    __syncthreads();
+  }
+  if (tid == 0) *out_idx = si[0];  // 0-based, -1 when n == 0
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(WG) void k_iamax(int n, const uint32_t* x, int incx, int* result) {
+  __shared__ int idx;
+  if (n < 1 || incx <= 0) {
+    if (threadIdx.x == 0) *result = 0;
+    return;
+  }
+  block_iamax(n, x, incx, &idx);
+  if (threadIdx.x == 0) *result = idx + 1;
+}
+
+// LASWP (Rlaswp.cpp:32-111): one lane per column; ipiv entries minus `ipoff` are rows
+// (1-based) of the block starting at `a`.
+__global__ __launch_bounds__(WG) void k_laswp(int n, uint32_t* a, int lda, int k1, int k2,
+                                              const int* ipiv, int incx, int ipoff, const int* stop) {
+  const int col = blockIdx.x * WG + threadIdx.x;
+  if (col >= n || stopped(stop)) return;
+  int ix0, i1, i2, inc;
+  if (incx > 0) {
+    ix0 = k1;
+    i1 = k1;
+    i2 = k2;
+    inc = 1;
+  } else {
+    ix0 = k1 + (k1 - k2) * incx;
+    i1 = k2;
+    i2 = k1;
+    inc = -1;
+  }
+  uint32_t* ac = a + (size_t)col * lda;
+  int ix = ix0;
+  for (int i = i1; inc > 0 ? i <= i2 : i >= i2; i += inc) {
+    int ip = ipiv[ix - 1] - ipoff;
+    if (ip != i) {
+      uint32_t t = ac[i - 1];
+      ac[i - 1] = ac[ip - 1];
+      ac[ip - 1] = t;
+    }
+    ix += incx;
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// Rgetrf2 leaves (Rgetrf2.cpp:86-132); ipiv is written as a global 1-based row index
+// (row_off + local), info as a global column index (col_off + 1), first one only.
+// ---------------------------------------------------------------------------------
+__global__ __launch_bounds__(WG) void k_getrf2_row(uint32_t* a, int* ipiv, int* info, int row_off,
+                                                   int col_off) {
+  if (threadIdx.x != 0) return;
+  ipiv[0] = row_off + 1;
+  if (a[0] == 0 && *info == 0) *info = col_off + 1;
+}
+
+__global__ __launch_bounds__(WG) void k_getrf2_col(int m, uint32_t* a, int* ipiv, int* info, int row_off,
+                                                   int col_off) {
+  __shared__ int piv;
+  block_iamax(m, a, 1, &piv);  // 0-based
+  const int p = piv;
+  const uint32_t ap = a[p];
+  if (threadIdx.x == 0) ipiv[0] = row_off + p + 1;
+  if (ap == 0) {
+    if (threadIdx.x == 0 && *info == 0) *info = col_off + 1;
+    return;
+  }
+  __syncthreads();
+  const uint32_t a0 = a[0];
+  __syncthreads();
+  if (threadIdx.x == 0 && p != 0) {
+    a[0] = ap;
+    a[p] = a0;
+  }
+  __syncthreads();
+  // sfmin = FLT_MIN as posit32 = minpos; abs(pivot) >= sfmin except for NaR
+  const uint32_t piv_v = ap;
+  if (!lt(abs_p(piv_v), P_MINPOS)) {
+    const uint32_t r = div(P_ONE, piv_v);  // Rscal(m-1, one/a[0], ...)
+    for (int i = 1 + threadIdx.x; i < m; i += WG) a[i] = mul(r, a[i]);
+  } else {
+    for (int i = 1 + threadIdx.x; i < m; i += WG) a[i] = div(a[i], piv_v);
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// Rpotrf2 leaf (Rpotrf2.cpp:86-100): not positive (or NaR) -> info, else sqrt
+// ---------------------------------------------------------------------------------
+__global__ void k_potrf2_leaf(uint32_t* a, int* info, int col_off) {
+  if (threadIdx.x != 0 || *info) return;
+  uint32_t v = a[0];
+  if (le(v, 0)) {
+    *info = col_off + 1;
+    return;
+  }
+  a[0] = pga::sqrt(v);
+}
+
+}  // namespace la
+
+// ===================================================================================
+// Host-side drivers (launch sequences).  All take device pointers and one stream.
+// ===================================================================================
+namespace lah {
+
+using namespace la;
+
+static unsigned blocks_for(long n) { return (unsigned)((n + WG - 1) / WG); }
+
+struct Ctx {
+  hipStream_t s;
+  GemmImpl impl;
+};
+
+static hipError_t trsm(const Ctx& c, bool left, bool upper, bool trans, bool nounit, int m, int n,
+                       uint32_t alpha, const uint32_t* A, int lda, uint32_t* B, int ldb,
+                       const int* stop) {
+  if (m == 0 || n == 0) return hipSuccess;
+  if (alpha == 0) {
+    hipLaunchKernelGGL(k_zero, dim3(blocks_for(m)), dim3(WG), 0, c.s, m, n, B, ldb, stop);
+    return hipGetLastError();
+  }
+  if (left) {
+    uint32_t* work = nullptr;
+    if (trans && !upper) {
+      pga_status st;
+      work = (uint32_t*)scratch(5, (size_t)3 * m * n * sizeof(uint32_t), &st);
+      if (st) return hipErrorOutOfMemory;
+    }
+#define PGA_TL(U, T)                                                                               \
+  hipLaunchKernelGGL((k_trsm_left<U, T>), dim3(n), dim3(WG), 0, c.s, m, n, alpha, (int)nounit, A,  \
+                     lda, B, ldb, work, stop)
+    if (upper && !trans) PGA_TL(true, false);
+    else if (!upper && !trans) PGA_TL(false, false);
+    else if (upper && trans) PGA_TL(true, true);
+    else PGA_TL(false, true);
+#undef PGA_TL
+  } else {
+#define PGA_TR(U, T)                                                                              \
+  hipLaunchKernelGGL((k_trsm_right<U, T>), dim3(blocks_for(m)), dim3(WG), 0, c.s, m, n, alpha,    \
+                     (int)nounit, A, lda, B, ldb, stop)
+    if (upper && !trans) PGA_TR(true, false);
+    else if (!upper && !trans) PGA_TR(false, false);
+    else if (upper && trans) PGA_TR(true, true);
+    else PGA_TR(false, true);
+#undef PGA_TR
+  }
+  return hipGetLastError();
+}
+
+static hipError_t laswp(const Ctx& c, int n, uint32_t* a, int lda, int k1, int k2, const int* ipiv,
+                        int incx, int ipoff, const int* stop) {
+  if (n <= 0 || incx == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_laswp, dim3(blocks_for(n)), dim3(WG), 0, c.s, n, a, lda, k1, k2, ipiv, incx, ipoff,
+                     stop);
+  return hipGetLastError();
+}
+
+#define PGA_TRY(x)                         \
+  do {                                     \
+    hipError_t _e = (x);                   \
+    if (_e != hipSuccess) return _e;       \
+  } while (0)
+
+// Rgetrf2 (Rgetrf2.cpp:32-185) on the block at `a` (rows row_off.., columns col_off..
+// of the matrix that ipiv / info describe).  ipiv points at the block's first entry.
+static hipError_t getrf2(const Ctx& c, int m, int n, uint32_t* a, int lda, int* ipiv, int* info,
+                         int row_off, int col_off) {
+  if (m == 0 || n == 0) return hipSuccess;
+  if (m == 1) {
+    hipLaunchKernelGGL(k_getrf2_row, dim3(1), dim3(WG), 0, c.s, a, ipiv, info, row_off, col_off);
+    return hipGetLastError();
+  }
+  if (n == 1) {
+    hipLaunchKernelGGL(k_getrf2_col, dim3(1), dim3(WG), 0, c.s, m, a, ipiv, info, row_off, col_off);
+    return hipGetLastError();
+  }
+  const int n1 = (m < n ? m : n) / 2, n2 = n - n1;
+  uint32_t* a12 = a + (size_t)n1 * lda;
+  PGA_TRY(getrf2(c, m, n1, a, lda, ipiv, info, row_off, col_off));
+  PGA_TRY(laswp(c, n2, a12, lda, 1, n1, ipiv, 1, row_off, nullptr));
+  PGA_TRY(trsm(c, true, false, false, false, n1, n2, P_ONE, a, lda, a12, lda, nullptr));
+  PGA_TRY(rgemm_launch(false, false, m - n1, n2, n1, P_MONE, a + n1, lda, a12, lda, P_ONE,
+                       a12 + n1, lda, c.s, c.impl));
+  PGA_TRY(getrf2(c, m - n1, n2, a12 + n1, lda, ipiv + n1, info, row_off + n1, col_off + n1));
+  const int mn = m < n ? m : n;
+  PGA_TRY(laswp(c, n1, a, lda, n1 + 1, mn, ipiv, 1, row_off, nullptr));
+  return hipSuccess;
+}
+
+// Rgetrf_opencl (Rgetrf_opencl.cpp:32-154): blocked right-looking LU, nb columns per
+// panel; ipiv holds global 1-based rows, info the first zero pivot column.
+static hipError_t getrf(const Ctx& c, int m, int n, uint32_t* a, int lda, int* ipiv, int* info, int nb) {
+  const int mn = m < n ? m : n;
+  if (nb <= 1 || nb >= mn) return getrf2(c, m, n, a, lda, ipiv, info, 0, 0);
+  for (int j = 0; j < mn; j += nb) {
+    const int jb = (mn - j < nb) ? mn - j : nb;
+    uint32_t* ajj = a + j + (size_t)j * lda;
+    PGA_TRY(getrf2(c, m - j, jb, ajj, lda, ipiv + j, info, j, j));
+    // rows j..j+jb-1 of the columns left of the panel (Rgetrf_opencl.cpp:123)
+    PGA_TRY(laswp(c, j, a, lda, j + 1, j + jb, ipiv, 1, 0, nullptr));
+    if (j + jb < n) {
+      uint32_t* a12 = a + j + (size_t)(j + jb) * lda;
+      PGA_TRY(laswp(c, n - j - jb, a + (size_t)(j + jb) * lda, lda, j + 1, j + jb, ipiv, 1, 0, nullptr));
+      PGA_TRY(trsm(c, true, false, false, false, jb, n - j - jb, P_ONE, ajj, lda, a12, lda, nullptr));
+      if (j + jb < m)
+        PGA_TRY(rgemm_launch(false, false, m - j - jb, n - j - jb, jb, P_MONE, ajj + jb, lda, a12, lda,
+                             P_ONE, a12 + jb, lda, c.s, c.impl));
+    }
+  }
+  return hipSuccess;
+}
+
+// Rgetrs (Rgetrs.cpp:32-118)
+static hipError_t getrs(const Ctx& c, bool trans, int n, int nrhs, const uint32_t* a, int lda,
+                        const int* ipiv, uint32_t* b, int ldb) {
+  if (n == 0 || nrhs == 0) return hipSuccess;
+  if (!trans) {
+    PGA_TRY(laswp(c, nrhs, b, ldb, 1, n, ipiv, 1, 0, nullptr));
+    PGA_TRY(trsm(c, true, false, false, false, n, nrhs, P_ONE, a, lda, b, ldb, nullptr));
+    PGA_TRY(trsm(c, true, true, false, true, n, nrhs, P_ONE, a, lda, b, ldb, nullptr));
+  } else {
+    PGA_TRY(trsm(c, true, true, true, true, n, nrhs, P_ONE, a, lda, b, ldb, nullptr));
+    PGA_TRY(trsm(c, true, false, true, false, n, nrhs, P_ONE, a, lda, b, ldb, nullptr));
+    PGA_TRY(laswp(c, nrhs, b, ldb, 1, n, ipiv, -1, 0, nullptr));
+  }
+  return hipSuccess;
+}
+
+// Rsyrk (Rsyrk.cpp:31-215) as a triangular GEMM: 'N' is C op= alpha*A*A^T with zero
+// multipliers skipped, 'T' is the sum-first form alpha*A^T*A + beta*C.
+static hipError_t syrk(const Ctx& c, bool upper, bool trans, int n, int k, uint32_t alpha,
+                       const uint32_t* a, int lda, uint32_t beta, uint32_t* cc, int ldc, const int* stop) {
+  if (n == 0 || ((alpha == 0 || k == 0) && beta == P_ONE)) return hipSuccess;
+  GemmOpts o;
+  o.skip_zero_b = !trans;
+  o.tri = upper ? 2 : 1;
+  o.stop = stop;
+  if (!trans) return rgemm_launch(false, true, n, n, k, alpha, a, lda, a, lda, beta, cc, ldc, c.s, c.impl, o);
+  return rgemm_launch(true, false, n, n, k, alpha, a, lda, a, lda, beta, cc, ldc, c.s, c.impl, o);
+}
+
+// Rpotrf2 (Rpotrf2.cpp:32-151); info (device) doubles as the stop flag
+static hipError_t potrf2(const Ctx& c, bool upper, int n, uint32_t* a, int lda, int* info, int col_off) {
+  if (n == 0) return hipSuccess;
+  if (n == 1) {
+    hipLaunchKernelGGL(k_potrf2_leaf, dim3(1), dim3(64), 0, c.s, a, info, col_off);
+    return hipGetLastError();
+  }
+  const int n1 = n / 2, n2 = n - n1;
+  uint32_t* a22 = a + n1 + (size_t)n1 * lda;
+  PGA_TRY(potrf2(c, upper, n1, a, lda, info, col_off));
+  if (upper) {
+    PGA_TRY(trsm(c, true, true, true, true, n1, n2, P_ONE, a, lda, a + (size_t)n1 * lda, lda, info));
+    PGA_TRY(syrk(c, true, true, n2, n1, P_MONE, a + (size_t)n1 * lda, lda, P_ONE, a22, lda, info));
+  } else {
+    PGA_TRY(trsm(c, false, false, true, true, n2, n1, P_ONE, a, lda, a + n1, lda, info));
+    PGA_TRY(syrk(c, false, false, n2, n1, P_MONE, a + n1, lda, P_ONE, a22, lda, info));
+  }
+  return potrf2(c, upper, n2, a22, lda, info, col_off + n1);
+}
+
+// Rpotrf_opencl (Rpotrf_opencl.cpp:32-166): blocked left-looking Cholesky
+static hipError_t potrf(const Ctx& c, bool upper, int n, uint32_t* a, int lda, int* info, int nb) {
+  if (nb <= 1 || nb >= n) return potrf2(c, upper, n, a, lda, info, 0);
+  auto A = [&](int r, int col) { return a + r + (size_t)col * lda; };
+  for (int j = 0; j < n; j += nb) {
+    const int jb = (n - j < nb) ? n - j : nb;
+    if (upper) {
+      PGA_TRY(syrk(c, true, true, jb, j, P_MONE, A(0, j), lda, P_ONE, A(j, j), lda, info));
+      PGA_TRY(potrf2(c, true, jb, A(j, j), lda, info, j));
+      if (j + jb < n) {
+        GemmOpts o;
+        o.stop = info;
+        PGA_TRY(rgemm_launch(true, false, jb, n - j - jb, j, P_MONE, A(0, j), lda, A(0, j + jb), lda,
+                             P_ONE, A(j, j + jb), lda, c.s, c.impl, o));
+        PGA_TRY(trsm(c, true, true, true, true, jb, n - j - jb, P_ONE, A(j, j), lda, A(j, j + jb), lda,
+                     info));
+      }
+    } else {
+      PGA_TRY(syrk(c, false, false, jb, j, P_MONE, A(j, 0), lda, P_ONE, A(j, j), lda, info));
+      PGA_TRY(potrf2(c, false, jb, A(j, j), lda, info, j));
+      if (j + jb < n) {
+        GemmOpts o;
+        o.stop = info;
+        PGA_TRY(rgemm_launch(false, true, n - j - jb, jb, j, P_MONE, A(j + jb, 0), lda, A(j, 0), lda,
+                             P_ONE, A(j + jb, j), lda, c.s, c.impl, o));
+        PGA_TRY(trsm(c, false, false, true, true, n - j - jb, jb, P_ONE, A(j, j), lda, A(j + jb, j), lda,
+                     info));
+      }
+    }
+  }
+  return hipSuccess;
+}
+
+// Rpotrs (Rpotrs.cpp:32-110)
+static hipError_t potrs(const Ctx& c, bool upper, int n, int nrhs, const uint32_t* a, int lda, uint32_t* b,
+                        int ldb) {
+  if (n == 0 || nrhs == 0) return hipSuccess;
+  if (upper) {
+    PGA_TRY(trsm(c, true, true, true, true, n, nrhs, P_ONE, a, lda, b, ldb, nullptr));
+    PGA_TRY(trsm(c, true, true, false, true, n, nrhs, P_ONE, a, lda, b, ldb, nullptr));
+  } else {
+    PGA_TRY(trsm(c, true, false, false, true, n, nrhs, P_ONE, a, lda, b, ldb, nullptr));
+    PGA_TRY(trsm(c, true, false, true, true, n, nrhs, P_ONE, a, lda, b, ldb, nullptr));
+  }
+  return hipSuccess;
+}
+
+#undef PGA_TRY
+
+}  // namespace lah
+
+// exported to the C ABI layer (capi_lapack.cpp)
+hipError_t la_trsm(hipStream_t s, bool left, bool upper, bool trans, bool nounit, int m, int n, uint32_t alpha,
+                   const uint32_t* A, int lda, uint32_t* B, int ldb) {
+  return lah::trsm({s, runtime().gemm_impl}, left, upper, trans, nounit, m, n, alpha, A, lda, B, ldb, nullptr);
+}
+hipError_t la_syrk(hipStream_t s, bool upper, bool trans, int n, int k, uint32_t alpha, const uint32_t* a, int lda,
+                   uint32_t beta, uint32_t* c, int ldc) {
+  return lah::syrk({s, runtime().gemm_impl}, upper, trans, n, k, alpha, a, lda, beta, c, ldc, nullptr);
+}
+hipError_t la_gemv(hipStream_t s, bool trans, int m, int n, uint32_t alpha, const uint32_t* A, int lda,
+                   const uint32_t* x, int incx, uint32_t beta, uint32_t* y, int incy) {
+  if (m == 0 || n == 0 || (alpha == 0 && beta == P_ONE)) return hipSuccess;
+  const int leny = trans ? n : m;
+  hipLaunchKernelGGL(la::k_gemv, dim3(lah::blocks_for(leny)), dim3(la::WG), 0, s, (int)trans, m, n, alpha, A,
+                     lda, x, incx, beta, y, incy);
+  return hipGetLastError();
+}
+hipError_t la_scal(hipStream_t s, int n, uint32_t da, uint32_t* x, int incx) {
+  if (n <= 0 || incx <= 0) return hipSuccess;
+  unsigned g = lah::blocks_for(n);
+  if (g > 65536) g = 65536;
+  hipLaunchKernelGGL(la::k_scal, dim3(g), dim3(la::WG), 0, s, n, da, x, incx);
+  return hipGetLastError();
+}
+hipError_t la_iamax(hipStream_t s, int n, const uint32_t* x, int incx, int* result) {
+  hipLaunchKernelGGL(la::k_iamax, dim3(1), dim3(la::WG), 0, s, n, x, incx, result);
+  return hipGetLastError();
+}
+hipError_t la_laswp(hipStream_t s, int n, uint32_t* a, int lda, int k1, int k2, const int* ipiv, int incx) {
+  return lah::laswp({s, runtime().gemm_impl}, n, a, lda, k1, k2, ipiv, incx, 0, nullptr);
+}
+hipError_t la_getrf(hipStream_t s, int m, int n, uint32_t* a, int lda, int* ipiv, int* info, int nb) {
+  return lah::getrf({s, runtime().gemm_impl}, m, n, a, lda, ipiv, info, nb);
+}
+hipError_t la_getrf2(hipStream_t s, int m, int n, uint32_t* a, int lda, int* ipiv, int* info) {
+  return lah::getrf2({s, runtime().gemm_impl}, m, n, a, lda, ipiv, info, 0, 0);
+}
+hipError_t la_getrs(hipStream_t s, bool trans, int n, int nrhs, const uint32_t* a, int lda, const int* ipiv,
+                    uint32_t* b, int ldb) {
+  return lah::getrs({s, runtime().gemm_impl}, trans, n, nrhs, a, lda, ipiv, b, ldb);
+}
+hipError_t la_potrf(hipStream_t s, bool upper, int n, uint32_t* a, int lda, int* info, int nb) {
+  return lah::potrf({s, runtime().gemm_impl}, upper, n, a, lda, info, nb);
+}
+hipError_t la_potrf2(hipStream_t s, bool upper, int n, uint32_t* a, int lda, int* info) {
+  return lah::potrf2({s, runtime().gemm_impl}, upper, n, a, lda, info, 0);
+}
+hipError_t la_potrs(hipStream_t s, bool upper, int n, int nrhs, const uint32_t* a, int lda, uint32_t* b,
+                    int ldb) {
+  return lah::potrs({s, runtime().gemm_impl}, upper, n, nrhs, a, lda, b, ldb);
+}
+
+}  // namespace pga

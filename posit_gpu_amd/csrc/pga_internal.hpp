@@ -10,10 +10,18 @@ namespace pga {
 
 enum class GemmImpl { Tiled = 0, Exact = 1 };
 
+// Options of the device GEMM used by the factorizations.
+struct GemmOpts {
+  bool skip_zero_b = false;  // skip l when the B-side multiplier is zero (Rsyrk 'N')
+  int tri = 0;               // 0 full C, 1 store i >= j only, 2 store i <= j only
+  const int* stop = nullptr; // device flag: when nonzero the launch does nothing
+  int tile = 0;              // 0 automatic, else 64 / 32 / 16
+};
+
 // Device-pointer GEMM launch (rgemm.hip); arguments already validated.
 hipError_t rgemm_launch(bool ta, bool tb, int m, int n, int k, uint32_t alpha, const uint32_t* A,
                         int lda, const uint32_t* B, int ldb, uint32_t beta, uint32_t* C, int ldc,
-                        hipStream_t s, GemmImpl impl);
+                        hipStream_t s, GemmImpl impl, const GemmOpts& o = GemmOpts());
 
 // Runtime state (runtime.hip)
 struct Runtime {
@@ -21,6 +29,7 @@ struct Runtime {
   int device = -1;
   hipStream_t stream = nullptr;
   GemmImpl gemm_impl = GemmImpl::Tiled;
+  int cus = 256;
 };
 Runtime& runtime();
 

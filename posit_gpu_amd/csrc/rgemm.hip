@@ -14,11 +14,11 @@ namespace gemm {
 // ---------------------------------------------------------------------------------
 // Tiled kernel
 // ---------------------------------------------------------------------------------
-template <int BM, int BN, int BK, int TM, int TN, bool TA, bool TB>
+template <int BM, int BN, int BK, int TM, int TN, bool TA, bool TB, bool SKIP0>
 __global__ __launch_bounds__((BM / TM) * (BN / TN)) void rgemm_tiled(
     int m, int n, int k, uint32_t alpha, const uint32_t* __restrict__ A, int lda,
     const uint32_t* __restrict__ B, int ldb, uint32_t beta, uint32_t* __restrict__ C, int ldc,
-    int tiles_m) {
+    int tiles_m, int tri, const int* __restrict__ stop) {
   constexpr int NX = BM / TM;
   constexpr int NY = BN / TN;
   constexpr int NT = NX * NY;
@@ -44,6 +44,10 @@ __global__ __launch_bounds__((BM / TM) * (BN / TN)) void rgemm_tiled(
   const int bid = blockIdx.x;
   const int i0 = (bid % tiles_m) * BM;
   const int j0 = (bid / tiles_m) * BN;
+  if (stop && *stop) return;  // an earlier step of a factorization failed
+  // triangular update (Rsyrk): skip tiles with no element in the stored triangle
+  if (tri == 1 && i0 + BM - 1 < j0) return;
+  if (tri == 2 && j0 + BN - 1 < i0) return;
 
   // ---- accumulators ----
   Val acc[TM][TN];
@@ -102,7 +106,9 @@ __global__ __launch_bounds__((BM / TM) * (BN / TN)) void rgemm_tiled(
       int j = j0 + jj, l = kt + ll;
       uint32_t v = 0;
       if (j < n && l < k) v = TB ? B[j + (size_t)l * ldb] : B[l + (size_t)j * ldb];
-      if (!SUM_FIRST) v = alpha_one ? v : (alpha_mone ? pga::neg(v) : pga::mul(alpha, v));
+      // alpha*b; with SKIP0 a zero b stays zero (it marks a skipped step even if alpha is NaR)
+      if (!SUM_FIRST && !(SKIP0 && v == 0))
+        v = alpha_one ? v : (alpha_mone ? pga::neg(v) : pga::mul(alpha, v));
       Val d = dec::decode(v);
       sTm[ll][jj] = d.m;
       sTs[ll][jj] = d.sc;
@@ -191,7 +197,9 @@ __global__ __launch_bounds__((BM / TM) * (BN / TN)) void rgemm_tiled(
           for (int l = 0; l < lmax; ++l) {
             Val a{sAm[l][tx + r * NX], sAs[l][tx + r * NX], sAg[l][tx + r * NX]};
             Val t{sTm[l][ty + c * NY], sTs[l][ty + c * NY], sTg[l][ty + c * NY]};
-            pc = pga::add(pc, pga::mul(dec::encode(t), dec::encode(a)));
+            uint32_t tp = dec::encode(t);
+            if (SKIP0 && tp == 0) continue;  // Rsyrk.cpp:149,169 skip zero multipliers
+            pc = pga::add(pc, pga::mul(tp, dec::encode(a)));
           }
           acc[r][c] = dec::decode(pc);
         }
@@ -212,7 +220,8 @@ __global__ __launch_bounds__((BM / TM) * (BN / TN)) void rgemm_tiled(
 #pragma unroll
     for (int c = 0; c < TN; ++c) {
       int i = i0 + tx + r * NX, j = j0 + ty + c * NY;
-      if (i < m && j < n) {
+      bool keep = (tri == 0) || (tri == 1 ? i >= j : i <= j);
+      if (i < m && j < n && keep) {
         uint32_t v = dec::encode(acc[r][c]);
         if (SUM_FIRST) {
           uint32_t at = pga::mul(alpha, v);
@@ -228,20 +237,23 @@ __global__ __launch_bounds__((BM / TM) * (BN / TN)) void rgemm_tiled(
 // Exact kernel: one element per thread, pattern-level operations only.  Used for
 // validation of the tiled kernel and selectable at run time.
 // ---------------------------------------------------------------------------------
-template <bool TA, bool TB>
+template <bool TA, bool TB, bool SKIP0>
 __global__ __launch_bounds__(256) void rgemm_exact(int m, int n, int k, uint32_t alpha,
                                                    const uint32_t* __restrict__ A, int lda,
                                                    const uint32_t* __restrict__ B, int ldb,
                                                    uint32_t beta, uint32_t* __restrict__ C,
-                                                   int ldc) {
+                                                   int ldc, int tri, const int* __restrict__ stop) {
   int i = blockIdx.x * 16 + (threadIdx.x % 16);
   int j = blockIdx.y * 16 + (threadIdx.x / 16);
   if (i >= m || j >= n) return;
+  if (stop && *stop) return;
+  if ((tri == 1 && i < j) || (tri == 2 && i > j)) return;
   uint32_t* cij = C + i + (size_t)j * ldc;
   if (!TA) {
     uint32_t c = (beta == 0) ? 0u : (beta == P_ONE ? *cij : pga::mul(beta, *cij));
     for (int l = 0; l < k; ++l) {
       uint32_t b = TB ? B[j + (size_t)l * ldb] : B[l + (size_t)j * ldb];
+      if (SKIP0 && b == 0) continue;
       c = pga::add(c, pga::mul(pga::mul(alpha, b), A[i + (size_t)l * lda]));
     }
     *cij = c;
@@ -257,38 +269,79 @@ __global__ __launch_bounds__(256) void rgemm_exact(int m, int n, int k, uint32_t
 }
 
 // alpha == 0 (Rgemm.cpp:142-157): C = beta==0 ? 0 : beta*C
-__global__ __launch_bounds__(256) void scale_c(int m, int n, uint32_t beta, uint32_t* C, int ldc) {
+__global__ __launch_bounds__(256) void scale_c(int m, int n, uint32_t beta, uint32_t* C, int ldc,
+                                               int j0, int tri, const int* stop) {
   int i = blockIdx.x * 256 + threadIdx.x;
   int j = blockIdx.y;
   if (i >= m) return;
+  if (stop && *stop) return;
+  if ((tri == 1 && i < j0 + j) || (tri == 2 && i > j0 + j)) return;
   uint32_t* p = C + i + (size_t)j * ldc;
   *p = (beta == 0) ? 0u : pga::mul(beta, *p);
 }
 
-constexpr int BM = 64, BN = 64, BK = 16, TM = 4, TN = 4;
-constexpr int NTHREADS = (BM / TM) * (BN / TN);
-
-template <bool TA, bool TB>
-static hipError_t launch_tiled(int m, int n, int k, uint32_t alpha, const uint32_t* A, int lda,
-                               const uint32_t* B, int ldb, uint32_t beta, uint32_t* C, int ldc,
-                               hipStream_t s) {
+// Tile shapes: 64x64 with 4x4 elements per lane for large C; 32x32 (2x2) and 16x16
+// (1x1) keep enough waves in flight for skinny C (e.g. Cholesky's N = 64 updates),
+// since K is never split.
+template <int BM, int TM, bool TA, bool TB, bool SKIP0>
+static hipError_t launch_tiled_cfg(int m, int n, int k, uint32_t alpha, const uint32_t* A, int lda,
+                                   const uint32_t* B, int ldb, uint32_t beta, uint32_t* C, int ldc,
+                                   hipStream_t s, int tri, const int* stop) {
+  constexpr int BN = BM, TN = TM, BK = 16;
+  constexpr int NT = (BM / TM) * (BN / TN);
   int tm = (m + BM - 1) / BM, tn = (n + BN - 1) / BN;
   long long blocks = (long long)tm * tn;
   if (blocks > 0x7FFFFFFF) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((rgemm_tiled<BM, BN, BK, TM, TN, TA, TB>), dim3((unsigned)blocks),
-                     dim3(NTHREADS), 0, s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tm);
+  hipLaunchKernelGGL((rgemm_tiled<BM, BN, BK, TM, TN, TA, TB, SKIP0>), dim3((unsigned)blocks), dim3(NT),
+                     0, s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tm, tri, stop);
   return hipGetLastError();
 }
 
-template <bool TA, bool TB>
+// Enough work for every CU several times over at the given tile size
+static int pick_tile(int m, int n, int cus) {
+  long long t64 = (long long)((m + 63) / 64) * ((n + 63) / 64);
+  long long t32 = (long long)((m + 31) / 32) * ((n + 31) / 32);
+  if (t64 >= 2LL * cus) return 64;
+  if (t32 >= 2LL * cus) return 32;
+  return 16;
+}
+
+template <bool TA, bool TB, bool SKIP0>
+static hipError_t launch_tiled(int m, int n, int k, uint32_t alpha, const uint32_t* A, int lda,
+                               const uint32_t* B, int ldb, uint32_t beta, uint32_t* C, int ldc,
+                               hipStream_t s, int tri, const int* stop, int tile) {
+  if (tile == 64)
+    return launch_tiled_cfg<64, 4, TA, TB, SKIP0>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, tri, stop);
+  if (tile == 32)
+    return launch_tiled_cfg<32, 2, TA, TB, SKIP0>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, tri, stop);
+  return launch_tiled_cfg<16, 1, TA, TB, SKIP0>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, tri, stop);
+}
+
+template <bool TA, bool TB, bool SKIP0>
 static hipError_t launch_exact(int m, int n, int k, uint32_t alpha, const uint32_t* A, int lda,
                                const uint32_t* B, int ldb, uint32_t beta, uint32_t* C, int ldc,
-                               hipStream_t s) {
+                               hipStream_t s, int tri, const int* stop) {
   dim3 grid((m + 15) / 16, (n + 15) / 16);
   if (grid.y > 65535) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((rgemm_exact<TA, TB>), grid, dim3(256), 0, s, m, n, k, alpha, A, lda, B, ldb,
-                     beta, C, ldc);
+  hipLaunchKernelGGL((rgemm_exact<TA, TB, SKIP0>), grid, dim3(256), 0, s, m, n, k, alpha, A, lda, B, ldb,
+                     beta, C, ldc, tri, stop);
   return hipGetLastError();
+}
+
+template <bool SKIP0>
+static hipError_t dispatch(bool ta, bool tb, int m, int n, int k, uint32_t alpha, const uint32_t* A,
+                           int lda, const uint32_t* B, int ldb, uint32_t beta, uint32_t* C, int ldc,
+                           hipStream_t s, GemmImpl impl, int tri, const int* stop, int tile) {
+  if (impl == GemmImpl::Exact) {
+    if (!ta && !tb) return launch_exact<false, false, SKIP0>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, tri, stop);
+    if (!ta && tb) return launch_exact<false, true, SKIP0>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, tri, stop);
+    if (ta && !tb) return launch_exact<true, false, SKIP0>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, tri, stop);
+    return launch_exact<true, true, SKIP0>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, tri, stop);
+  }
+  if (!ta && !tb) return launch_tiled<false, false, SKIP0>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, tri, stop, tile);
+  if (!ta && tb) return launch_tiled<false, true, SKIP0>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, tri, stop, tile);
+  if (ta && !tb) return launch_tiled<true, false, SKIP0>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, tri, stop, tile);
+  return launch_tiled<true, true, SKIP0>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, tri, stop, tile);
 }
 
 }  // namespace gemm
@@ -296,31 +349,21 @@ static hipError_t launch_exact(int m, int n, int k, uint32_t alpha, const uint32
 // Device-pointer GEMM with Rgemm's quick returns; the caller has validated arguments.
 hipError_t rgemm_launch(bool ta, bool tb, int m, int n, int k, uint32_t alpha, const uint32_t* A,
                         int lda, const uint32_t* B, int ldb, uint32_t beta, uint32_t* C, int ldc,
-                        hipStream_t s, GemmImpl impl) {
+                        hipStream_t s, GemmImpl impl, const GemmOpts& o) {
   using namespace gemm;
   if (m == 0 || n == 0 || ((alpha == 0 || k == 0) && beta == P_ONE)) return hipSuccess;
   if (alpha == 0) {
-    if (n > 65535) {
-      for (int j0 = 0; j0 < n; j0 += 65535) {
-        int nn = (n - j0 < 65535) ? n - j0 : 65535;
-        hipLaunchKernelGGL(scale_c, dim3((m + 255) / 256, nn), dim3(256), 0, s, m, nn, beta,
-                           C + (size_t)j0 * ldc, ldc);
-      }
-    } else {
-      hipLaunchKernelGGL(scale_c, dim3((m + 255) / 256, n), dim3(256), 0, s, m, n, beta, C, ldc);
+    for (int j0 = 0; j0 < n; j0 += 65535) {
+      int nn = (n - j0 < 65535) ? n - j0 : 65535;
+      hipLaunchKernelGGL(scale_c, dim3((m + 255) / 256, nn), dim3(256), 0, s, m, nn, beta,
+                         C + (size_t)j0 * ldc, ldc, j0, o.tri, o.stop);
     }
     return hipGetLastError();
   }
-  if (impl == GemmImpl::Exact) {
-    if (!ta && !tb) return launch_exact<false, false>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s);
-    if (!ta && tb) return launch_exact<false, true>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s);
-    if (ta && !tb) return launch_exact<true, false>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s);
-    return launch_exact<true, true>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s);
-  }
-  if (!ta && !tb) return launch_tiled<false, false>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s);
-  if (!ta && tb) return launch_tiled<false, true>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s);
-  if (ta && !tb) return launch_tiled<true, false>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s);
-  return launch_tiled<true, true>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s);
+  int tile = o.tile ? o.tile : pick_tile(m, n, runtime().cus);
+  if (o.skip_zero_b)
+    return dispatch<true>(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, impl, o.tri, o.stop, tile);
+  return dispatch<false>(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, impl, o.tri, o.stop, tile);
 }
 
 }  // namespace pga

@@ -25,16 +25,40 @@ namespace gemm {
 
 using dec::Val;
 
+// Single-instruction helpers.  The compiler would otherwise expand min(x, 1) into a
+// compare and a select, (1 << n) - 1 into a shift and a not, and a zero test around the
+// leading-zero count.
+PGA_D uint32_t ffbh_raw(uint32_t x) {  // leading zeros; all ones for x == 0
+  uint32_t r;
+  asm("v_ffbh_u32 %0, %1" : "=v"(r) : "v"(x));
+  return r;
+}
+PGA_D uint32_t nonzero_bit(uint32_t x) {  // x != 0 ? 1 : 0
+  uint32_t r;
+  asm("v_min_u32 %0, 1, %1" : "=v"(r) : "v"(x));
+  return r;
+}
+PGA_D uint32_t low_mask(uint32_t n) {  // (1 << n) - 1 for n in [0, 31]
+  uint32_t r;
+  asm("v_bfm_b32 %0, %1, 0" : "=v"(r) : "v"(n));
+  return r;
+}
+PGA_D uint32_t clear_mask(uint32_t mask, uint32_t x) {  // x & ~mask
+  uint32_t r;
+  asm("v_bfi_b32 %0, %1, 0, %2" : "=v"(r) : "v"(mask), "v"(x));
+  return r;
+}
+
 // Rounds y (hidden bit at 30; bits >= 2 exact, bits 1..0 sticky) to the precision of
 // scale `sc`: 28 - reg fraction bits, round to nearest, ties to even.  Valid for
 // FAST_MIN <= sc <= FAST_MAX; for other scales the result is unspecified (but
 // computed without undefined behaviour).  May return exactly 2^31 on a carry.
 PGA_D uint32_t round_to_scale(uint32_t y, int sc) {
   int k = sc >> 2;
-  uint32_t sh = (uint32_t)((k ^ (k >> 31)) + 3) & 31u;  // dropped bits: reg + 2
-  uint32_t mask = (1u << sh) - 1u;
-  uint32_t lsb = (y >> sh) & 1u;
-  return (y + (mask >> 1) + lsb) & ~mask;
+  uint32_t sh = (uint32_t)((k ^ (k >> 31)) + 3);  // dropped bits: reg + 2 (< 32 in range)
+  uint32_t low = low_mask(sh);                     // hardware uses sh mod 32
+  uint32_t lsb = __builtin_amdgcn_ubfe(y, sh, 1);  // last kept bit (tie breaker)
+  return clear_mask(low, y + (low >> 1) + lsb);
 }
 
 // c <- round(c + round(t * a)) in the decoded domain.  Returns true when the sum fell
@@ -48,7 +72,7 @@ PGA_D bool mac(Val& c, uint32_t tm, int ts, uint32_t tg, uint32_t am, int as, ui
   uint32_t lo = tm * am;
   uint32_t cy = hi >> 29;
   int ps = ts + as + (int)cy;
-  uint32_t pm = (hi << (2u - cy)) | (lo != 0u ? 1u : 0u);  // hidden at 30, sticky in bit 0
+  uint32_t pm = (hi << (2u - cy)) | nonzero_bit(lo);  // hidden at 30, sticky in bit 0
   uint32_t pg = tg ^ ag;
   pm = round_to_scale(pm, ps);
   uint32_t ov = pm >> 31;  // renormalise a rounding carry
@@ -56,23 +80,21 @@ PGA_D bool mac(Val& c, uint32_t tm, int ts, uint32_t tg, uint32_t am, int as, ui
   ps += (int)ov;
 
   // ---- sum c + p ----
-  // order by magnitude with one 64-bit compare of (scale, mantissa); scales are offset
-  // so that every scale that can occur (zeros included) is non-negative
-  uint64_t kp = ((uint64_t)(uint32_t)(ps + (1 << 28)) << 32) | pm;
-  uint64_t kc = ((uint64_t)(uint32_t)(c.sc + (1 << 28)) << 32) | c.m;
+  // order by magnitude with one signed 64-bit compare of (scale, mantissa)
+  int64_t kp = (int64_t)(((uint64_t)(uint32_t)ps << 32) | pm);
+  int64_t kc = (int64_t)(((uint64_t)(uint32_t)c.sc << 32) | c.m);
   bool pbig = kp > kc;
   uint32_t mb = pbig ? pm : c.m;
   uint32_t ms = pbig ? c.m : pm;
   int sb = pbig ? ps : c.sc;
   int ss = pbig ? c.sc : ps;
   uint32_t gb = pbig ? pg : c.sg;
-  uint32_t d = (uint32_t)(sb - ss);
-  d = d < 31u ? d : 31u;
-  uint32_t lost = ms & ((1u << d) - 1u);
-  uint32_t al = (ms >> d) | (lost != 0u ? 1u : 0u);
+  uint32_t d = min((uint32_t)(sb - ss), 31u);
+  uint32_t lost = __builtin_amdgcn_ubfe(ms, 0, d);
+  uint32_t al = (ms >> d) | nonzero_bit(lost);
   uint32_t x = ((pg ^ c.sg) != 0u) ? mb - al : mb + al;
   // leading zeros; a zero sum gets a huge count so its scale lands far below zero
-  uint32_t lz = x ? (uint32_t)__builtin_clz(x) : (1u << 21);
+  uint32_t lz = min(ffbh_raw(x), 1u << 21);
   int sn = sb + 1 - (int)lz;
   uint32_t z = x << (lz & 31u);
   uint32_t y = (z >> 1) | (z & 1u);
@@ -81,6 +103,38 @@ PGA_D bool mac(Val& c, uint32_t tm, int ts, uint32_t tg, uint32_t am, int as, ui
   c.sg = gb;
   // a nonzero result below the fast range (zeros are below ZERO_LIMIT)
   return (uint32_t)(sn - dec::ZERO_LIMIT) < (uint32_t)(dec::FAST_MIN - dec::ZERO_LIMIT);
+}
+
+// c <- round(c + p) for decoded values, exact in every case: the decoded sum when both
+// operands are finite and the result stays in the fast range, the pattern-level
+// SoftPosit addition otherwise.
+PGA_D void add_checked(Val& c, const Val& p) {
+  if (max(c.sc, p.sc) < dec::FAST_MAX) {
+    Val t = c;
+    uint32_t mb, ms, gb;
+    int64_t kp = (int64_t)(((uint64_t)(uint32_t)p.sc << 32) | p.m);
+    int64_t kc = (int64_t)(((uint64_t)(uint32_t)c.sc << 32) | c.m);
+    bool pbig = kp > kc;
+    mb = pbig ? p.m : c.m;
+    ms = pbig ? c.m : p.m;
+    int sb = pbig ? p.sc : c.sc, ss = pbig ? c.sc : p.sc;
+    gb = pbig ? p.sg : c.sg;
+    uint32_t d = min((uint32_t)(sb - ss), 31u);
+    uint32_t al = (ms >> d) | nonzero_bit(__builtin_amdgcn_ubfe(ms, 0, d));
+    uint32_t x = ((p.sg ^ c.sg) != 0u) ? mb - al : mb + al;
+    uint32_t lz = min(ffbh_raw(x), 1u << 21);
+    int sn = sb + 1 - (int)lz;
+    uint32_t z = x << (lz & 31u);
+    t.m = round_to_scale((z >> 1) | (z & 1u), sn);
+    t.sc = sn;
+    t.sg = gb;
+    if ((uint32_t)(sn - dec::ZERO_LIMIT) >= (uint32_t)(dec::FAST_MIN - dec::ZERO_LIMIT)) {
+      if (sn < dec::ZERO_LIMIT) t.sc = dec::ZERO_SC;
+      c = t;
+      return;
+    }
+  }
+  c = dec::decode(pga::add(dec::encode(c), dec::encode(p)));
 }
 
 // Largest accumulator / product scale for which BK more steps cannot pass FAST_MAX:

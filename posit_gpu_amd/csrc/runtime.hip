@@ -159,6 +159,10 @@ pga_status pga_init(int device) {
   PGA_HIP(hipSetDevice(device));
   PGA_HIP(hipStreamCreateWithFlags(&g_rt.stream, hipStreamNonBlocking));
   g_rt.device = device;
+  {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) g_rt.cus = prop.multiProcessorCount;
+  }
   const char* impl = getenv("PGA_GEMM_IMPL");
   if (impl && strcmp(impl, "exact") == 0) g_rt.gemm_impl = GemmImpl::Exact;
   g_rt.initialized = true;
@@ -402,5 +406,13 @@ pga_status pga_fill_uniform_p32_device(uint32_t* p, size_t n, uint64_t seed, dou
   PGA_HIP(hipGetLastError());
   return PGA_SUCCESS;
 }
+
+uint32_t pga_p32_add(uint32_t a, uint32_t b) { return pga::add(a, b); }
+uint32_t pga_p32_sub(uint32_t a, uint32_t b) { return pga::sub(a, b); }
+uint32_t pga_p32_mul(uint32_t a, uint32_t b) { return pga::mul(a, b); }
+uint32_t pga_p32_div(uint32_t a, uint32_t b) { return pga::div(a, b); }
+uint32_t pga_p32_sqrt(uint32_t a) { return pga::sqrt(a); }
+uint32_t pga_p32_from_double(double x) { return pga::from_double(x); }
+double pga_p32_to_double(uint32_t a) { return pga::to_double(a); }
 
 }  // extern "C"

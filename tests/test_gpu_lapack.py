@@ -1,0 +1,251 @@
+"""GPU parity of the posit32 BLAS-2/3 and LU / Cholesky routines (through the C ABI)
+against the oracle's restatements of the reference routines.  Bit-exact: factors,
+pivots, info codes and solutions."""
+import os
+
+import numpy as np
+import pytest
+
+from datagen import posit_matrix, uniform_doubles
+
+pytestmark = pytest.mark.gpu
+
+ONE, MONE, NAR = 0x40000000, 0xC0000000, 0x80000000
+THREADS = min(16, os.cpu_count() or 1)
+
+
+def _same(got, want):
+    bad = np.argwhere(np.asarray(got) != np.asarray(want))
+    assert bad.size == 0, f"{len(bad)} mismatches, first {bad[:4].tolist()}"
+
+
+def _tri(oracle, seed, n, upper, unit_scale=True):
+    """Well-conditioned triangular posit matrix (dominant diagonal)."""
+    T = posit_matrix(oracle, seed, n, n, -0.5, 0.5)
+    d = oracle.from_double_v(np.linspace(1.0, 2.0, n))
+    for i in range(n):
+        T[i, i] = d[i]
+    return T
+
+
+@pytest.mark.parametrize("side", ["L", "R"])
+@pytest.mark.parametrize("uplo", ["U", "L"])
+@pytest.mark.parametrize("transa", ["N", "T"])
+@pytest.mark.parametrize("diag", ["N", "U"])
+@pytest.mark.parametrize("alpha", [ONE, 0x3A000000, 0])
+def test_trsm(pga, oracle, side, uplo, transa, diag, alpha):
+    m, n = 37, 23
+    na = m if side == "L" else n
+    A = _tri(oracle, 5, na, uplo == "U")
+    B = posit_matrix(oracle, 6, m, n)
+    B[3, :] = 0  # zero right-hand-side entries exercise the reference's skips
+    want = B.copy(order="F")
+    assert oracle.rtrsm(side, uplo, transa, diag, m, n, alpha, A, want) == 0
+    got = B.copy(order="F")
+    pga.Rtrsm(side, uplo, transa, diag, m, n, alpha, A, na, got, m)
+    _same(got, want)
+
+
+@pytest.mark.parametrize("uplo", ["U", "L"])
+@pytest.mark.parametrize("trans", ["N", "T"])
+@pytest.mark.parametrize("alpha,beta", [(MONE, ONE), (0x3C000000, 0), (0, 0x44000000)])
+def test_syrk(pga, oracle, uplo, trans, alpha, beta):
+    n, k = 45, 70
+    A = posit_matrix(oracle, 7, n if trans == "N" else k, k if trans == "N" else n)
+    A[:, 5] = 0  # zero multipliers are skipped in the 'N' form
+    C = posit_matrix(oracle, 8, n, n)
+    want = C.copy(order="F")
+    assert oracle.rsyrk(uplo, trans, n, k, alpha, A, beta, want) == 0
+    got = C.copy(order="F")
+    pga.Rsyrk(uplo, trans, n, k, alpha, A, A.shape[0], beta, got, n)
+    _same(got, want)
+
+
+@pytest.mark.parametrize("trans", ["N", "T"])
+@pytest.mark.parametrize("incx,incy", [(1, 1), (2, -1), (-3, 2)])
+def test_gemv(pga, oracle, trans, incx, incy):
+    m, n = 50, 33
+    A = posit_matrix(oracle, 9, m, n)
+    lenx, leny = (n, m) if trans == "N" else (m, n)
+    x = posit_matrix(oracle, 10, (lenx - 1) * abs(incx) + 1, 1)[:, 0].copy()
+    y = posit_matrix(oracle, 11, (leny - 1) * abs(incy) + 1, 1)[:, 0].copy()
+    for alpha, beta in [(ONE, 0), (MONE, ONE), (0x3A000000, 0xC4000000)]:
+        want = y.copy()
+        assert oracle.rgemv(trans, m, n, alpha, A, x, incx, beta, want, incy) == 0
+        got = y.copy()
+        pga.Rgemv(trans, m, n, alpha, A, m, x, incx, beta, got, incy)
+        _same(got, want)
+
+
+def test_scal_iamax_laswp(pga, oracle):
+    rng = np.random.default_rng(3)
+    x = posit_matrix(oracle, 12, 1000, 1)[:, 0].copy()
+    want = x.copy()
+    oracle.L.orc_rscal(1000 // 3, 0x3A000000, want.ctypes.data, 3)
+    got = x.copy()
+    pga.Rscal(1000 // 3, 0x3A000000, got, 3)
+    _same(got, want)
+    for trial in range(20):
+        v = x.copy()
+        if trial % 3 == 0:
+            v[rng.integers(0, v.size, 4)] = NAR
+        if trial % 3 == 1:  # ties: the first maximum wins
+            v[[100, 500, 900]] = v[np.argmax(np.abs(oracle.to_double_v(v)))]
+        inc = 1 + trial % 2
+        n = v.size // inc
+        assert pga.iRamax(n, v, inc) == oracle.iramax(v, inc, n)
+    A = posit_matrix(oracle, 13, 60, 17)
+    ipiv = rng.integers(1, 61, 20).astype(np.int32)
+    ipiv = np.maximum(ipiv, np.arange(1, 21, dtype=np.int32))
+    for incx in (1, -1):
+        want = A.copy(order="F")
+        oracle.L.orc_rlaswp(17, want.ctypes.data, 60, 3, 18, ipiv.ctypes.data, incx)
+        got = A.copy(order="F")
+        pga.Rlaswp(17, got, 60, 3, 18, ipiv, incx)
+        _same(got, want)
+
+
+@pytest.mark.parametrize("m,n", [(150, 150), (200, 120), (90, 160)])
+@pytest.mark.parametrize("nb", [0, 1, 16, 64])
+def test_getrf(pga, oracle, m, n, nb):
+    A = posit_matrix(oracle, 20 + m + n, m, n)
+    want = A.copy(order="F")
+    _, wpiv, winfo = oracle.rgetrf(want, nb=nb, nthreads=THREADS)
+    got = A.copy(order="F")
+    ipiv = np.zeros(min(m, n), np.int32)
+    info = pga.Rgetrf_opencl(m, n, got, m, ipiv, nb)
+    assert info == winfo
+    _same(ipiv, wpiv)
+    _same(got, want)
+
+
+@pytest.mark.parametrize("kind", ["zero_column", "nar", "rank_deficient"])
+def test_getrf_special(pga, oracle, kind):
+    n = 96
+    A = posit_matrix(oracle, 30, n, n)
+    if kind == "zero_column":
+        A[:, 40] = 0
+    elif kind == "nar":
+        A[7, 3] = NAR
+        A[50, 60] = NAR
+    else:
+        A[:, 10] = A[:, 20]
+    for nb in (0, 8, 32):
+        want = A.copy(order="F")
+        _, wpiv, winfo = oracle.rgetrf(want, nb=nb, nthreads=THREADS)
+        got = A.copy(order="F")
+        ipiv = np.zeros(n, np.int32)
+        info = pga.Rgetrf_opencl(n, n, got, n, ipiv, nb)
+        assert info == winfo, (nb, info, winfo)
+        _same(ipiv, wpiv)
+        _same(got, want)
+
+
+def test_getrf2_and_getrs(pga, oracle):
+    n, nrhs = 130, 5
+    A = posit_matrix(oracle, 40, n, n)
+    want = A.copy(order="F")
+    _, wpiv, winfo = oracle.rgetrf2(want, nthreads=THREADS)
+    got = A.copy(order="F")
+    ipiv = np.zeros(n, np.int32)
+    assert pga.Rgetrf2(n, n, got, n, ipiv) == winfo
+    _same(ipiv, wpiv)
+    _same(got, want)
+    B = posit_matrix(oracle, 41, n, nrhs)
+    for trans in ("N", "T"):
+        wb = B.copy(order="F")
+        assert oracle.rgetrs(trans, want, wpiv, wb) == (0, 0)
+        gb = B.copy(order="F")
+        assert pga.Rgetrs(trans, n, nrhs, got, n, ipiv, gb, n) == 0
+        _same(gb, wb)
+
+
+def _spd(oracle, seed, n):
+    M = uniform_doubles(seed, n * n).reshape(n, n)
+    S = M @ M.T / n + np.eye(n)
+    return np.asfortranarray(oracle.from_double_v(S.ravel(order="F")).reshape(n, n, order="F"))
+
+
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("nb", [0, 1, 16, 40])
+def test_potrf_potrs(pga, oracle, uplo, nb):
+    n, nrhs = 150, 3
+    A = _spd(oracle, 50, n)
+    want = A.copy(order="F")
+    assert oracle.rpotrf(uplo, want, nb=nb, nthreads=THREADS) == (0, 0)
+    got = A.copy(order="F")
+    assert pga.Rpotrf_opencl(uplo, n, got, n, nb) == 0
+    _same(got, want)
+    B = posit_matrix(oracle, 51, n, nrhs)
+    wb = B.copy(order="F")
+    assert oracle.rpotrs(uplo, want, wb) == (0, 0)
+    gb = B.copy(order="F")
+    assert pga.Rpotrs(uplo, n, nrhs, got, n, gb, n) == 0
+    _same(gb, wb)
+
+
+@pytest.mark.parametrize("uplo", ["L", "U"])
+def test_potrf_not_pd(pga, oracle, uplo):
+    n = 100
+    A = _spd(oracle, 52, n)
+    A[61, 61] = oracle.from_double(-2.0)
+    for nb in (0, 1, 16):
+        want = A.copy(order="F")
+        _, winfo = oracle.rpotrf(uplo, want, nb=nb, nthreads=THREADS)
+        got = A.copy(order="F")
+        info = pga.Rpotrf_opencl(uplo, n, got, n, nb)
+        assert info == winfo and info > 0
+        _same(got, want)  # the factorization stops at the same place
+
+
+def test_potrf2(pga, oracle):
+    n = 77
+    A = _spd(oracle, 53, n)
+    for uplo in "LU":
+        want = A.copy(order="F")
+        assert oracle.rpotrf2(uplo, want) == (0, 0)
+        got = A.copy(order="F")
+        assert pga.Rpotrf2(uplo, n, got, n) == 0
+        _same(got, want)
+
+
+def test_lu_1024_nb256_device(pga, oracle):
+    """Device-resident blocked LU (config-3 blocking) at n = 1024, bit-exact, plus the
+    solve with b = A*ones built by the posit GEMV."""
+    n, nb = 1024, 256
+    A = posit_matrix(oracle, 60, n, n)
+    dA = pga.DeviceArray.from_host(A)
+    dp = pga.DeviceArray(n, np.int32, "C")
+    di = pga.DeviceArray(1, np.int32, "C")
+    pga.getrf_device(n, n, dA, n, dp, di, nb)
+    pga.synchronize()
+    want = A.copy(order="F")
+    _, wpiv, winfo = oracle.rgetrf(want, nb=nb, nthreads=THREADS)
+    assert int(di.to_host()[0]) == winfo
+    _same(dp.to_host(), wpiv)
+    _same(dA.to_host(), want)
+    ones = np.full(n, ONE, np.uint32)
+    b = np.zeros(n, np.uint32)
+    oracle.rgemv("N", n, n, ONE, A, ones, 1, 0, b, 1)
+    gb = b.copy()
+    pga.Rgemv("N", n, n, ONE, A, n, ones, 1, 0, gb, 1)
+    _same(gb, b)
+    B = np.asfortranarray(b.reshape(n, 1))
+    oracle.rgetrs("N", want, wpiv, B)
+    dB = pga.DeviceArray.from_host(b.reshape(n, 1))
+    pga.getrs_device("N", n, 1, dA, n, dp, dB, n)
+    pga.synchronize()
+    _same(dB.to_host(), B)
+
+
+def test_cholesky_1024_device(pga, oracle):
+    n = 1024
+    A = _spd(oracle, 61, n)
+    dA = pga.DeviceArray.from_host(A)
+    di = pga.DeviceArray(1, np.int32, "C")
+    pga.potrf_device("L", n, dA, n, di, 0)
+    pga.synchronize()
+    want = A.copy(order="F")
+    assert oracle.rpotrf("L", want, nb=0, nthreads=THREADS) == (0, 0)
+    assert int(di.to_host()[0]) == 0
+    _same(np.tril(dA.to_host()), np.tril(want))
