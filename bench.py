@@ -76,6 +76,12 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--config", default="gemm", choices=["gemm", "lu", "cholesky", "batched", "copycal"],
+                    help="gemm: trailing-update GEMMs (headline); lu / cholesky: blocked "
+                         "factorization + solve at n=8192; batched: 4096 small GEMMs; copycal: a "
+                         "streaming kernel of known size (2 x 1 GiB read, 1 GiB written) that "
+                         "calibrates the FETCH_SIZE / WRITE_SIZE counters")
+    ap.add_argument("--n", type=int, default=8192, help="order for --config lu / cholesky")
     ap.add_argument("--only-shape", type=int, default=0,
                     help="run only the M=N=<value> GEMM (used for per-dispatch profiling)")
     args = ap.parse_args()
@@ -92,6 +98,14 @@ def main():
     pga.init(0)
     info = pga.device_info()
     log(f"device: {info['name']} {info['arch']} CUs={info['compute_units']} clock={info['clock_khz']} kHz")
+    if args.config == "lu":
+        return bench_lu(pga, info, args)
+    if args.config == "cholesky":
+        return bench_cholesky(pga, info, args)
+    if args.config == "batched":
+        return bench_batched(pga, info, args)
+    if args.config == "copycal":
+        return bench_copycal(pga, info, args)
 
     bufs = []
     for i, n in enumerate(SHAPES):
@@ -182,6 +196,186 @@ def main():
         out["cpu_baseline"] = cpu_baseline()
     else:
         out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+def _timed(pga, fn):
+    e0, e1 = pga.Event(), pga.Event()
+    e0.record()
+    fn()
+    e1.record()
+    e1.synchronize()
+    return e0.elapsed_ms(e1) / 1e3
+
+
+def _oracle():
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_lib import Oracle
+
+    return Oracle()
+
+
+def bench_lu(pga, info, args):
+    """Rgetrf_opencl (nb=256) + Rgetrs on the device, b = A*ones by the posit GEMV;
+    block-size invariance (nb=64 gives the same bits) as the full-size parity check."""
+    n, nb = args.n, 256
+    dA0 = pga.DeviceArray((n, n))
+    pga.fill_uniform(dA0, 4242)
+    dA = pga.DeviceArray((n, n))
+    dA.copy_from(dA0)
+    ones = pga.DeviceArray.from_host(np.full(n, P_ONE, np.uint32))
+    db = pga.DeviceArray(n, np.uint32, "C")
+    pga.gemv_device("N", n, n, P_ONE, dA0, n, ones, 1, 0, db, 1)
+    dx = pga.DeviceArray((n, 1))
+    dx.copy_from(db)
+    dp = pga.DeviceArray(n, np.int32, "C")
+    di = pga.DeviceArray(1, np.int32, "C")
+    pga.synchronize()
+    t_f = _timed(pga, lambda: pga.getrf_device(n, n, dA, n, dp, di, nb))
+    t_s = _timed(pga, lambda: pga.getrs_device("N", n, 1, dA, n, dp, dx, n))
+    berr = pga.backward_error_device(n, dA0, n, dx, db)
+    info_v = int(di.to_host()[0])
+    F1 = dA.to_host()
+    dA.copy_from(dA0)
+    pga.getrf_device(n, n, dA, n, dp, di, 64)
+    pga.synchronize()
+    same = bool(np.array_equal(F1, dA.to_host()))
+    del F1
+    ops = 2.0 / 3.0 * n ** 3
+    cpu = None
+    if not args.no_cpu_baseline:
+        # the oracle's blocked LU (port of Rgetrf_opencl / Rgetrf2) at n_cpu
+        O = _oracle()
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from datagen import posit_matrix
+
+        cores = min(16, os.cpu_count() or 1)
+        n_cpu = 768
+        Ac = posit_matrix(O, 4243, n_cpu, n_cpu)
+        t0 = time.perf_counter()
+        O.rgetrf(Ac, nb=64, nthreads=cores)
+        t_cpu = time.perf_counter() - t0
+        cpu = {"value": round(2.0 / 3.0 * n_cpu ** 3 / t_cpu / 1e9, 4), "unit": "Gop/s", "cores": cores,
+               "kind": "port", "sample": f"oracle Rgetrf n={n_cpu} nb=64, {t_cpu:.2f} s"}
+    out = {
+        "metric": "posit32 LU Gop/s ((2/3)n^3/t)", "value": round(ops / t_f / 1e9, 2), "unit": "Gop/s",
+        "n_gpus": 1, "higher_is_better": True, "dtype": "posit32 (integer VALU)",
+        "data": "synthetic: seeded uniform[-1,1) FP64 -> posit32, generated on the device",
+        "config": {"workload": f"Rgetrf_opencl n={n} nb={nb} + Rgetrs (b = A*ones)", "factor_s": round(t_f, 4),
+                   "solve_s": round(t_s, 4), "info": info_v, "backward_error": berr,
+                   "nb256_equals_nb64": same, "device": info["name"]},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_cholesky(pga, info, args):
+    """Rpotrf_opencl (lower, nb=64) + Rpotrs on A = M*M^T/n + I (M uniform[-1,1))."""
+    n = args.n
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from datagen import uniform_doubles
+
+    M = uniform_doubles(4300, n * n).reshape(n, n)
+    S = M @ M.T
+    S /= n
+    S[np.diag_indices(n)] += 1.0
+    del M
+    dS = pga.DeviceArray.from_host(np.asfortranarray(S))
+    del S
+    dA0 = pga.DeviceArray((n, n))
+    pga._check(pga._lib.pga_p32_from_f64_device(dS.ptr, dA0.ptr, n * n, None), "from_f64")
+    dS.free()
+    dA = pga.DeviceArray((n, n))
+    dA.copy_from(dA0)
+    ones = pga.DeviceArray.from_host(np.full(n, P_ONE, np.uint32))
+    db = pga.DeviceArray(n, np.uint32, "C")
+    pga.gemv_device("N", n, n, P_ONE, dA0, n, ones, 1, 0, db, 1)
+    dx = pga.DeviceArray((n, 1))
+    dx.copy_from(db)
+    di = pga.DeviceArray(1, np.int32, "C")
+    pga.synchronize()
+    t_f = _timed(pga, lambda: pga.potrf_device("L", n, dA, n, di, 0))
+    t_s = _timed(pga, lambda: pga.potrs_device("L", n, 1, dA, n, dx, n))
+    berr = pga.backward_error_device(n, dA0, n, dx, db)
+    info_v = int(di.to_host()[0])
+    L1 = np.tril(dA.to_host())
+    dA.copy_from(dA0)
+    pga.potrf_device("L", n, dA, n, di, 128)
+    pga.synchronize()
+    same = bool(np.array_equal(L1, np.tril(dA.to_host())))
+    del L1
+    ops = n ** 3 / 3.0
+    out = {
+        "metric": "posit32 Cholesky Gop/s ((1/3)n^3/t)", "value": round(ops / t_f / 1e9, 2), "unit": "Gop/s",
+        "n_gpus": 1, "higher_is_better": True, "dtype": "posit32 (integer VALU)",
+        "data": "synthetic: M*M^T/n + I with M seeded uniform[-1,1), FP64 -> posit32",
+        "config": {"workload": f"Rpotrf_opencl lower n={n} nb=64 + Rpotrs", "factor_s": round(t_f, 4),
+                   "solve_s": round(t_s, 4), "info": info_v, "backward_error": berr,
+                   "lower_nb64_equals_nb128": same, "device": info["name"]},
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_batched(pga, info, args):
+    """4096 independent GEMMs, M, N, K uniform on multiples of 16 in [16, 512], alpha=beta=1,
+    one persistent launch (autotuned) captured in a hipGraph, vs per-call launches."""
+    rng = np.random.default_rng(4096)
+    count = 4096
+    shapes = rng.integers(1, 33, (count, 3)) * 16
+    probs, bufs = [], []
+    total_ops = 0.0
+    for i, (m, n, k) in enumerate(shapes):
+        m, n, k = int(m), int(n), int(k)
+        dA, dB, dC = pga.DeviceArray((m, k)), pga.DeviceArray((k, n)), pga.DeviceArray((m, n))
+        pga.fill_uniform(dA, 10 * i + 1)
+        pga.fill_uniform(dB, 10 * i + 2)
+        pga.fill_uniform(dC, 10 * i + 3)
+        probs.append(dict(m=m, n=n, k=k, alpha=P_ONE, beta=P_ONE, a=dA, lda=m, b=dB, ldb=k, c=dC, ldc=m))
+        bufs.append((dA, dB, dC))
+        total_ops += 2.0 * m * n * k
+    pga.synchronize()
+    t0 = time.perf_counter()
+    plan = pga.BatchPlan("N", "N", probs, tile=0)
+    t_tune = time.perf_counter() - t0
+    plan.capture_graph()
+    plan.launch_graph()
+    pga.synchronize()
+    reps = max(1, args.steps)
+    t_graph = min(_timed(pga, plan.launch_graph) for _ in range(reps))
+    def per_call():
+        for p in probs:
+            pga.rgemm_device("N", "N", p["m"], p["n"], p["k"], P_ONE, p["a"], p["lda"], p["b"], p["ldb"], P_ONE,
+                             p["c"], p["ldc"])
+    t_plain = min(_timed(pga, per_call) for _ in range(max(1, reps // 2)))
+    out = {
+        "metric": "batched posit32 GEMM Gop/s (sum 2MNK/t)", "value": round(total_ops / t_graph / 1e9, 2),
+        "unit": "Gop/s", "n_gpus": 1, "higher_is_better": True, "dtype": "posit32 (integer VALU)",
+        "data": "synthetic: seeded uniform[-1,1) posit32, shapes from a seeded generator",
+        "config": {"workload": "4096 GEMMs, M,N,K uniform on {16,...,512}, alpha=beta=1", "plan": plan.info(),
+                   "graph_s": round(t_graph, 5), "per_call_s": round(t_plain, 5),
+                   "speedup_vs_per_call": round(t_plain / t_graph, 2), "plan_build_s": round(t_tune, 2),
+                   "device": info["name"]},
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_copycal(pga, info, args):
+    """o[i] = a[i] + b[i] over 2^28 posit32 words: 2 GiB fetched, 1 GiB written per launch
+    (algorithmic), the reference for scaling the HBM traffic counters."""
+    n = 1 << 28
+    da, db, do = (pga.DeviceArray(n, np.uint32, "C") for _ in range(3))
+    pga.fill_uniform(da, 1)
+    pga.fill_uniform(db, 2)
+    pga.synchronize()
+    ts = []
+    for _ in range(max(1, args.steps)):
+        ts.append(_timed(pga, lambda: pga._check(
+            pga._lib.pga_p32_elementwise_device(0, da.ptr, db.ptr, do.ptr, n, None), "elementwise")))
+    t = min(ts)
+    out = {"metric": "streaming add GB/s", "value": round(3.0 * 4 * n / t / 1e9, 1), "unit": "GB/s",
+           "n_gpus": 1, "higher_is_better": True, "config": {"workload": "posit32 add of two 1 GiB arrays",
+                                                             "read_bytes": 8 * n, "write_bytes": 4 * n,
+                                                             "ms": round(t * 1e3, 4)}}
     print(json.dumps(out), flush=True)
 
 

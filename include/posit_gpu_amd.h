@@ -155,6 +155,30 @@ pga_status pga_rpotrs(char uplo, int n, int nrhs, const uint32_t *a, int lda, ui
 pga_status pga_rpotrs_device(char uplo, int n, int nrhs, const uint32_t *a, int lda, uint32_t *b,
                              int ldb, void *stream);
 
+/* ---- batched GEMM (one persistent launch, hipGraph-capturable) -------------------
+ * A plan describes `count` independent problems C_i = alpha_i*op(A_i)*op(B_i) +
+ * beta_i*C_i (device pointers, one transa/transb for the batch) with the semantics of
+ * pga_rgemm_device for each.  Executing it launches a one-lane counter reset and one
+ * persistent kernel in which workgroups take tiles of all problems from a queue (largest
+ * problems first).  tile = 0 lets the autotuner time tile shapes {64,32,16} x grid sizes
+ * {1,2,4,8} x CUs on scratch outputs and remember the winner per batch signature (also
+ * in the file named by PGA_TUNE_CACHE); otherwise tile and grid (0 = 2 per CU) are used as
+ * given.  Problems with alpha == 0 or k == 0 and beta != 1 are rejected
+ * (PGA_ERROR_UNSUPPORTED).  The reference has no batched form: this extends RgemmG
+ * (libRpack/Rgemm_opencl.cpp:199) to many problems per launch. */
+typedef struct pga_batch_plan pga_batch_plan;
+pga_status pga_batch_plan_create(pga_batch_plan **plan, char transa, char transb, int count,
+                                 const int *m, const int *n, const int *k, const uint32_t *alpha,
+                                 const uint32_t *const *a, const int *lda,
+                                 const uint32_t *const *b, const int *ldb, const uint32_t *beta,
+                                 uint32_t *const *c, const int *ldc, int tile, int grid);
+pga_status pga_batch_plan_execute(pga_batch_plan *plan, void *stream);
+pga_status pga_batch_plan_capture_graph(pga_batch_plan *plan, void *stream);
+pga_status pga_batch_plan_launch_graph(pga_batch_plan *plan, void *stream);
+pga_status pga_batch_plan_info(const pga_batch_plan *plan, int *tile, int *grid, int *total_tiles,
+                               double *tune_ms);
+pga_status pga_batch_plan_destroy(pga_batch_plan *plan);
+
 /* ---- conversions (device codecs) -------------------------------------------------
  * double <-> posit32 on device arrays: round to nearest, ties to the even pattern,
  * saturating at minpos/maxpos; NaN and infinities become NaR.  (The reference converts

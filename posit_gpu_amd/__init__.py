@@ -469,3 +469,59 @@ def backward_error_device(n, a, lda, x, b, stream=None):
     _check(_lib.pga_backward_error_device(n, _dp(a), lda, _dp(x), _dp(b), ctypes.byref(r), stream),
            "pga_backward_error_device")
     return r.value
+
+
+# ---------------------------------------------------------------------------------
+# batched GEMM plans
+# ---------------------------------------------------------------------------------
+_PP = ctypes.POINTER(_P)
+_sig("pga_batch_plan_create", [ctypes.POINTER(_P), _C, _C, _I, _IP, _IP, _IP, ctypes.POINTER(_U32), _PP, _IP,
+                               _PP, _IP, ctypes.POINTER(_U32), _PP, _IP, _I, _I])
+_sig("pga_batch_plan_execute", [_P, _P])
+_sig("pga_batch_plan_capture_graph", [_P, _P])
+_sig("pga_batch_plan_launch_graph", [_P, _P])
+_sig("pga_batch_plan_info", [_P, _IP, _IP, _IP, ctypes.POINTER(ctypes.c_double)])
+_sig("pga_batch_plan_destroy", [_P])
+
+
+class BatchPlan:
+    """Batched GEMM C_i = alpha_i*op(A_i)*op(B_i) + beta_i*C_i over device buffers.
+
+    problems: list of dicts with keys m, n, k, alpha, beta, a, lda, b, ldb, c, ldc where
+    a, b, c are device pointers (ints) or DeviceArrays.  tile=0 autotunes."""
+
+    def __init__(self, transa, transb, problems, tile=0, grid=0):
+        init()
+        cnt = len(problems)
+        ia = lambda key: (ctypes.c_int * max(1, cnt))(*[int(p[key]) for p in problems])  # noqa: E731
+        ua = lambda key: (_U32 * max(1, cnt))(*[int(p[key]) for p in problems])  # noqa: E731
+        pa = lambda key: (_P * max(1, cnt))(*[_dp(p[key]) for p in problems])  # noqa: E731
+        self._keep = [ia("m"), ia("n"), ia("k"), ua("alpha"), pa("a"), ia("lda"), pa("b"), ia("ldb"), ua("beta"),
+                      pa("c"), ia("ldc")]
+        m, n, k, al, a, lda, b, ldb, be, c, ldc = self._keep
+        h = _P()
+        _check(_lib.pga_batch_plan_create(ctypes.byref(h), _ch(transa), _ch(transb), cnt, m, n, k, al, a, lda, b,
+                                          ldb, be, c, ldc, tile, grid), "pga_batch_plan_create")
+        self.h = h.value
+
+    def execute(self, stream=None):
+        _check(_lib.pga_batch_plan_execute(self.h, stream), "pga_batch_plan_execute")
+
+    def capture_graph(self, stream=None):
+        _check(_lib.pga_batch_plan_capture_graph(self.h, stream), "pga_batch_plan_capture_graph")
+
+    def launch_graph(self, stream=None):
+        _check(_lib.pga_batch_plan_launch_graph(self.h, stream), "pga_batch_plan_launch_graph")
+
+    def info(self):
+        t, g, n = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        ms = ctypes.c_double()
+        _check(_lib.pga_batch_plan_info(self.h, ctypes.byref(t), ctypes.byref(g), ctypes.byref(n), ctypes.byref(ms)),
+               "pga_batch_plan_info")
+        return {"tile": t.value, "grid": g.value, "total_tiles": n.value, "tune_ms": ms.value}
+
+    def __del__(self):
+        try:
+            _lib.pga_batch_plan_destroy(self.h)
+        except Exception:
+            pass

@@ -23,6 +23,15 @@ hipError_t rgemm_launch(bool ta, bool tb, int m, int n, int k, uint32_t alpha, c
                         int lda, const uint32_t* B, int ldb, uint32_t beta, uint32_t* C, int ldc,
                         hipStream_t s, GemmImpl impl, const GemmOpts& o = GemmOpts());
 
+// One problem of a batched GEMM (device-resident descriptor array).
+struct BatchProblem {
+  int m, n, k, lda, ldb, ldc;
+  uint32_t alpha, beta;
+  const uint32_t* a;
+  const uint32_t* b;
+  uint32_t* c;
+};
+
 // Runtime state (runtime.hip)
 struct Runtime {
   bool initialized = false;

@@ -14,11 +14,12 @@ namespace gemm {
 // ---------------------------------------------------------------------------------
 // Tiled kernel
 // ---------------------------------------------------------------------------------
+// One BM x BN tile of C at (i0, j0): the body shared by the plain and the batched kernel.
 template <int BM, int BN, int BK, int TM, int TN, bool TA, bool TB, bool SKIP0>
-__global__ __launch_bounds__((BM / TM) * (BN / TN)) void rgemm_tiled(
-    int m, int n, int k, uint32_t alpha, const uint32_t* __restrict__ A, int lda,
-    const uint32_t* __restrict__ B, int ldb, uint32_t beta, uint32_t* __restrict__ C, int ldc,
-    int tiles_m, int tri, const int* __restrict__ stop) {
+__device__ __forceinline__ void gemm_tile(int m, int n, int k, uint32_t alpha,
+                                          const uint32_t* __restrict__ A, int lda,
+                                          const uint32_t* __restrict__ B, int ldb, uint32_t beta,
+                                          uint32_t* __restrict__ C, int ldc, int i0, int j0, int tri) {
   constexpr int NX = BM / TM;
   constexpr int NY = BN / TN;
   constexpr int NT = NX * NY;
@@ -40,14 +41,6 @@ __global__ __launch_bounds__((BM / TM) * (BN / TN)) void rgemm_tiled(
   const int tx = tid % NX;
   const int ty = tid / NX;
   const int wave = tid / 64;
-  // tiles in column-major order of the tile grid: consecutive workgroups share B slices
-  const int bid = blockIdx.x;
-  const int i0 = (bid % tiles_m) * BM;
-  const int j0 = (bid / tiles_m) * BN;
-  if (stop && *stop) return;  // an earlier step of a factorization failed
-  // triangular update (Rsyrk): skip tiles with no element in the stored triangle
-  if (tri == 1 && i0 + BM - 1 < j0) return;
-  if (tri == 2 && j0 + BN - 1 < i0) return;
 
   // ---- accumulators ----
   Val acc[TM][TN];
@@ -233,6 +226,56 @@ __global__ __launch_bounds__((BM / TM) * (BN / TN)) void rgemm_tiled(
   }
 }
 
+template <int BM, int BN, int BK, int TM, int TN, bool TA, bool TB, bool SKIP0>
+__global__ __launch_bounds__((BM / TM) * (BN / TN)) void rgemm_tiled(
+    int m, int n, int k, uint32_t alpha, const uint32_t* __restrict__ A, int lda,
+    const uint32_t* __restrict__ B, int ldb, uint32_t beta, uint32_t* __restrict__ C, int ldc,
+    int tiles_m, int tri, const int* __restrict__ stop) {
+  // tiles in column-major order of the tile grid: consecutive workgroups share B slices
+  const int bid = blockIdx.x;
+  const int i0 = (bid % tiles_m) * BM;
+  const int j0 = (bid / tiles_m) * BN;
+  if (stop && *stop) return;  // an earlier step of a factorization failed
+  // triangular update (Rsyrk): skip tiles with no element in the stored triangle
+  if (tri == 1 && i0 + BM - 1 < j0) return;
+  if (tri == 2 && j0 + BN - 1 < i0) return;
+  gemm_tile<BM, BN, BK, TM, TN, TA, TB, SKIP0>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, i0, j0, tri);
+}
+
+// Batched form: one persistent launch works through the tiles of many independent
+// problems, taking them from a global counter (problems are listed in the order the
+// caller wants them scheduled, e.g. most expensive first).  Each tile is computed by the
+// same gemm_tile as the plain kernel, so every element gets the same bits as a plain call.
+template <int BM, int BN, int BK, int TM, int TN, bool TA, bool TB>
+__global__ __launch_bounds__((BM / TM) * (BN / TN)) void rgemm_batched_persistent(
+    const BatchProblem* __restrict__ probs, const int* __restrict__ tile_start, int nprob, int total_tiles,
+    int* __restrict__ counter) {
+  __shared__ int s_tile;
+  for (;;) {
+    if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1);
+    __syncthreads();
+    const int t = s_tile;
+    __syncthreads();
+    if (t >= total_tiles) return;  // every workgroup ends here once the queue is empty
+    // problem owning tile t: last p with tile_start[p] <= t
+    int lo = 0, hi = nprob - 1;
+    while (lo < hi) {
+      int mid = (lo + hi + 1) >> 1;
+      if (tile_start[mid] <= t) lo = mid;
+      else hi = mid - 1;
+    }
+    const BatchProblem& p = probs[lo];
+    const int tl = t - tile_start[lo];
+    const int tm = (p.m + BM - 1) / BM;
+    gemm_tile<BM, BN, BK, TM, TN, TA, TB, false>(p.m, p.n, p.k, p.alpha, p.a, p.lda, p.b, p.ldb, p.beta, p.c,
+                                                 p.ldc, (tl % tm) * BM, (tl / tm) * BN, 0);
+  }
+}
+
+__global__ void reset_counter(int* counter) {
+  if (threadIdx.x == 0) *counter = 0;
+}
+
 // ---------------------------------------------------------------------------------
 // Exact kernel: one element per thread, pattern-level operations only.  Used for
 // validation of the tiled kernel and selectable at run time.
@@ -344,7 +387,37 @@ static hipError_t dispatch(bool ta, bool tb, int m, int n, int k, uint32_t alpha
   return launch_tiled<true, true, SKIP0>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, tri, stop, tile);
 }
 
+template <int BM, int TM, bool TA, bool TB>
+static hipError_t launch_persistent(int grid, const BatchProblem* probs, const int* tile_start, int nprob,
+                                    int total_tiles, int* counter, hipStream_t s) {
+  constexpr int NT = (BM / TM) * (BM / TM);
+  hipLaunchKernelGGL(reset_counter, dim3(1), dim3(64), 0, s, counter);
+  hipLaunchKernelGGL((rgemm_batched_persistent<BM, BM, 16, TM, TM, TA, TB>), dim3(grid), dim3(NT), 0, s, probs,
+                     tile_start, nprob, total_tiles, counter);
+  return hipGetLastError();
+}
+
+template <bool TA, bool TB>
+static hipError_t launch_persistent_tile(int tile, int grid, const BatchProblem* probs, const int* tile_start,
+                                         int nprob, int total_tiles, int* counter, hipStream_t s) {
+  if (tile == 64) return launch_persistent<64, 4, TA, TB>(grid, probs, tile_start, nprob, total_tiles, counter, s);
+  if (tile == 32) return launch_persistent<32, 2, TA, TB>(grid, probs, tile_start, nprob, total_tiles, counter, s);
+  return launch_persistent<16, 1, TA, TB>(grid, probs, tile_start, nprob, total_tiles, counter, s);
+}
+
 }  // namespace gemm
+
+int batched_tiles(int m, int n, int tile) { return ((m + tile - 1) / tile) * ((n + tile - 1) / tile); }
+
+hipError_t batched_launch(bool ta, bool tb, int tile, int grid, const BatchProblem* probs, const int* tile_start,
+                          int nprob, int total_tiles, int* counter, hipStream_t s) {
+  using namespace gemm;
+  if (grid < 1) grid = 1;
+  if (!ta && !tb) return launch_persistent_tile<false, false>(tile, grid, probs, tile_start, nprob, total_tiles, counter, s);
+  if (!ta && tb) return launch_persistent_tile<false, true>(tile, grid, probs, tile_start, nprob, total_tiles, counter, s);
+  if (ta && !tb) return launch_persistent_tile<true, false>(tile, grid, probs, tile_start, nprob, total_tiles, counter, s);
+  return launch_persistent_tile<true, true>(tile, grid, probs, tile_start, nprob, total_tiles, counter, s);
+}
 
 // Device-pointer GEMM with Rgemm's quick returns; the caller has validated arguments.
 hipError_t rgemm_launch(bool ta, bool tb, int m, int n, int k, uint32_t alpha, const uint32_t* A,

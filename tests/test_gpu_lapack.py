@@ -230,9 +230,9 @@ def test_lu_1024_nb256_device(pga, oracle):
     gb = b.copy()
     pga.Rgemv("N", n, n, ONE, A, n, ones, 1, 0, gb, 1)
     _same(gb, b)
-    B = np.asfortranarray(b.reshape(n, 1))
+    B = b.reshape(n, 1).copy(order="F")  # the oracle solves in place
     oracle.rgetrs("N", want, wpiv, B)
-    dB = pga.DeviceArray.from_host(b.reshape(n, 1))
+    dB = pga.DeviceArray.from_host(b.reshape(n, 1).copy(order="F"))
     pga.getrs_device("N", n, 1, dA, n, dp, dB, n)
     pga.synchronize()
     _same(dB.to_host(), B)
