@@ -12,7 +12,8 @@ Besides the JSON fields of the run contract the line carries:
                 the library stream), where VALU lane-ops per launch = 64 * SQ_INSTS_VALU
                 measured with rocprofv3 (profiles/rgemm_valu_counts.json, scaled by MNK);
                 peak = CUs * 4 SIMD * 16 lanes * clock; traffic = HBM bytes per launch from
-                FETCH_SIZE + WRITE_SIZE in the same file.
+                FETCH_SIZE + WRITE_SIZE in the same file (scaled by a streaming kernel's
+                counts, bench.py --config copycal).
   cpu_baseline  the oracle's restatement of the reference CPU Rgemm (OpenMP over columns,
                 as libRpack/Rgemm.cpp:168) on a bounded sample of the 16384 shape.
 """
@@ -158,14 +159,20 @@ def main():
             "frac": None, "traffic": None, "kernel": "rgemm_tiled NN 64x64x16",
             "avg_ms": round(per_shape_ms[0], 4), "gops": round(2 * macs / t_dom / 1e9, 2)}
     if os.path.exists(COUNTS):
+        # rocprofv3 counts of one launch of this kernel at M = N = 16384, K = 256:
+        # VALU lane-ops per multiply-add = 64 * SQ_INSTS_VALU / MNK (the per-element work
+        # does not depend on the shape); HBM bytes = FETCH_SIZE and WRITE_SIZE scaled by
+        # what they read for a streaming kernel of known size.
         c = json.load(open(COUNTS))
         lane_ops_per_mac = c["valu_lane_ops_per_mac"]
         achieved = lane_ops_per_mac * macs / t_dom
         roof["achieved"] = round(achieved / 1e12, 3)
         roof["frac"] = round(achieved / peak, 4)
         roof["valu_insts_per_posit_op"] = round(lane_ops_per_mac / 2.0, 2)
-        if c.get("hbm_bytes_per_mac") is not None:
-            roof["traffic"] = int(c["hbm_bytes_per_mac"] * macs)
+        if c.get("shape_mnk") == [n0, n0, K]:
+            roof["traffic"] = int(c["hbm_bytes"])
+            roof["algorithmic_bytes"] = int(c["algorithmic_bytes"])
+            roof["hbm_GBps"] = round(c["hbm_bytes"] / t_dom / 1e9, 1)
         roof["counts_from"] = os.path.relpath(COUNTS, ROOT)
 
     out = {
