@@ -77,9 +77,11 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--config", default="gemm", choices=["gemm", "lu", "cholesky", "batched", "copycal"],
+    ap.add_argument("--config", default="gemm",
+                    choices=["gemm", "lu", "cholesky", "batched", "batched_factor", "copycal"],
                     help="gemm: trailing-update GEMMs (headline); lu / cholesky: blocked "
-                         "factorization + solve at n=8192; batched: 4096 small GEMMs; copycal: a "
+                         "factorization + solve at n=8192; batched: 4096 small GEMMs; batched_factor: "
+                         "2048 LU + 2048 Cholesky factorizations of n in [32, 512]; copycal: a "
                          "streaming kernel of known size (2 x 1 GiB read, 1 GiB written) that "
                          "calibrates the FETCH_SIZE / WRITE_SIZE counters")
     ap.add_argument("--n", type=int, default=8192, help="order for --config lu / cholesky")
@@ -107,6 +109,8 @@ def main():
         return bench_batched(pga, info, args)
     if args.config == "copycal":
         return bench_copycal(pga, info, args)
+    if args.config == "batched_factor":
+        return bench_batched_factor(pga, info, args)
 
     bufs = []
     for i, n in enumerate(SHAPES):
@@ -362,6 +366,88 @@ def bench_batched(pga, info, args):
                    "graph_s": round(t_graph, 5), "per_call_s": round(t_plain, 5),
                    "speedup_vs_per_call": round(t_plain / t_graph, 2), "plan_build_s": round(t_tune, 2),
                    "device": info["name"]},
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_batched_factor(pga, info, args):
+    """2048 LU (Rgetrf_opencl, nb = 64) and 2048 lower Cholesky (Rpotrf_opencl, nb = 64)
+    factorizations, n uniform in [32, 512], in one persistent launch replayed from a
+    hipGraph, against the per-call device routines one problem after another."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from datagen import uniform_doubles
+
+    rng = np.random.default_rng(2048)
+    count = 2048
+    ns = [("getrf", int(n)) for n in rng.integers(32, 513, count)] + \
+         [("potrf", int(n)) for n in rng.integers(32, 513, count)]
+    items = []
+    for i, (kind, n) in enumerate(ns):
+        d0 = pga.DeviceArray((n, n))
+        if kind == "getrf":
+            pga.fill_uniform(d0, 50000 + i)
+        else:
+            M = uniform_doubles(60000 + i, n * n).reshape(n, n)
+            S = M @ M.T
+            S /= n
+            S[np.diag_indices(n)] += 1.0
+            dS = pga.DeviceArray.from_host(np.asfortranarray(S))
+            pga._check(pga._lib.pga_p32_from_f64_device(dS.ptr, d0.ptr, n * n, None), "from_f64")
+            pga.synchronize()
+            dS.free()
+        items.append(dict(kind=kind, n=n, orig=d0, a=pga.DeviceArray((n, n)),
+                          ipiv=pga.DeviceArray(n, np.int32, "C") if kind == "getrf" else None,
+                          info=pga.DeviceArray(1, np.int32, "C")))
+    pga.synchronize()
+    log(f"inputs ready: {len(items)} matrices")
+
+    def restore():
+        for it in items:
+            it["a"].copy_from(it["orig"])
+        pga.synchronize()
+
+    ops = sum((2.0 / 3.0 if k == "getrf" else 1.0 / 3.0) * n ** 3 for k, n in ns)
+    tasks = [dict(kind=it["kind"], n=it["n"], a=it["a"], lda=it["n"], ipiv=it["ipiv"], info=it["info"], uplo="L",
+                  nb=64) for it in items]
+    t0 = time.perf_counter()
+    plan = pga.FactorPlan(tasks)
+    plan.capture_graph()
+    t_build = time.perf_counter() - t0
+    times = []
+    for _ in range(max(1, min(args.steps, 3))):
+        restore()
+        times.append(_timed(pga, plan.launch_graph))
+    t_graph = min(times)
+    log(f"graph replay {times}")
+    got = [(it["a"].to_host(), None if it["ipiv"] is None else it["ipiv"].to_host(), int(it["info"].to_host()[0]))
+           for it in items]
+    restore()
+
+    def per_call():
+        for it in items:
+            if it["kind"] == "getrf":
+                pga.getrf_device(it["n"], it["n"], it["a"], it["n"], it["ipiv"], it["info"], 64)
+            else:
+                pga.potrf_device("L", it["n"], it["a"], it["n"], it["info"], 64)
+
+    t_plain = _timed(pga, per_call)
+    log(f"per-call {t_plain:.3f} s")
+    mismatches = 0
+    for it, (a, piv, inf) in zip(items, got):
+        same = np.array_equal(a, it["a"].to_host()) and inf == int(it["info"].to_host()[0])
+        if piv is not None:
+            same = same and np.array_equal(piv, it["ipiv"].to_host())
+        mismatches += 0 if same else 1
+    out = {
+        "metric": "batched posit32 factorizations/s", "value": round(len(items) / t_graph, 1),
+        "unit": "factorizations/s", "n_gpus": 1, "higher_is_better": True, "dtype": "posit32 (integer VALU)",
+        "data": "synthetic: LU inputs uniform[-1,1), Cholesky inputs M*M^T/n + I, posit32",
+        "config": {"workload": f"{count} Rgetrf_opencl (nb=64) + {count} lower Rpotrf_opencl (nb=64), n uniform "
+                               "in [32, 512], one persistent launch in a hipGraph",
+                   "graph_s": round(t_graph, 4), "per_call_s": round(t_plain, 4),
+                   "speedup_vs_per_call": round(t_plain / t_graph, 2), "gops": round(ops / t_graph / 1e9, 2),
+                   "plan": plan.info(), "plan_build_s": round(t_build, 3),
+                   "bitwise_mismatches_vs_per_call": mismatches, "device": info["name"]},
     }
     print(json.dumps(out), flush=True)
 

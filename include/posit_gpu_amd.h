@@ -179,6 +179,51 @@ pga_status pga_batch_plan_info(const pga_batch_plan *plan, int *tile, int *grid,
                                double *tune_ms);
 pga_status pga_batch_plan_destroy(pga_batch_plan *plan);
 
+/* ---- batched factorizations and solves (one persistent launch, hipGraph-capturable) --
+ * Each task is one of the per-call routines below on its own device matrix, computed by
+ * one workgroup with the same operations in the same order as the per-call driver, so
+ * factors, pivots, info codes and solutions equal the per-call results bit for bit:
+ *   GETRF  = pga_getrf_device  (Rgetrf_opencl.cpp:32-154, block size nb, 0 -> 64)
+ *   POTRF  = pga_potrf_device  (Rpotrf_opencl.cpp:32-166, uplo, block size nb, 0 -> 64)
+ *   GETRS  = pga_getrs_device 'N' on an earlier GETRF's factors (Rgetrs.cpp:86-100)
+ *   POTRS  = pga_potrs_device  (Rpotrs.cpp:86-104)
+ *   GESV / POSV = the factorization, then the solve when info == 0.
+ * Tasks are square, n <= PGA_FACTOR_TASK_MAX_N; a, ipiv, b, info are device pointers
+ * (info: one int, written with the factorization's info, 0 for a solve).  Tasks run
+ * most expensive first; tasks must not share output memory.  The reference has no
+ * batched form: this extends Rgetrf_opencl / Rpotrf_opencl / Rgetrs / Rpotrs. */
+#define PGA_FACTOR_TASK_MAX_N 1024
+enum {
+  PGA_TASK_GETRF = 0,
+  PGA_TASK_POTRF = 1,
+  PGA_TASK_GETRS = 2,
+  PGA_TASK_POTRS = 3,
+  PGA_TASK_GESV = 4,
+  PGA_TASK_POSV = 5
+};
+typedef struct {
+  int kind;      /* PGA_TASK_* */
+  char uplo;     /* 'L' or 'U' for the Cholesky tasks */
+  int n;         /* order */
+  int nrhs;      /* right-hand sides of the solve tasks */
+  int nb;        /* block size of the mirrored per-call routine (0: iMlaenv's 64) */
+  uint32_t *a;   /* n x n, column-major, leading dimension lda */
+  int lda;
+  int *ipiv;     /* n entries, 1-based (LU tasks) */
+  uint32_t *b;   /* n x nrhs, leading dimension ldb (solve tasks) */
+  int ldb;
+  int *info;     /* may be NULL */
+} pga_factor_task;
+typedef struct pga_factor_plan pga_factor_plan;
+/* grid: workgroups of the persistent launch (0: as many as fit on the device at once) */
+pga_status pga_factor_plan_create(pga_factor_plan **plan, int count, const pga_factor_task *tasks,
+                                  int grid);
+pga_status pga_factor_plan_execute(pga_factor_plan *plan, void *stream);
+pga_status pga_factor_plan_capture_graph(pga_factor_plan *plan, void *stream);
+pga_status pga_factor_plan_launch_graph(pga_factor_plan *plan, void *stream);
+pga_status pga_factor_plan_info(const pga_factor_plan *plan, int *count, int *grid);
+pga_status pga_factor_plan_destroy(pga_factor_plan *plan);
+
 /* ---- conversions (device codecs) -------------------------------------------------
  * double <-> posit32 on device arrays: round to nearest, ties to the even pattern,
  * saturating at minpos/maxpos; NaN and infinities become NaR.  (The reference converts

@@ -174,6 +174,12 @@ class DeviceArray:
             _check(_lib.pga_memcpy_d2h(h.ctypes.data, self.ptr, self.nbytes), "pga_memcpy_d2h")
         return h
 
+    def copy_from_host(self, a):
+        h = np.asfortranarray(a, dtype=self.dtype) if self.order == "F" else np.ascontiguousarray(a, dtype=self.dtype)
+        assert h.nbytes == self.nbytes
+        if self.nbytes:
+            _check(_lib.pga_memcpy_h2d(self.ptr, h.ctypes.data, self.nbytes), "pga_memcpy_h2d")
+
     def copy_from(self, other):
         assert other.nbytes == self.nbytes
         _check(_lib.pga_memcpy_d2d(self.ptr, other.ptr, self.nbytes), "pga_memcpy_d2d")
@@ -523,5 +529,78 @@ class BatchPlan:
     def __del__(self):
         try:
             _lib.pga_batch_plan_destroy(self.h)
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------------
+# batched factorizations / solves (one persistent launch)
+# ---------------------------------------------------------------------------------
+TASK_GETRF, TASK_POTRF, TASK_GETRS, TASK_POTRS, TASK_GESV, TASK_POSV = range(6)
+FACTOR_TASK_MAX_N = 1024
+_TASK_KINDS = {"getrf": TASK_GETRF, "potrf": TASK_POTRF, "getrs": TASK_GETRS, "potrs": TASK_POTRS,
+               "gesv": TASK_GESV, "posv": TASK_POSV}
+
+
+class FactorTask(ctypes.Structure):
+    """pga_factor_task of include/posit_gpu_amd.h."""
+    _fields_ = [("kind", ctypes.c_int), ("uplo", ctypes.c_char), ("n", ctypes.c_int), ("nrhs", ctypes.c_int),
+                ("nb", ctypes.c_int), ("a", _P), ("lda", ctypes.c_int), ("ipiv", _P), ("b", _P), ("ldb", ctypes.c_int),
+                ("info", _P)]
+
+
+_sig("pga_factor_plan_create", [ctypes.POINTER(_P), _I, ctypes.POINTER(FactorTask), _I])
+_sig("pga_factor_plan_execute", [_P, _P])
+_sig("pga_factor_plan_capture_graph", [_P, _P])
+_sig("pga_factor_plan_launch_graph", [_P, _P])
+_sig("pga_factor_plan_info", [_P, _IP, _IP])
+_sig("pga_factor_plan_destroy", [_P])
+
+
+class FactorPlan:
+    """Many independent factorizations / solves in one persistent launch.
+
+    tasks: list of dicts with keys kind ("getrf", "potrf", "getrs", "potrs", "gesv",
+    "posv" or the TASK_* value), n, a, lda and, as the kind needs, uplo, nb, ipiv, b,
+    ldb, nrhs, info; a, ipiv, b, info are DeviceArrays or device pointers."""
+
+    def __init__(self, tasks, grid=0):
+        init()
+        cnt = len(tasks)
+        arr = (FactorTask * max(1, cnt))()
+        for i, t in enumerate(tasks):
+            kind = t["kind"]
+            arr[i].kind = _TASK_KINDS[kind] if isinstance(kind, str) else int(kind)
+            arr[i].uplo = _ch(t.get("uplo", "L"))
+            arr[i].n = int(t["n"])
+            arr[i].nrhs = int(t.get("nrhs", 0))
+            arr[i].nb = int(t.get("nb", 0))
+            arr[i].a = _dp(t["a"])
+            arr[i].lda = int(t.get("lda", max(1, t["n"])))
+            arr[i].ipiv = _dp(t.get("ipiv")) if t.get("ipiv") is not None else None
+            arr[i].b = _dp(t.get("b")) if t.get("b") is not None else None
+            arr[i].ldb = int(t.get("ldb", max(1, t["n"])))
+            arr[i].info = _dp(t.get("info")) if t.get("info") is not None else None
+        h = _P()
+        _check(_lib.pga_factor_plan_create(ctypes.byref(h), cnt, arr, grid), "pga_factor_plan_create")
+        self.h = h.value
+
+    def execute(self, stream=None):
+        _check(_lib.pga_factor_plan_execute(self.h, stream), "pga_factor_plan_execute")
+
+    def capture_graph(self, stream=None):
+        _check(_lib.pga_factor_plan_capture_graph(self.h, stream), "pga_factor_plan_capture_graph")
+
+    def launch_graph(self, stream=None):
+        _check(_lib.pga_factor_plan_launch_graph(self.h, stream), "pga_factor_plan_launch_graph")
+
+    def info(self):
+        c, g = ctypes.c_int(), ctypes.c_int()
+        _check(_lib.pga_factor_plan_info(self.h, ctypes.byref(c), ctypes.byref(g)), "pga_factor_plan_info")
+        return {"count": c.value, "grid": g.value}
+
+    def __del__(self):
+        try:
+            _lib.pga_factor_plan_destroy(self.h)
         except Exception:
             pass

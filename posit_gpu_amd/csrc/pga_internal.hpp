@@ -32,6 +32,15 @@ struct BatchProblem {
   uint32_t* c;
 };
 
+// One task of a batched factorization plan (device-resident descriptor array).
+struct FactorTask {
+  int kind, uplo, n, nrhs, nb, lda, ldb;
+  uint32_t* a;
+  int* ipiv;
+  uint32_t* b;
+  int* info;
+};
+
 // Runtime state (runtime.hip)
 struct Runtime {
   bool initialized = false;
