@@ -19,6 +19,7 @@ Besides the JSON fields of the run contract the line carries:
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -78,13 +79,16 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--config", default="gemm",
-                    choices=["gemm", "lu", "cholesky", "batched", "batched_factor", "copycal"],
+                    choices=["gemm", "lu", "cholesky", "batched", "batched_factor", "serve", "copycal"],
                     help="gemm: trailing-update GEMMs (headline); lu / cholesky: blocked "
                          "factorization + solve at n=8192; batched: 4096 small GEMMs; batched_factor: "
-                         "2048 LU + 2048 Cholesky factorizations of n in [32, 512]; copycal: a "
+                         "2048 LU + 2048 Cholesky factorizations of n in [32, 512]; serve: Poisson "
+                         "stream of solve requests, served vs plain per-request; copycal: a "
                          "streaming kernel of known size (2 x 1 GiB read, 1 GiB written) that "
                          "calibrates the FETCH_SIZE / WRITE_SIZE counters")
     ap.add_argument("--n", type=int, default=8192, help="order for --config lu / cholesky")
+    ap.add_argument("--requests", type=int, default=400, help="requests for --config serve")
+    ap.add_argument("--qps", type=float, default=60.0, help="arrival rate for --config serve")
     ap.add_argument("--only-shape", type=int, default=0,
                     help="run only the M=N=<value> GEMM (used for per-dispatch profiling)")
     args = ap.parse_args()
@@ -111,6 +115,8 @@ def main():
         return bench_copycal(pga, info, args)
     if args.config == "batched_factor":
         return bench_batched_factor(pga, info, args)
+    if args.config == "serve":
+        return bench_serve(pga, info, args)
 
     bufs = []
     for i, n in enumerate(SHAPES):
@@ -448,6 +454,118 @@ def bench_batched_factor(pga, info, args):
                    "speedup_vs_per_call": round(t_plain / t_graph, 2), "gops": round(ops / t_graph / 1e9, 2),
                    "plan": plan.info(), "plan_build_s": round(t_build, 3),
                    "bitwise_mismatches_vs_per_call": mismatches, "device": info["name"]},
+    }
+    print(json.dumps(out), flush=True)
+
+
+def _serve_requests(pga, count, seed):
+    """Requests of the serving workload: 50% LU / 50% SPD Cholesky, n log-uniform in
+    [64, 4096], 1..16 right-hand sides uniform[-1,1), 20% re-solves against an earlier
+    factor.  Matrices are leading principal blocks of one 4096 x 4096 general matrix
+    (uniform[-1,1)) and one SPD matrix M*M^T/4096 + I (M uniform[-1,1)), converted to
+    posit32 on the device."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from datagen import uniform_doubles
+
+    N = 4096
+    G = pga.DeviceArray((N, N))
+    pga.fill_uniform(G, seed)
+    Gh = G.to_host()
+    M = uniform_doubles(seed + 1, N * N).reshape(N, N)
+    S = M @ M.T
+    S /= N
+    S[np.diag_indices(N)] += 1.0
+    del M
+    dS = pga.DeviceArray.from_host(np.asfortranarray(S))
+    del S
+    P = pga.DeviceArray((N, N))
+    pga._check(pga._lib.pga_p32_from_f64_device(dS.ptr, P.ptr, N * N, None), "from_f64")
+    pga.synchronize()
+    dS.free()
+    Sh = P.to_host()
+    rng = np.random.default_rng(seed + 2)
+    reqs, news = [], []
+    for i in range(count):
+        nrhs = int(rng.integers(1, 17))
+        if news and rng.random() < 0.2:
+            j = news[int(rng.integers(0, len(news)))]
+            n = reqs[j]["n"]
+            reqs.append(dict(kind="resolve", n=n, of=j, b=None, nrhs=nrhs))
+        else:
+            n = int(round(math.exp(rng.uniform(math.log(64), math.log(4096)))))
+            kind = "lu" if rng.random() < 0.5 else "chol"
+            src = Gh if kind == "lu" else Sh
+            reqs.append(dict(kind=kind, n=n, a=np.asfortranarray(src[:n, :n]), nrhs=nrhs))
+            news.append(i)
+    for r in reqs:
+        r["b"] = np.asfortranarray(Gh[: r["n"], 4096 - r["nrhs"]:])  # uniform[-1,1) columns
+    return reqs
+
+
+def _serve_run(pga, reqs, mode, arrivals, batch_max_n=512, batch_slots=2, workers=4):
+    """Submit reqs at the given arrival times (seconds from start); return results."""
+    budget = 64 << 30
+    server = pga.Server(budget, mode, batch_max_n=batch_max_n, batch_slots=batch_slots, workers=workers,
+                        chol_uplo="U")
+    ids = [None] * len(reqs)
+    t0 = time.perf_counter()
+    for i, r in enumerate(reqs):
+        wait = arrivals[i] - (time.perf_counter() - t0)
+        if wait > 0:
+            time.sleep(wait)
+        if r["kind"] == "resolve":
+            ids[i] = server.submit("resolve", r["n"], b=r["b"], factor=ids[r["of"]])
+        else:
+            ids[i] = server.submit(r["kind"], r["n"], r["a"], r["b"])
+    res = [server.wait(i) for i in ids]
+    wall = time.perf_counter() - t0
+    st = server.stats()
+    server.close()
+    lat = np.array([r["t_done"] - r["t_submit"] for r in res])
+    first = min(r["t_submit"] for r in res)
+    span = max(r["t_done"] for r in res) - first
+    ok = sum(1 for r in res if r["status"] == 0 and r["info"] == 0)
+    return res, {"solves_per_s": round(len(res) / span, 2), "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 2),
+                 "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 2), "wall_s": round(wall, 3), "ok": ok,
+                 "batches": st["batches"], "batched_requests": st["batched_requests"],
+                 "peak_bytes": st["bytes_peak"], "evictions": st["evictions"]}
+
+
+def bench_serve(pga, info, args):
+    """Serving: capacity (every request submitted at once) and latency under a Poisson
+    arrival stream at a fixed rate, for the served form (continuous batching) and for
+    plain per-request launches; every solution compared bit for bit between the two."""
+    reqs = _serve_requests(pga, args.requests, 777)
+    log(f"{len(reqs)} requests generated")
+    zero = [0.0] * len(reqs)
+    rng = np.random.default_rng(778)
+    poisson = np.cumsum(rng.exponential(1.0 / args.qps, len(reqs))).tolist()
+    cap_s, cap_s_st = _serve_run(pga, reqs, "batched", zero)
+    log(f"served capacity {cap_s_st}")
+    cap_p, cap_p_st = _serve_run(pga, reqs, "plain", zero)
+    log(f"plain capacity {cap_p_st}")
+    lat_s, lat_s_st = _serve_run(pga, reqs, "batched", poisson)
+    log(f"served at {args.qps} qps {lat_s_st}")
+    lat_p, lat_p_st = _serve_run(pga, reqs, "plain", poisson)
+    log(f"plain at {args.qps} qps {lat_p_st}")
+    mism = 0
+    for runs in zip(cap_s, cap_p, lat_s, lat_p):
+        xs = [r["x"] for r in runs]
+        infos = {r["info"] for r in runs}
+        if len(infos) != 1 or any(not np.array_equal(xs[0], x) for x in xs[1:]):
+            mism += 1
+    sizes = [r["n"] for r in reqs]
+    out = {
+        "metric": "served posit32 solves/s", "value": cap_s_st["solves_per_s"], "unit": "solves/s", "n_gpus": 1,
+        "higher_is_better": True, "dtype": "posit32 (integer VALU)",
+        "data": "synthetic: general uniform[-1,1) and SPD M*M^T/n + I blocks, RHS uniform[-1,1), posit32",
+        "config": {"workload": f"{len(reqs)} solve requests: 50% LU / 50% Cholesky (upper), n log-uniform "
+                               "[64, 4096], 1-16 RHS, 20% re-solves on resident factors",
+                   "n_median": int(np.median(sizes)), "resolves": sum(1 for r in reqs if r["kind"] == "resolve"),
+                   "served_capacity": cap_s_st, "plain_capacity": cap_p_st,
+                   "speedup_vs_plain": round(cap_s_st["solves_per_s"] / cap_p_st["solves_per_s"], 2),
+                   "qps": args.qps, "served_at_qps": lat_s_st, "plain_at_qps": lat_p_st,
+                   "bitwise_mismatches_across_runs": mism, "device": info["name"]},
     }
     print(json.dumps(out), flush=True)
 

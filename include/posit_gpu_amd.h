@@ -192,7 +192,7 @@ pga_status pga_batch_plan_destroy(pga_batch_plan *plan);
  * (info: one int, written with the factorization's info, 0 for a solve).  Tasks run
  * most expensive first; tasks must not share output memory.  The reference has no
  * batched form: this extends Rgetrf_opencl / Rpotrf_opencl / Rgetrs / Rpotrs. */
-#define PGA_FACTOR_TASK_MAX_N 1024
+#define PGA_FACTOR_TASK_MAX_N 512
 enum {
   PGA_TASK_GETRF = 0,
   PGA_TASK_POTRF = 1,
@@ -223,6 +223,63 @@ pga_status pga_factor_plan_capture_graph(pga_factor_plan *plan, void *stream);
 pga_status pga_factor_plan_launch_graph(pga_factor_plan *plan, void *stream);
 pga_status pga_factor_plan_info(const pga_factor_plan *plan, int *count, int *grid);
 pga_status pga_factor_plan_destroy(pga_factor_plan *plan);
+
+/* ---- serving: posit32 solve requests with continuous batching -----------------------
+ * A server owns an HBM arena of hbm_budget_bytes.  Requests are admitted in arrival
+ * order while their device memory (factor + right-hand sides) fits; factors kept for
+ * later re-solves stay resident and are evicted least-recently-used when memory is
+ * short.  In PGA_SERVER_BATCHED mode every admitted request with n <= batch_max_n joins
+ * the next persistent task launch (pga_factor_plan semantics) on one of batch_slots
+ * streams as soon as a slot is free, while larger requests run the per-call drivers on
+ * `workers` threads with a stream each; all of it overlaps on the device.  In
+ * PGA_SERVER_PLAIN mode one worker runs each request alone through the per-call drivers
+ * (pga_rgetrf_device + pga_rgetrs_device, pga_rpotrf_device + pga_rpotrs_device), one
+ * after another.  Both modes give identical bits: a factor-and-solve request computes
+ * Rgetrf_opencl / Rpotrf_opencl (nb 64) and, if info == 0, Rgetrs 'N' / Rpotrs.
+ * Host arrays a, b, x must stay valid until the request completes.  The reference has
+ * no serving layer: this wraps the routines of mplapack_posit.h:38-55. */
+enum { PGA_SERVER_BATCHED = 0, PGA_SERVER_PLAIN = 1 };
+enum { PGA_REQ_LU_SOLVE = 0, PGA_REQ_CHOL_SOLVE = 1, PGA_REQ_RESOLVE = 2 };
+/* request status in pga_request_result */
+enum { PGA_REQ_OK = 0, PGA_REQ_NOT_RESIDENT = 1, PGA_REQ_TOO_LARGE = 2, PGA_REQ_FAILED = 3 };
+typedef struct {
+  size_t hbm_budget_bytes; /* device arena for inputs, factors and solutions */
+  size_t staging_bytes;    /* pinned host staging of batched solutions (0: 256 MiB) */
+  int mode;                /* PGA_SERVER_BATCHED or PGA_SERVER_PLAIN */
+  int batch_max_n;         /* largest order that joins batched launches (<= PGA_FACTOR_TASK_MAX_N) */
+  int max_batch;           /* requests per batched launch (0: 1024) */
+  int batch_slots;         /* batched launches in flight at once (0: 2, at most 2) */
+  int workers;             /* threads running larger requests (0: 4, at most 8) */
+  char chol_uplo;          /* triangle of the Cholesky requests, 'U' or 'L' */
+} pga_server_config;
+typedef struct {
+  int kind;                /* PGA_REQ_* */
+  int n, nrhs;
+  const uint32_t *a;       /* host n x n, column-major (factor-and-solve requests) */
+  const uint32_t *b;       /* host n x nrhs right-hand sides */
+  uint32_t *x;             /* host n x nrhs solution */
+  int64_t factor;          /* PGA_REQ_RESOLVE: factor id (the id of the request that made it) */
+  int keep_factor;         /* keep this request's factor resident for later re-solves */
+} pga_request;
+typedef struct {
+  int64_t id, factor;
+  int status;              /* PGA_REQ_OK, ... */
+  int info;                /* of the factorization used */
+  double t_submit, t_admit, t_done; /* seconds since the server was created */
+} pga_request_result;
+typedef struct {
+  int64_t submitted, completed, solved, batches, batched_requests, evictions;
+  int resident_factors;
+  size_t bytes_in_use, bytes_peak;
+} pga_server_stats;
+typedef struct pga_server pga_server;
+pga_status pga_server_create(pga_server **server, const pga_server_config *config);
+pga_status pga_server_submit(pga_server *server, const pga_request *request, int64_t *id);
+pga_status pga_server_wait(pga_server *server, int64_t id, pga_request_result *result);
+pga_status pga_server_drain(pga_server *server);
+pga_status pga_server_release_factor(pga_server *server, int64_t factor);
+pga_status pga_server_get_stats(pga_server *server, pga_server_stats *stats);
+pga_status pga_server_destroy(pga_server *server);
 
 /* ---- conversions (device codecs) -------------------------------------------------
  * double <-> posit32 on device arrays: round to nearest, ties to the even pattern,

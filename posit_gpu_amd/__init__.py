@@ -537,7 +537,7 @@ class BatchPlan:
 # batched factorizations / solves (one persistent launch)
 # ---------------------------------------------------------------------------------
 TASK_GETRF, TASK_POTRF, TASK_GETRS, TASK_POTRS, TASK_GESV, TASK_POSV = range(6)
-FACTOR_TASK_MAX_N = 1024
+FACTOR_TASK_MAX_N = 512
 _TASK_KINDS = {"getrf": TASK_GETRF, "potrf": TASK_POTRF, "getrs": TASK_GETRS, "potrs": TASK_POTRS,
                "gesv": TASK_GESV, "posv": TASK_POSV}
 
@@ -602,5 +602,103 @@ class FactorPlan:
     def __del__(self):
         try:
             _lib.pga_factor_plan_destroy(self.h)
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------------
+# serving (continuous batching of solve requests)
+# ---------------------------------------------------------------------------------
+SERVER_BATCHED, SERVER_PLAIN = 0, 1
+REQ_LU_SOLVE, REQ_CHOL_SOLVE, REQ_RESOLVE = 0, 1, 2
+REQ_OK, REQ_NOT_RESIDENT, REQ_TOO_LARGE, REQ_FAILED = 0, 1, 2, 3
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", _SZ), ("staging_bytes", _SZ), ("mode", ctypes.c_int),
+                ("batch_max_n", ctypes.c_int), ("max_batch", ctypes.c_int), ("batch_slots", ctypes.c_int),
+                ("workers", ctypes.c_int), ("chol_uplo", ctypes.c_char)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("n", ctypes.c_int), ("nrhs", ctypes.c_int), ("a", _P), ("b", _P),
+                ("x", _P), ("factor", ctypes.c_int64), ("keep_factor", ctypes.c_int)]
+
+
+class RequestResult(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_int64), ("factor", ctypes.c_int64), ("status", ctypes.c_int), ("info", ctypes.c_int),
+                ("t_submit", ctypes.c_double), ("t_admit", ctypes.c_double), ("t_done", ctypes.c_double)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_int64), ("completed", ctypes.c_int64), ("solved", ctypes.c_int64),
+                ("batches", ctypes.c_int64), ("batched_requests", ctypes.c_int64), ("evictions", ctypes.c_int64),
+                ("resident_factors", ctypes.c_int), ("bytes_in_use", _SZ), ("bytes_peak", _SZ)]
+
+
+_sig("pga_server_create", [ctypes.POINTER(_P), ctypes.POINTER(ServerConfig)])
+_sig("pga_server_submit", [_P, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_int64)])
+_sig("pga_server_wait", [_P, ctypes.c_int64, ctypes.POINTER(RequestResult)])
+_sig("pga_server_drain", [_P])
+_sig("pga_server_release_factor", [_P, ctypes.c_int64])
+_sig("pga_server_get_stats", [_P, ctypes.POINTER(ServerStats)])
+_sig("pga_server_destroy", [_P])
+
+
+class Server:
+    """Posit32 solve requests served with continuous batching (see pga_server_* in
+    include/posit_gpu_amd.h).  Matrices are host uint32 arrays of posit patterns."""
+
+    def __init__(self, hbm_budget_bytes, mode="batched", batch_max_n=512, max_batch=1024, batch_slots=2, workers=3,
+                 chol_uplo="U", staging_bytes=0):
+        init()
+        cfg = ServerConfig(hbm_budget_bytes, staging_bytes, SERVER_PLAIN if mode == "plain" else SERVER_BATCHED,
+                           batch_max_n, max_batch, batch_slots, workers, _ch(chol_uplo))
+        h = _P()
+        _check(_lib.pga_server_create(ctypes.byref(h), ctypes.byref(cfg)), "pga_server_create")
+        self.h = h.value
+        self._live = {}
+
+    def submit(self, kind, n, a=None, b=None, factor=0, keep_factor=True):
+        """kind: "lu", "chol" (factor and solve) or "resolve" (against factor id)."""
+        kinds = {"lu": REQ_LU_SOLVE, "chol": REQ_CHOL_SOLVE, "resolve": REQ_RESOLVE}
+        a = None if a is None else np.asfortranarray(a, dtype=np.uint32)
+        b = None if b is None else np.asfortranarray(b, dtype=np.uint32).reshape(n, -1, order="F")
+        nrhs = 0 if b is None else b.shape[1]
+        x = np.zeros((n, max(nrhs, 1)), np.uint32, order="F")
+        r = Request(kinds[kind], n, nrhs, None if a is None else a.ctypes.data, None if b is None else b.ctypes.data,
+                    x.ctypes.data, int(factor), 1 if keep_factor else 0)
+        rid = ctypes.c_int64()
+        _check(_lib.pga_server_submit(self.h, ctypes.byref(r), ctypes.byref(rid)), "pga_server_submit")
+        self._live[rid.value] = (a, b, x, nrhs)
+        return rid.value
+
+    def wait(self, rid):
+        res = RequestResult()
+        _check(_lib.pga_server_wait(self.h, rid, ctypes.byref(res)), "pga_server_wait")
+        a, b, x, nrhs = self._live.pop(rid)
+        return {"id": res.id, "factor": res.factor, "status": res.status, "info": res.info,
+                "t_submit": res.t_submit, "t_admit": res.t_admit, "t_done": res.t_done,
+                "x": x[:, :nrhs] if nrhs else None}
+
+    def drain(self):
+        _check(_lib.pga_server_drain(self.h), "pga_server_drain")
+
+    def release_factor(self, factor):
+        _check(_lib.pga_server_release_factor(self.h, int(factor)), "pga_server_release_factor")
+
+    def stats(self):
+        st = ServerStats()
+        _check(_lib.pga_server_get_stats(self.h, ctypes.byref(st)), "pga_server_get_stats")
+        return {f: getattr(st, f) for f, _ in ServerStats._fields_}
+
+    def close(self):
+        if self.h:
+            _lib.pga_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
         except Exception:
             pass

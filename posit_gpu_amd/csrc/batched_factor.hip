@@ -468,12 +468,29 @@ struct pga_factor_plan {
 };
 
 namespace pga {
+
+// Workgroups for a persistent launch of `count` tasks: as many as fit on the device.
+int factor_tasks_grid(int count) {
+  static const int per_cu = [] {
+    int v = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, wg::k_factor_tasks, wg::NT, 0) != hipSuccess || v < 1) v = 1;
+    return v;
+  }();
+  return std::max(1, std::min(count, per_cu * runtime().cus));
+}
+
+// One persistent launch over a device task list (the counter is cleared by a kernel, so
+// the sequence can be captured in a graph).
+hipError_t factor_tasks_launch(const FactorTask* d_tasks, int count, int* d_counter, int grid, hipStream_t s) {
+  hipLaunchKernelGGL(wg::k_reset_counter, dim3(1), dim3(64), 0, s, d_counter);
+  hipLaunchKernelGGL(wg::k_factor_tasks, dim3(grid), dim3(wg::NT), 0, s, d_tasks, count, d_counter);
+  return hipGetLastError();
+}
+
 namespace {
 
 hipError_t run(pga_factor_plan& p, hipStream_t s) {
-  hipLaunchKernelGGL(wg::k_reset_counter, dim3(1), dim3(64), 0, s, p.d_counter);
-  hipLaunchKernelGGL(wg::k_factor_tasks, dim3(p.grid), dim3(wg::NT), 0, s, p.d_tasks, p.count, p.d_counter);
-  return hipGetLastError();
+  return factor_tasks_launch(p.d_tasks, p.count, p.d_counter, p.grid, s);
 }
 
 // relative cost of a task, for scheduling the longest first
@@ -542,12 +559,8 @@ pga_status pga_factor_plan_create(pga_factor_plan** out, int count, const pga_fa
   std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return task_cost(v[x]) > task_cost(v[y]); });
   for (int i : idx) p->host.push_back(v[i]);
   p->count = (int)p->host.size();
-  int per_cu = 0;
-  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, wg::k_factor_tasks, wg::NT, 0);
-  if (e != hipSuccess || per_cu < 1) per_cu = 1;
-  const int full = per_cu * runtime().cus;
-  p->grid = std::max(1, std::min(p->count, grid > 0 ? grid : full));
-  e = hipMalloc(&p->d_counter, sizeof(int));
+  p->grid = grid > 0 ? std::max(1, std::min(p->count, grid)) : factor_tasks_grid(p->count);
+  hipError_t e = hipMalloc(&p->d_counter, sizeof(int));
   if (e == hipSuccess && p->count) e = hipMalloc(&p->d_tasks, sizeof(FactorTask) * p->count);
   if (e == hipSuccess && p->count)
     e = hipMemcpy(p->d_tasks, p->host.data(), sizeof(FactorTask) * p->count, hipMemcpyHostToDevice);

@@ -360,11 +360,12 @@ static hipError_t trsm(const Ctx& c, bool left, bool upper, bool trans, bool nou
     return hipGetLastError();
   }
   if (left) {
+    // the left-lower-transposed form keeps decoded products in a work buffer of 3*m
+    // words per column: stream-ordered, so calls on different streams never share it
     uint32_t* work = nullptr;
     if (trans && !upper) {
-      pga_status st;
-      work = (uint32_t*)scratch(5, (size_t)3 * m * n * sizeof(uint32_t), &st);
-      if (st) return hipErrorOutOfMemory;
+      hipError_t e = hipMallocAsync((void**)&work, (size_t)3 * m * n * sizeof(uint32_t), c.s);
+      if (e != hipSuccess) return e;
     }
 #define PGA_TL(U, T)                                                                               \
   hipLaunchKernelGGL((k_trsm_left<U, T>), dim3(n), dim3(WG), 0, c.s, m, n, alpha, (int)nounit, A,  \
@@ -374,6 +375,12 @@ static hipError_t trsm(const Ctx& c, bool left, bool upper, bool trans, bool nou
     else if (upper && trans) PGA_TL(true, true);
     else PGA_TL(false, true);
 #undef PGA_TL
+    hipError_t e = hipGetLastError();
+    if (work) {
+      hipError_t f = hipFreeAsync(work, c.s);
+      if (e == hipSuccess) e = f;
+    }
+    return e;
   } else {
 #define PGA_TR(U, T)                                                                              \
   hipLaunchKernelGGL((k_trsm_right<U, T>), dim3(blocks_for(m)), dim3(WG), 0, c.s, m, n, alpha,    \

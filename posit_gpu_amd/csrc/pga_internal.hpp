@@ -41,6 +41,18 @@ struct FactorTask {
   int* info;
 };
 
+// Persistent task kernel of batched_factor.hip
+int factor_tasks_grid(int count);
+hipError_t factor_tasks_launch(const FactorTask* d_tasks, int count, int* d_counter, int grid, hipStream_t s);
+
+// Per-call device drivers of lapack.hip (stream-ordered; arguments already validated)
+hipError_t la_getrf(hipStream_t s, int m, int n, uint32_t* a, int lda, int* ipiv, int* info, int nb);
+hipError_t la_getrs(hipStream_t s, bool trans, int n, int nrhs, const uint32_t* a, int lda, const int* ipiv,
+                    uint32_t* b, int ldb);
+hipError_t la_potrf(hipStream_t s, bool upper, int n, uint32_t* a, int lda, int* info, int nb);
+hipError_t la_potrs(hipStream_t s, bool upper, int n, int nrhs, const uint32_t* a, int lda, uint32_t* b,
+                    int ldb);
+
 // Runtime state (runtime.hip)
 struct Runtime {
   bool initialized = false;

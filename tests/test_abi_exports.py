@@ -5,6 +5,8 @@ import glob
 import os
 import re
 
+import pytest
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB = os.path.join(ROOT, "posit_gpu_amd", "lib", "libposit_gpu_amd.so")
 
@@ -39,20 +41,24 @@ def test_status_strings_without_device():
     assert lib.pga_is_initialized() == 0
 
 
-def test_factor_task_layout_matches_binding(tmp_path):
-    """pga_factor_task as the C compiler lays it out equals the ctypes mirror."""
+@pytest.mark.parametrize("cname,pyname", [("pga_factor_task", "FactorTask"), ("pga_server_config", "ServerConfig"),
+                                           ("pga_request", "Request"), ("pga_request_result", "RequestResult"),
+                                           ("pga_server_stats", "ServerStats")])
+def test_struct_layout_matches_binding(tmp_path, cname, pyname):
+    """Each ABI struct as the C compiler lays it out equals its ctypes mirror."""
     import subprocess
 
     import posit_gpu_amd as pga
 
-    fields = [f for f, _ in pga.FactorTask._fields_]
+    cls = getattr(pga, pyname)
+    fields = [f for f, _ in cls._fields_]
     src = tmp_path / "layout.c"
     src.write_text('#include <stddef.h>\n#include <stdio.h>\n#include "posit_gpu_amd.h"\nint main(void){\n'
-                   '  printf("%zu", sizeof(pga_factor_task));\n' +
-                   "".join(f'  printf(" %zu", offsetof(pga_factor_task, {f}));\n' for f in fields) +
+                   f'  printf("%zu", sizeof({cname}));\n' +
+                   "".join(f'  printf(" %zu", offsetof({cname}, {f}));\n' for f in fields) +
                    "  return 0;\n}\n")
     exe = tmp_path / "layout"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
     got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
-    want = [ctypes.sizeof(pga.FactorTask)] + [getattr(pga.FactorTask, f).offset for f in fields]
+    want = [ctypes.sizeof(cls)] + [getattr(cls, f).offset for f in fields]
     assert got == want
