@@ -306,7 +306,7 @@ constexpr int FAST_MAX = 107;
 struct Val {
   uint32_t m;  // hidden bit at 30 (or exactly 2^31 after a rounding carry); 0 for zero
   int sc;      // 4k + e
-  uint32_t sg; // 0 or 0x80000000
+  uint32_t sg; // sign mask: 0 or 0xFFFFFFFF
 };
 
 PGA_HD Val decode(uint32_t a) {
@@ -320,7 +320,7 @@ PGA_HD Val decode(uint32_t a) {
     v.sc = NAR_SC;
     v.sg = 0;
   } else {
-    v.sg = a & 0x80000000u;
+    v.sg = (uint32_t)((int32_t)a >> 31);
     Parts p = parts(v.sg ? 0u - a : a);
     v.m = p.frac31;
     v.sc = p.scale;

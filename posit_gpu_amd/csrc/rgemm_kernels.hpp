@@ -53,12 +53,23 @@ PGA_D uint32_t clear_mask(uint32_t mask, uint32_t x) {  // x & ~mask
 // scale `sc`: 28 - reg fraction bits, round to nearest, ties to even.  Valid for
 // FAST_MIN <= sc <= FAST_MAX; for other scales the result is unspecified (but
 // computed without undefined behaviour).  May return exactly 2^31 on a carry.
-PGA_D uint32_t round_to_scale(uint32_t y, int sc) {
+PGA_HD uint32_t shift_of_scale(int sc) {  // dropped bits: reg + 2 (< 32 in range)
   int k = sc >> 2;
-  uint32_t sh = (uint32_t)((k ^ (k >> 31)) + 3);  // dropped bits: reg + 2 (< 32 in range)
+  return (uint32_t)((k ^ (k >> 31)) + 3);
+}
+PGA_D uint32_t round_shift(uint32_t y, uint32_t sh) {
   uint32_t low = low_mask(sh);                     // hardware uses sh mod 32
   uint32_t lsb = __builtin_amdgcn_ubfe(y, sh, 1);  // last kept bit (tie breaker)
   return clear_mask(low, y + (low >> 1) + lsb);
+}
+PGA_D uint32_t round_to_scale(uint32_t y, int sc) { return round_shift(y, shift_of_scale(sc)); }
+
+// Full 64-bit product of two 32-bit values in one instruction (the compiler otherwise
+// issues v_mul_hi_u32 and v_mul_lo_u32).
+PGA_D uint64_t mul_wide(uint32_t a, uint32_t b) {
+  uint64_t r, carry;
+  asm("v_mad_u64_u32 %0, %1, %2, %3, 0" : "=v"(r), "=s"(carry) : "v"(a), "v"(b));
+  return r;
 }
 
 // c <- round(c + round(t * a)) in the decoded domain.  Returns true when the sum fell
@@ -66,10 +77,15 @@ PGA_D uint32_t round_to_scale(uint32_t y, int sc) {
 // recompute.  Preconditions (guaranteed by the slice checks): t and a are finite,
 // every nonzero product has scale in [FAST_MIN, FAST_MAX - 1] and the accumulator
 // stays at or below FAST_MAX.
+// With CHECK false the caller has proved that no nonzero sum can fall below the fast
+// range (every operand of the slice is a multiple of 2^g with g >= FAST_MIN) and the
+// flag is not computed.
+template <bool CHECK = true>
 PGA_D bool mac(Val& c, uint32_t tm, int ts, uint32_t tg, uint32_t am, int as, uint32_t ag) {
   // ---- product t * a ----
-  uint32_t hi = __umulhi(tm, am);  // mantissas in [2^30, 2^31): product in [2^60, 2^62)
-  uint32_t lo = tm * am;
+  const uint64_t prod = mul_wide(tm, am);  // mantissas in [2^30, 2^31): product in [2^60, 2^62)
+  uint32_t hi = (uint32_t)(prod >> 32);
+  uint32_t lo = (uint32_t)prod;
   uint32_t cy = hi >> 29;
   int ps = ts + as + (int)cy;
   uint32_t pm = (hi << (2u - cy)) | nonzero_bit(lo);  // hidden at 30, sticky in bit 0
@@ -92,7 +108,8 @@ PGA_D bool mac(Val& c, uint32_t tm, int ts, uint32_t tg, uint32_t am, int as, ui
   uint32_t d = min((uint32_t)(sb - ss), 31u);
   uint32_t lost = __builtin_amdgcn_ubfe(ms, 0, d);
   uint32_t al = (ms >> d) | nonzero_bit(lost);
-  uint32_t x = ((pg ^ c.sg) != 0u) ? mb - al : mb + al;
+  const uint32_t sd = pg ^ c.sg;       // all ones when the signs differ
+  uint32_t x = ((al ^ sd) + mb) - sd;  // mb - al or mb + al
   // leading zeros; a zero sum gets a huge count so its scale lands far below zero
   uint32_t lz = min(ffbh_raw(x), 1u << 21);
   int sn = sb + 1 - (int)lz;
@@ -102,6 +119,7 @@ PGA_D bool mac(Val& c, uint32_t tm, int ts, uint32_t tg, uint32_t am, int as, ui
   c.sc = sn;
   c.sg = gb;
   // a nonzero result below the fast range (zeros are below ZERO_LIMIT)
+  if (!CHECK) return false;
   return (uint32_t)(sn - dec::ZERO_LIMIT) < (uint32_t)(dec::FAST_MIN - dec::ZERO_LIMIT);
 }
 
@@ -121,7 +139,8 @@ PGA_D void add_checked(Val& c, const Val& p) {
     gb = pbig ? p.sg : c.sg;
     uint32_t d = min((uint32_t)(sb - ss), 31u);
     uint32_t al = (ms >> d) | nonzero_bit(__builtin_amdgcn_ubfe(ms, 0, d));
-    uint32_t x = ((p.sg ^ c.sg) != 0u) ? mb - al : mb + al;
+    const uint32_t sd = p.sg ^ c.sg;
+    uint32_t x = ((al ^ sd) + mb) - sd;
     uint32_t lz = min(ffbh_raw(x), 1u << 21);
     int sn = sb + 1 - (int)lz;
     uint32_t z = x << (lz & 31u);

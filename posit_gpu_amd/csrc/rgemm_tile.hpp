@@ -2,6 +2,8 @@
 // in the reference's per-element order (plain, batched and per-problem task kernels).
 #pragma once
 
+#include <type_traits>
+
 #include "rgemm_kernels.hpp"
 
 namespace pga {
@@ -133,13 +135,22 @@ __device__ __forceinline__ void gemm_tile_impl(int m, int n, int k, uint32_t alp
     bool fast = (amax + tmax + 1 <= U);
     if (amin != INT32_MAX && tmin != INT32_MAX) fast = fast && (amin + tmin >= dec::FAST_MIN);
     // accumulators of this lane (zeros have scales far below every bound)
-    int cmax = INT32_MIN;
+    int cmax = INT32_MIN, cmin = INT32_MAX;
 #pragma unroll
     for (int r = 0; r < TM; ++r)
 #pragma unroll
-      for (int c = 0; c < TN; ++c) cmax = max(cmax, acc[r][c].sc + 1);
+      for (int c = 0; c < TN; ++c) {
+        cmax = max(cmax, acc[r][c].sc + 1);
+        if (acc[r][c].m != 0) cmin = min(cmin, acc[r][c].sc);
+      }
     fast = fast && (cmax <= U);
     const bool wave_fast = __all(fast);
+    // Every operand of the slice is a multiple of 2^g, g = (smallest nonzero scale) - 27
+    // (a posit has at most 27 fraction bits), and sums and roundings keep that, so a
+    // nonzero sum is at least 2^g: with g >= FAST_MIN no step can leave the fast range
+    // downwards and the per-step check is dropped.
+    const int pmin = (amin != INT32_MAX && tmin != INT32_MAX) ? amin + tmin : INT32_MAX;
+    const bool wave_nocheck = __all(min(pmin, cmin) >= dec::FAST_MIN + 27);
 
     const int lmax = min(BK, k - kt);
     bool redo = !wave_fast;
@@ -150,7 +161,7 @@ __device__ __forceinline__ void gemm_tile_impl(int m, int n, int k, uint32_t alp
 #pragma unroll
         for (int c = 0; c < TN; ++c) saved[r][c] = acc[r][c];
       bool low = false;
-      for (int l = 0; l < lmax; ++l) {
+      auto step = [&](int l, auto check) {
         uint32_t am[TM], ag[TM], tm[TN], tg[TN];
         int as_[TM], ts[TN];
 #pragma unroll
@@ -168,7 +179,24 @@ __device__ __forceinline__ void gemm_tile_impl(int m, int n, int k, uint32_t alp
 #pragma unroll
         for (int r = 0; r < TM; ++r)
 #pragma unroll
-          for (int c = 0; c < TN; ++c) low |= mac(acc[r][c], tm[c], ts[c], tg[c], am[r], as_[r], ag[r]);
+          for (int c = 0; c < TN; ++c)
+            low |= mac<decltype(check)::value>(acc[r][c], tm[c], ts[c], tg[c], am[r], as_[r], ag[r]);
+      };
+      // two steps per iteration: the loop then carries no register rotation
+      if (wave_nocheck) {
+        int l = 0;
+        for (; l + 1 < lmax; l += 2) {
+          step(l, std::false_type());
+          step(l + 1, std::false_type());
+        }
+        if (l < lmax) step(l, std::false_type());
+      } else {
+        int l = 0;
+        for (; l + 1 < lmax; l += 2) {
+          step(l, std::true_type());
+          step(l + 1, std::true_type());
+        }
+        if (l < lmax) step(l, std::true_type());
       }
       if (__any(low)) {
         redo = true;
