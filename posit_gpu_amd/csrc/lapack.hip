@@ -8,9 +8,12 @@
 // recursion is driven from the host, the work is launched on one stream.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "pga_internal.hpp"
 #include "posit32_core.hpp"
 #include "rgemm_kernels.hpp"
+#include "wg_factor.hpp"
 
 namespace pga {
 namespace la {
@@ -104,6 +107,44 @@ __global__ __launch_bounds__(WG) void k_trsm_left(int m, int n, uint32_t alpha, 
   }
 }
 
+// TRSM, left side, not transposed (Rtrsm.cpp:115-155), `cpw` columns of B per
+// workgroup: the same column sweep with the (row, column) pairs of a step spread over
+// the lanes, so wide right-hand sides take fewer workgroup-steps.
+template <bool UPPER>
+__global__ __launch_bounds__(WG) void k_trsm_left_cols(int m, int n, uint32_t alpha, int nounit,
+                                                       const uint32_t* __restrict__ A, int lda,
+                                                       uint32_t* __restrict__ B, int ldb, int cpw,
+                                                       const int* stop) {
+  const int c0 = blockIdx.x * cpw;
+  if (c0 >= n || stopped(stop)) return;
+  const int C = min(cpw, n - c0);
+  uint32_t* b = B + (size_t)c0 * ldb;
+  const int tid = threadIdx.x;
+  if (alpha != P_ONE) {
+    for (int e = tid; e < m * C; e += WG) b[e % m + (size_t)(e / m) * ldb] = mul(alpha, b[e % m + (size_t)(e / m) * ldb]);
+    __syncthreads();
+  }
+  for (int s = 0; s < m; ++s) {
+    const int k = UPPER ? m - 1 - s : s;
+    const int lo = UPPER ? 0 : k + 1, cnt = UPPER ? k : m - 1 - k;
+    const uint32_t akk = nounit ? A[k + (size_t)k * lda] : P_ONE;
+    for (int e = tid; e < cnt * C; e += WG) {
+      const int i = lo + e % cnt, c = e / cnt;
+      uint32_t bk = b[k + (size_t)c * ldb];
+      if (bk != 0) {  // a zero b_k skips step k (Rtrsm.cpp:128,146)
+        if (nounit) bk = div(bk, akk);
+        b[i + (size_t)c * ldb] = sub(b[i + (size_t)c * ldb], mul(bk, A[i + (size_t)k * lda]));
+      }
+    }
+    __syncthreads();
+    if (nounit)  // b_k = b_k / a_kk once every lane has used the old b_k
+      for (int c = tid; c < C; c += WG) {
+        const uint32_t bk = b[k + (size_t)c * ldb];
+        if (bk != 0) b[k + (size_t)c * ldb] = div(bk, akk);
+      }
+  }
+}
+
 // ---------------------------------------------------------------------------------
 // TRSM, right side: rows of B are independent; one lane per row (Rtrsm.cpp:189-286)
 // ---------------------------------------------------------------------------------
@@ -167,6 +208,41 @@ __global__ __launch_bounds__(WG) void k_trsm_right(int m, int n, uint32_t alpha,
   }
 }
 
+// TRSM, right side, transposed (Rtrsm.cpp:241-285) as a sweep over k: step k scales
+// column k, then subtracts it from the columns it feeds (j < k upper, j > k lower), then
+// applies alpha to column k.  Every element sees its terms in the reference's order; the
+// (row, column) pairs of a step are independent.  A workgroup owns `rows` rows.
+template <bool UPPER>
+__global__ __launch_bounds__(WG) void k_trsm_right_sweep(int m, int n, uint32_t alpha, int nounit,
+                                                         const uint32_t* __restrict__ A, int lda,
+                                                         uint32_t* __restrict__ B, int ldb, int rows,
+                                                         const int* stop) {
+  const int r0 = blockIdx.x * rows;
+  if (r0 >= m || stopped(stop)) return;
+  const int R = min(rows, m - r0);
+  uint32_t* b = B + r0;
+  const int tid = threadIdx.x;
+  for (int s = 0; s < n; ++s) {
+    const int k = UPPER ? n - 1 - s : s;
+    if (nounit) {
+      const uint32_t t = div(P_ONE, A[k + (size_t)k * lda]);
+      for (int i = tid; i < R; i += WG) b[i + (size_t)k * ldb] = mul(t, b[i + (size_t)k * ldb]);
+      __syncthreads();
+    }
+    const int cnt = UPPER ? k : n - 1 - k;
+    for (int e = tid; e < R * cnt; e += WG) {
+      const int i = e % R, j = UPPER ? e / R : k + 1 + e / R;
+      const uint32_t t = A[j + (size_t)k * lda];
+      if (t != 0) b[i + (size_t)j * ldb] = sub(b[i + (size_t)j * ldb], mul(t, b[i + (size_t)k * ldb]));
+    }
+    if (alpha != P_ONE) {
+      __syncthreads();
+      for (int i = tid; i < R; i += WG) b[i + (size_t)k * ldb] = mul(alpha, b[i + (size_t)k * ldb]);
+    }
+    __syncthreads();
+  }
+}
+
 // B = 0 (alpha == 0, Rtrsm.cpp:101-108)
 __global__ __launch_bounds__(WG) void k_zero(int m, int n, uint32_t* B, int ldb, const int* stop) {
   const int i = blockIdx.x * WG + threadIdx.x;
@@ -207,37 +283,38 @@ __global__ __launch_bounds__(WG) void k_scal(int n, uint32_t da, uint32_t* x, in
     x[i * incx] = mul(da, x[i * incx]);
 }
 
-// IAMAX (iRamax.cpp:31-88): first index of the largest |x| in signed pattern order
+// IAMAX (iRamax.cpp:31-88): first index of the largest |x| in signed pattern order.
+// The largest value wins, ties go to the smaller index (a total order, so the reduction
+// order does not matter): the sequential first maximum.
 __device__ void block_iamax(int n, const uint32_t* x, long incx, int* out_idx) {
-  __shared__ uint32_t sv[WG];
-  __shared__ int si[WG];
+  __shared__ int sv[WG / 64], si[WG / 64];
   const int tid = threadIdx.x;
-  uint32_t best = 0;
-  int bi = -1;
+  int bv = INT32_MIN, bi = INT32_MAX;  // bi == INT32_MAX: nothing seen
   for (int i = tid; i < n; i += WG) {
-    uint32_t v = abs_p(x[(long)i * incx]);
-    if (bi < 0 || (int32_t)v > (int32_t)best) {
-      best = v;
+    const int v = (int)abs_p(x[(long)i * incx]);
+    if (bi == INT32_MAX || v > bv) {
+      bv = v;
       bi = i;
     }
   }
-  sv[tid] = best;
-  si[tid] = bi;
-  __syncthreads();
-  for (int s = WG / 2; s > 0; s >>= 1) {
-    if (tid < s) {
-      int oi = si[tid + s];
-      uint32_t ov = sv[tid + s];
-      int mi = si[tid];
-      if (oi >= 0 && (mi < 0 || (int32_t)ov > (int32_t)sv[tid] ||
-                      ((int32_t)ov == (int32_t)sv[tid] && oi < mi))) {
-        sv[tid] = ov;
-        si[tid] = oi;
-      }
+  auto take = [&](int ov, int oi) {
+    if (oi != INT32_MAX && (bi == INT32_MAX || ov > bv || (ov == bv && oi < bi))) {
+      bv = ov;
+      bi = oi;
     }
-    __syncthreads();
+  };
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) take(__shfl_xor(bv, off), __shfl_xor(bi, off));
+  if ((tid & 63) == 0) {
+    sv[tid >> 6] = bv;
+    si[tid >> 6] = bi;
   }
-  if (tid == 0) *out_idx = si[0];  // 0-based, -1 when n == 0
+  __syncthreads();
+  bv = sv[0];
+  bi = si[0];
+#pragma unroll
+  for (int w = 1; w < WG / 64; ++w) take(sv[w], si[w]);
+  if (tid == 0) *out_idx = (bi == INT32_MAX) ? -1 : bi;  // 0-based, -1 when n == 0
   __syncthreads();
 }
 
@@ -293,33 +370,64 @@ __global__ __launch_bounds__(WG) void k_getrf2_row(uint32_t* a, int* ipiv, int* 
   if (a[0] == 0 && *info == 0) *info = col_off + 1;
 }
 
-__global__ __launch_bounds__(WG) void k_getrf2_col(int m, uint32_t* a, int* ipiv, int* info, int row_off,
-                                                   int col_off) {
+// Leaf for column j of a Rgetrf2 call on the m_root x w_root block at `root` (rows
+// j..m_root-1 of column j): the pivot's row interchange is applied at once to every
+// column of the block (the reference applies it to the other columns later, with the
+// Rlaswp calls of Rgetrf2.cpp:143,169; nothing reads those rows in between); the pivot
+// column itself is swapped only for a nonzero pivot (Rgetrf2.cpp:109-117).
+//
+// When this leaf ends the left half of a small recursion node [ts, te) split at tm
+// (tm - ts <= LEAF_TRSM_N1, te - tm <= LEAF_TRSM_N2), the same workgroup then runs that
+// node's Rtrsm (A12 := L11^-1 A12, Rgetrf2.cpp:151-154) in LDS: it is the next step of
+// the reference's order either way, and saves a launch.
+constexpr int LEAF_TRSM_N1 = 16, LEAF_TRSM_N2 = 32;
+
+__global__ __launch_bounds__(WG) void k_getrf2_leaf(int m_root, int w_root, uint32_t* root, int lda, int j,
+                                                    int* ipiv, int* info, int row_off, int col_off, int ts,
+                                                    int tm, int te) {
   __shared__ int piv;
-  block_iamax(m, a, 1, &piv);  // 0-based
-  const int p = piv;
-  const uint32_t ap = a[p];
-  if (threadIdx.x == 0) ipiv[0] = row_off + p + 1;
+  __shared__ uint32_t sL[LEAF_TRSM_N1][LEAF_TRSM_N1], sB[LEAF_TRSM_N1][LEAF_TRSM_N2];
+  uint32_t* col = root + (size_t)j * lda;
+  block_iamax(m_root - j, col + j, 1, &piv);
+  const int p = j + piv;
+  const uint32_t ap = col[p];
+  if (threadIdx.x == 0) ipiv[j] = row_off + p + 1;
+  if (p != j) {
+    for (int c = threadIdx.x; c < w_root; c += WG) {
+      if (c == j && ap == 0) continue;
+      uint32_t* x = root + (size_t)c * lda;
+      const uint32_t t = x[j];
+      x[j] = x[p];
+      x[p] = t;
+    }
+  }
+  __syncthreads();
   if (ap == 0) {
-    if (threadIdx.x == 0 && *info == 0) *info = col_off + 1;
-    return;
-  }
-  __syncthreads();
-  const uint32_t a0 = a[0];
-  __syncthreads();
-  if (threadIdx.x == 0 && p != 0) {
-    a[0] = ap;
-    a[p] = a0;
-  }
-  __syncthreads();
-  // sfmin = FLT_MIN as posit32 = minpos; abs(pivot) >= sfmin except for NaR
-  const uint32_t piv_v = ap;
-  if (!lt(abs_p(piv_v), P_MINPOS)) {
-    const uint32_t r = div(P_ONE, piv_v);  // Rscal(m-1, one/a[0], ...)
-    for (int i = 1 + threadIdx.x; i < m; i += WG) a[i] = mul(r, a[i]);
+    if (threadIdx.x == 0 && *info == 0) *info = col_off + j + 1;
+  } else if (!lt(abs_p(ap), P_MINPOS)) {
+    // sfmin = FLT_MIN as posit32 = minpos: every pivot but NaR takes the Rscal branch
+    const uint32_t r = div(P_ONE, ap);
+    for (int i = j + 1 + threadIdx.x; i < m_root; i += WG) col[i] = mul(r, col[i]);
   } else {
-    for (int i = 1 + threadIdx.x; i < m; i += WG) a[i] = div(a[i], piv_v);
+    for (int i = j + 1 + threadIdx.x; i < m_root; i += WG) col[i] = div(col[i], ap);
   }
+  if (tm <= ts) return;
+  // ---- the node's Rtrsm: L11 unit lower (n1 x n1 at (ts, ts)), B = A12 (n1 x n2 at (ts, tm))
+  const int n1 = tm - ts, n2 = te - tm;
+  __syncthreads();
+  for (int e = threadIdx.x; e < n1 * n1; e += WG) sL[e % n1][e / n1] = root[ts + e % n1 + (size_t)(ts + e / n1) * lda];
+  for (int e = threadIdx.x; e < n1 * n2; e += WG) sB[e % n1][e / n1] = root[ts + e % n1 + (size_t)(tm + e / n1) * lda];
+  __syncthreads();
+  for (int k = 0; k + 1 < n1; ++k) {  // Rtrsm.cpp:139-155: a zero b_k skips step k
+    const int rows = n1 - 1 - k;
+    for (int e = threadIdx.x; e < rows * n2; e += WG) {
+      const int i = k + 1 + e % rows, c = e / rows;
+      const uint32_t bk = sB[k][c];
+      if (bk != 0) sB[i][c] = sub(sB[i][c], mul(bk, sL[i][k]));
+    }
+    __syncthreads();
+  }
+  for (int e = threadIdx.x; e < n1 * n2; e += WG) root[ts + e % n1 + (size_t)(tm + e / n1) * lda] = sB[e % n1][e / n1];
 }
 
 // ---------------------------------------------------------------------------------
@@ -335,6 +443,27 @@ __global__ void k_potrf2_leaf(uint32_t* a, int* info, int col_off) {
   a[0] = pga::sqrt(v);
 }
 
+// ---------------------------------------------------------------------------------
+// Rpotrf2 on a whole diagonal block by one workgroup (wg_factor.hpp: the same recursion
+// and operation order as the launches of lah::potrf2, in one launch).  info doubles as
+// the stop flag, as for every step of lah::potrf.
+// ---------------------------------------------------------------------------------
+// The block is staged in LDS: each recursion step then waits on LDS, not on L2.
+constexpr int POTRF2_ONE_WG_MAX = 64;
+
+template <bool UPPER>
+__global__ __launch_bounds__(WG, 2) void k_potrf2_wg(int n, uint32_t* a, int lda, int* info, int col_off) {
+  __shared__ wg::Shared sh;
+  __shared__ uint32_t blk[POTRF2_ONE_WG_MAX * POTRF2_ONE_WG_MAX];
+  if (*info) return;  // every lane reads the same flag
+  for (int e = threadIdx.x; e < n * n; e += WG) blk[e] = a[e % n + (size_t)(e / n) * lda];
+  __syncthreads();
+  const int f = wg::potrf2<UPPER>(n, blk, n, sh);
+  __syncthreads();
+  for (int e = threadIdx.x; e < n * n; e += WG) a[e % n + (size_t)(e / n) * lda] = blk[e];
+  if (f && threadIdx.x == 0) *info = col_off + f;
+}
+
 }  // namespace la
 
 // ===================================================================================
@@ -346,6 +475,12 @@ using namespace la;
 
 static unsigned blocks_for(long n) { return (unsigned)((n + WG - 1) / WG); }
 
+#define PGA_TRY_E(x)                       \
+  do {                                     \
+    hipError_t _e = (x);                   \
+    if (_e != hipSuccess) return _e;       \
+  } while (0)
+
 struct Ctx {
   hipStream_t s;
   GemmImpl impl;
@@ -353,10 +488,63 @@ struct Ctx {
 
 static hipError_t trsm(const Ctx& c, bool left, bool upper, bool trans, bool nounit, int m, int n,
                        uint32_t alpha, const uint32_t* A, int lda, uint32_t* B, int ldb,
+                       const int* stop);
+
+// B := alpha * scale; elementwise (the first step of every Rtrsm branch with alpha != 1)
+__global__ __launch_bounds__(WG) void k_scale_block(int m, int n, uint32_t alpha, uint32_t* B, int ldb,
+                                                    const int* stop) {
+  if (stopped(stop)) return;
+  for (long e = blockIdx.x * (long)WG + threadIdx.x; e < (long)m * n; e += (long)gridDim.x * WG)
+    B[e % m + (size_t)(e / m) * ldb] = mul(alpha, B[e % m + (size_t)(e / m) * ldb]);
+}
+
+// Left, lower, not transposed (Rtrsm.cpp:139-155) in blocks of TRSM_BLOCK rows: each
+// block is solved by the sweep kernel, then its rows update every row below it through
+// the GEMM with zero multipliers skipped.  Each element still receives its terms in
+// ascending k, and a zero b_k still skips its term, so the result is the same bits; the
+// bulk of the work runs on the GEMM's decoded path.
+constexpr int TRSM_BLOCK = 32;
+
+static hipError_t trsm_lln_blocked(const Ctx& c, bool nounit, int m, int n, uint32_t alpha, const uint32_t* A,
+                                   int lda, uint32_t* B, int ldb, const int* stop) {
+  if (alpha != P_ONE) {
+    const long total = (long)m * n;
+    hipLaunchKernelGGL(k_scale_block, dim3((unsigned)std::min<long>((total + WG - 1) / WG, 65536)), dim3(WG), 0,
+                       c.s, m, n, alpha, B, ldb, stop);
+    PGA_TRY_E(hipGetLastError());
+  }
+  GemmOpts o;
+  o.skip_zero_b = true;
+  o.stop = stop;
+  for (int i0 = 0; i0 < m; i0 += TRSM_BLOCK) {
+    const int ib = std::min(TRSM_BLOCK, m - i0);
+    PGA_TRY_E(trsm(c, true, false, false, nounit, ib, n, P_ONE, A + i0 + (size_t)i0 * lda, lda, B + i0, ldb, stop));
+    if (i0 + ib < m)
+      PGA_TRY_E(rgemm_launch(false, false, m - i0 - ib, n, ib, P_MONE, A + i0 + ib + (size_t)i0 * lda, lda, B + i0,
+                             ldb, P_ONE, B + i0 + ib, ldb, c.s, c.impl, o));
+  }
+  return hipSuccess;
+}
+
+static hipError_t trsm(const Ctx& c, bool left, bool upper, bool trans, bool nounit, int m, int n,
+                       uint32_t alpha, const uint32_t* A, int lda, uint32_t* B, int ldb,
                        const int* stop) {
   if (m == 0 || n == 0) return hipSuccess;
+  if (left && !upper && !trans && alpha != 0 && m > 2 * TRSM_BLOCK)
+    return trsm_lln_blocked(c, nounit, m, n, alpha, A, lda, B, ldb, stop);
   if (alpha == 0) {
     hipLaunchKernelGGL(k_zero, dim3(blocks_for(m)), dim3(WG), 0, c.s, m, n, B, ldb, stop);
+    return hipGetLastError();
+  }
+  if (left && !trans && (long)n >= 8L * runtime().cus) {
+    const int cpw = (int)std::min<long>(8, (n + 4L * runtime().cus - 1) / (4L * runtime().cus));
+    const unsigned grid = (unsigned)((n + cpw - 1) / cpw);
+    if (upper)
+      hipLaunchKernelGGL(k_trsm_left_cols<true>, dim3(grid), dim3(WG), 0, c.s, m, n, alpha, (int)nounit, A, lda,
+                         B, ldb, cpw, stop);
+    else
+      hipLaunchKernelGGL(k_trsm_left_cols<false>, dim3(grid), dim3(WG), 0, c.s, m, n, alpha, (int)nounit, A, lda,
+                         B, ldb, cpw, stop);
     return hipGetLastError();
   }
   if (left) {
@@ -381,14 +569,23 @@ static hipError_t trsm(const Ctx& c, bool left, bool upper, bool trans, bool nou
       if (e == hipSuccess) e = f;
     }
     return e;
+  } else if (trans) {
+    // enough row blocks for every CU twice where m allows, at least 4 rows each
+    const int rows = (int)std::min<long>(256, std::max<long>(4, (m + 2L * runtime().cus - 1) / (2L * runtime().cus)));
+    const unsigned grid = (unsigned)((m + rows - 1) / rows);
+    if (upper)
+      hipLaunchKernelGGL((k_trsm_right_sweep<true>), dim3(grid), dim3(WG), 0, c.s, m, n, alpha, (int)nounit, A,
+                         lda, B, ldb, rows, stop);
+    else
+      hipLaunchKernelGGL((k_trsm_right_sweep<false>), dim3(grid), dim3(WG), 0, c.s, m, n, alpha, (int)nounit, A,
+                         lda, B, ldb, rows, stop);
   } else {
-#define PGA_TR(U, T)                                                                              \
-  hipLaunchKernelGGL((k_trsm_right<U, T>), dim3(blocks_for(m)), dim3(WG), 0, c.s, m, n, alpha,    \
+    // the non-transposed forms sum each element over earlier columns: one lane per row
+#define PGA_TR(U)                                                                                 \
+  hipLaunchKernelGGL((k_trsm_right<U, false>), dim3(blocks_for(m)), dim3(WG), 0, c.s, m, n, alpha, \
                      (int)nounit, A, lda, B, ldb, stop)
-    if (upper && !trans) PGA_TR(true, false);
-    else if (!upper && !trans) PGA_TR(false, false);
-    else if (upper && trans) PGA_TR(true, true);
-    else PGA_TR(false, true);
+    if (upper) PGA_TR(true);
+    else PGA_TR(false);
 #undef PGA_TR
   }
   return hipGetLastError();
@@ -408,30 +605,50 @@ static hipError_t laswp(const Ctx& c, int n, uint32_t* a, int lda, int k1, int k
     if (_e != hipSuccess) return _e;       \
   } while (0)
 
-// Rgetrf2 (Rgetrf2.cpp:32-185) on the block at `a` (rows row_off.., columns col_off..
-// of the matrix that ipiv / info describe).  ipiv points at the block's first entry.
+// Rgetrf2 (Rgetrf2.cpp:32-185) on the m x n block at `a` (rows row_off.., columns
+// col_off.. of the matrix that ipiv / info describe; ipiv points at the block's first
+// entry).  The recursion's nodes cover columns [s, e) and rows [s, m) of the block; its
+// row interchanges are made by the leaves across all n columns of the block, which
+// replaces the Rlaswp calls between the recursive calls (same bits, fewer launches).
+// pend: a small node [ps, pe) split at pm whose Rtrsm the last leaf of this subtree runs
+// (pm <= ps: none).
+static hipError_t getrf2_node(const Ctx& c, int m, int n, uint32_t* a, int lda, int* ipiv, int* info,
+                              int row_off, int col_off, int s, int e, int ps, int pm, int pe) {
+  const int rows = m - s, cols = e - s;
+  if (rows <= 0 || cols <= 0) return hipSuccess;
+  if (rows == 1) {  // Rgetrf2.cpp:86-95: nothing to interchange
+    hipLaunchKernelGGL(k_getrf2_row, dim3(1), dim3(WG), 0, c.s, a + s + (size_t)s * lda, ipiv + s, info,
+                       row_off + s, col_off + s);
+    PGA_TRY(hipGetLastError());
+    if (pm > ps)
+      PGA_TRY(trsm(c, true, false, false, false, pm - ps, pe - pm, P_ONE, a + ps + (size_t)ps * lda, lda,
+                   a + ps + (size_t)pm * lda, lda, nullptr));
+    return hipSuccess;
+  }
+  if (cols == 1) {
+    hipLaunchKernelGGL(k_getrf2_leaf, dim3(1), dim3(WG), 0, c.s, m, n, a, lda, s, ipiv, info, row_off, col_off,
+                       ps, pm, pe);
+    return hipGetLastError();
+  }
+  const int n1 = (rows < cols ? rows : cols) / 2, mid = s + n1, n2 = cols - n1;
+  uint32_t* a11 = a + s + (size_t)s * lda;
+  uint32_t* a12 = a + s + (size_t)mid * lda;
+  const bool fuse = (n1 <= LEAF_TRSM_N1 && n2 <= LEAF_TRSM_N2);
+  if (fuse) {
+    PGA_TRY(getrf2_node(c, m, n, a, lda, ipiv, info, row_off, col_off, s, mid, s, mid, e));
+  } else {
+    PGA_TRY(getrf2_node(c, m, n, a, lda, ipiv, info, row_off, col_off, s, mid, 0, 0, 0));
+    PGA_TRY(trsm(c, true, false, false, false, n1, n2, P_ONE, a11, lda, a12, lda, nullptr));
+  }
+  PGA_TRY(rgemm_launch(false, false, rows - n1, n2, n1, P_MONE, a11 + n1, lda, a12, lda, P_ONE, a12 + n1, lda,
+                       c.s, c.impl));
+  return getrf2_node(c, m, n, a, lda, ipiv, info, row_off, col_off, mid, e, ps, pm, pe);
+}
+
 static hipError_t getrf2(const Ctx& c, int m, int n, uint32_t* a, int lda, int* ipiv, int* info,
                          int row_off, int col_off) {
   if (m == 0 || n == 0) return hipSuccess;
-  if (m == 1) {
-    hipLaunchKernelGGL(k_getrf2_row, dim3(1), dim3(WG), 0, c.s, a, ipiv, info, row_off, col_off);
-    return hipGetLastError();
-  }
-  if (n == 1) {
-    hipLaunchKernelGGL(k_getrf2_col, dim3(1), dim3(WG), 0, c.s, m, a, ipiv, info, row_off, col_off);
-    return hipGetLastError();
-  }
-  const int n1 = (m < n ? m : n) / 2, n2 = n - n1;
-  uint32_t* a12 = a + (size_t)n1 * lda;
-  PGA_TRY(getrf2(c, m, n1, a, lda, ipiv, info, row_off, col_off));
-  PGA_TRY(laswp(c, n2, a12, lda, 1, n1, ipiv, 1, row_off, nullptr));
-  PGA_TRY(trsm(c, true, false, false, false, n1, n2, P_ONE, a, lda, a12, lda, nullptr));
-  PGA_TRY(rgemm_launch(false, false, m - n1, n2, n1, P_MONE, a + n1, lda, a12, lda, P_ONE,
-                       a12 + n1, lda, c.s, c.impl));
-  PGA_TRY(getrf2(c, m - n1, n2, a12 + n1, lda, ipiv + n1, info, row_off + n1, col_off + n1));
-  const int mn = m < n ? m : n;
-  PGA_TRY(laswp(c, n1, a, lda, n1 + 1, mn, ipiv, 1, row_off, nullptr));
-  return hipSuccess;
+  return getrf2_node(c, m, n, a, lda, ipiv, info, row_off, col_off, 0, n, 0, 0, 0);
 }
 
 // Rgetrf_opencl (Rgetrf_opencl.cpp:32-154): blocked right-looking LU, nb columns per
@@ -487,8 +704,16 @@ static hipError_t syrk(const Ctx& c, bool upper, bool trans, int n, int k, uint3
 }
 
 // Rpotrf2 (Rpotrf2.cpp:32-151); info (device) doubles as the stop flag
+// Orders up to POTRF2_ONE_WG_MAX run as one single-workgroup launch (the recursion's
+// steps are then barriers instead of ~3n kernel launches); larger ones recurse with
+// device-wide GEMMs down to that size.
 static hipError_t potrf2(const Ctx& c, bool upper, int n, uint32_t* a, int lda, int* info, int col_off) {
   if (n == 0) return hipSuccess;
+  if (n <= POTRF2_ONE_WG_MAX) {
+    if (upper) hipLaunchKernelGGL(k_potrf2_wg<true>, dim3(1), dim3(WG), 0, c.s, n, a, lda, info, col_off);
+    else hipLaunchKernelGGL(k_potrf2_wg<false>, dim3(1), dim3(WG), 0, c.s, n, a, lda, info, col_off);
+    return hipGetLastError();
+  }
   if (n == 1) {
     hipLaunchKernelGGL(k_potrf2_leaf, dim3(1), dim3(64), 0, c.s, a, info, col_off);
     return hipGetLastError();

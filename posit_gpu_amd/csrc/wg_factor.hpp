@@ -1,0 +1,445 @@
+// wg_factor.hpp -- LU / Cholesky routines run by one workgroup of 256 lanes.
+//
+// Each routine performs the reference routine's operations in the reference's order
+// with workgroup-wide steps separated by barriers: the blocking of Rgetrf_opencl
+// (Rgetrf_opencl.cpp:93-146) and Rpotrf_opencl (Rpotrf_opencl.cpp:86-160), the recursion
+// of Rgetrf2 (Rgetrf2.cpp:84-183) and Rpotrf2 (Rpotrf2.cpp:84-145), and the loop orders of
+// Rtrsm, Rsyrk and Rgemm.  Factors, pivots, info codes and solutions therefore equal the
+// per-call results bit for bit, also for matrices with zero pivots, NaR entries or a
+// failing Cholesky step (where the zero-multiplier skips of Rtrsm / Rsyrk and the state
+// left behind at the failure depend on that structure).
+//
+// Recursions are walked leaf by leaf: after leaf j of a Rgetrf2 / Rpotrf2 call on w
+// columns, the one node whose left half ends at j runs its middle step (the Rtrsm and
+// the Rgemm / Rsyrk between the two recursive calls).  That is the reference's
+// depth-first order.  Row interchanges are applied to every other column as soon as a
+// pivot is chosen; the reference applies them later with Rlaswp, and no step reads
+// those rows in between.
+//
+// Kernels that call these routines must be declared __launch_bounds__(256, 2): the GEMM
+// tile functions are not inlined and then keep their 248 registers without spills.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "pga_internal.hpp"
+#include "rgemm_tile.hpp"
+
+namespace pga {
+namespace wg {
+
+constexpr int NT = 256;            // lanes per workgroup
+constexpr int MAX_N = PGA_FACTOR_TASK_MAX_N;
+
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+
+struct Shared {
+  uint32_t tile[gemm::TileLds<64, 64, 16>::words];  // GEMM tiles of every shape
+  int perm[MAX_N];                                 // row interchanges of right-hand sides
+  int red_v[NT / 64];
+  int red_i[NT / 64];
+  int bcast;
+};
+
+static __device__ __forceinline__ uint32_t* at(uint32_t* a, int lda, int i, int j) {
+  return a + i + (size_t)j * lda;
+}
+
+// ---------------------------------------------------------------------------------
+// C op= -op(A) op(B) with beta = 1 in the order of Rgemm (Rgemm.cpp:163-246) or, with
+// SKIP0, of Rsyrk 'N' (Rsyrk.cpp:158-176: a zero B-side multiplier skips the step);
+// tri 1 / 2 stores only the lower / upper triangle (Rsyrk).  The per-element order does
+// not depend on the tile shape, so the shape is chosen for the least padding.
+// ---------------------------------------------------------------------------------
+template <int T, bool TA, bool TB, bool SKIP0>
+static __device__ __noinline__ void gemm_t(int m, int n, int k, const uint32_t* A, int lda, const uint32_t* B, int ldb,
+                                    uint32_t* C, int ldc, int tri, lds_u32* lds) {
+  const int tm = (m + T - 1) / T, tn = (n + T - 1) / T;
+  for (int t = 0; t < tm * tn; ++t) {
+    const int i0 = (t % tm) * T, j0 = (t / tm) * T;
+    if (tri == 1 && i0 + T - 1 < j0) continue;
+    if (tri == 2 && j0 + T - 1 < i0) continue;
+    gemm::gemm_tile_in<T, T, 16, T / 16, T / 16, TA, TB, SKIP0>(m, n, k, P_MONE, A, lda, B, ldb, P_ONE, C, ldc,
+                                                                i0, j0, tri, (uint32_t*)lds);
+  }
+  __syncthreads();
+}
+
+// Small products one element per lane with the pattern-level operations, in the same
+// order and with the same epilogue as gemm_tile (alpha = -1 is a negation, beta = 1).
+template <bool TA, bool TB, bool SKIP0>
+static __device__ __noinline__ void gemm_direct(int m, int n, int k, const uint32_t* A, int lda, const uint32_t* B,
+                                                int ldb, uint32_t* C, int ldc, int tri) {
+  for (int e = threadIdx.x; e < m * n; e += NT) {
+    const int i = e % m, j = e / m;
+    if ((tri == 1 && i < j) || (tri == 2 && i > j)) continue;
+    uint32_t* cij = C + i + (size_t)j * ldc;
+    if (!TA) {  // c = C; c = c + (alpha*b_lj)*a_il, l ascending (Rgemm.cpp:163-185)
+      uint32_t c = *cij;
+      for (int l = 0; l < k; ++l) {
+        const uint32_t b = TB ? B[j + (size_t)l * ldb] : B[l + (size_t)j * ldb];
+        if (SKIP0 && b == 0) continue;  // Rsyrk.cpp:169
+        c = add(c, mul(neg(b), A[i + (size_t)l * lda]));
+      }
+      *cij = c;
+    } else {  // t = sum a_li*b_lj from zero; C = alpha*t + beta*C (Rgemm.cpp:186-204)
+      uint32_t t = 0;
+      for (int l = 0; l < k; ++l) {
+        const uint32_t b = TB ? B[j + (size_t)l * ldb] : B[l + (size_t)j * ldb];
+        t = add(t, mul(A[l + (size_t)i * lda], b));
+      }
+      *cij = add(neg(t), *cij);
+    }
+  }
+  __syncthreads();
+}
+
+template <bool TA, bool TB, bool SKIP0>
+__device__ __forceinline__ void gemm(int m, int n, int k, const uint32_t* A, int lda, const uint32_t* B, int ldb,
+                                     uint32_t* C, int ldc, int tri, Shared& sh) {
+  if (m <= 0 || n <= 0 || k <= 0) return;  // Rgemm.cpp:134-136 (beta = 1)
+  if ((long)m * n * k <= 4096) {  // a tile's staging would cost more than the work
+    gemm_direct<TA, TB, SKIP0>(m, n, k, A, lda, B, ldb, C, ldc, tri);
+    return;
+  }
+  auto cover = [&](int T) { return (long)((m + T - 1) / T) * ((n + T - 1) / T) * T * T; };
+  // relative cost per covered element: fewer elements per lane amortise the LDS
+  // reads over fewer multiply-adds
+  const long c64 = cover(64) * 8, c32 = cover(32) * 9, c16 = cover(16) * 11;
+  lds_u32* lds = (lds_u32*)sh.tile;
+  if (c64 <= c32 && c64 <= c16) gemm_t<64, TA, TB, SKIP0>(m, n, k, A, lda, B, ldb, C, ldc, tri, lds);
+  else if (c32 <= c16) gemm_t<32, TA, TB, SKIP0>(m, n, k, A, lda, B, ldb, C, ldc, tri, lds);
+  else gemm_t<16, TA, TB, SKIP0>(m, n, k, A, lda, B, ldb, C, ldc, tri, lds);
+}
+
+// ---------------------------------------------------------------------------------
+// Triangular solves (Rtrsm.cpp), alpha = 1.  Left no-transpose forms sweep over k with
+// every (row, column) pair of a step in parallel; the transposed upper form (a row's
+// sum runs over earlier rows in ascending order) is the same sweep; the right-side
+// form sweeps over columns.
+// ---------------------------------------------------------------------------------
+// B := L^-1 B, L unit lower m x m (Rtrsm.cpp:139-155); b_k == 0 skips step k
+static __device__ __noinline__ void trsm_llnu(int m, int n, const uint32_t* L, int ldl, uint32_t* B, int ldb) {
+  for (int k = 0; k + 1 < m; ++k) {
+    const int rows = m - 1 - k;
+    for (int e = threadIdx.x; e < rows * n; e += NT) {
+      const int i = k + 1 + e % rows, j = e / rows;
+      const uint32_t bk = B[k + (size_t)j * ldb];
+      if (bk != 0) B[i + (size_t)j * ldb] = sub(B[i + (size_t)j * ldb], mul(bk, L[i + (size_t)k * ldl]));
+    }
+    __syncthreads();
+  }
+}
+
+// B := L^-1 B, L non-unit lower (Rtrsm.cpp:139-155 with the division of :147-149)
+static __device__ __noinline__ void trsm_llnn(int m, int n, const uint32_t* L, int ldl, uint32_t* B, int ldb) {
+  for (int k = 0; k < m; ++k) {
+    const uint32_t lkk = L[k + (size_t)k * ldl];
+    for (int j = threadIdx.x; j < n; j += NT) {
+      const uint32_t bk = B[k + (size_t)j * ldb];
+      if (bk != 0) B[k + (size_t)j * ldb] = div(bk, lkk);
+    }
+    __syncthreads();
+    const int rows = m - 1 - k;
+    for (int e = threadIdx.x; e < rows * n; e += NT) {
+      const int i = k + 1 + e % rows, j = e / rows;
+      const uint32_t bk = B[k + (size_t)j * ldb];  // nonzero stays nonzero after the division
+      if (bk != 0) B[i + (size_t)j * ldb] = sub(B[i + (size_t)j * ldb], mul(bk, L[i + (size_t)k * ldl]));
+    }
+    __syncthreads();
+  }
+}
+
+// B := U^-1 B, U non-unit upper (Rtrsm.cpp:120-137)
+static __device__ __noinline__ void trsm_lunn(int m, int n, const uint32_t* U, int ldu, uint32_t* B, int ldb) {
+  for (int k = m - 1; k >= 0; --k) {
+    const uint32_t ukk = U[k + (size_t)k * ldu];
+    for (int j = threadIdx.x; j < n; j += NT) {
+      const uint32_t bk = B[k + (size_t)j * ldb];
+      if (bk != 0) B[k + (size_t)j * ldb] = div(bk, ukk);
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < k * n; e += NT) {
+      const int i = e % k, j = e / k;
+      const uint32_t bk = B[k + (size_t)j * ldb];
+      if (bk != 0) B[i + (size_t)j * ldb] = sub(B[i + (size_t)j * ldb], mul(bk, U[i + (size_t)k * ldu]));
+    }
+    __syncthreads();
+  }
+}
+
+// B := U^-T B, U non-unit upper (Rtrsm.cpp:161-173): row i is b_i minus a_ki*b_k for
+// k = 0..i-1 in ascending order, then divided by u_ii; no zero skips in this form
+static __device__ __noinline__ void trsm_lutn(int m, int n, const uint32_t* U, int ldu, uint32_t* B, int ldb) {
+  for (int k = 0; k < m; ++k) {
+    const uint32_t ukk = U[k + (size_t)k * ldu];
+    for (int j = threadIdx.x; j < n; j += NT) B[k + (size_t)j * ldb] = div(B[k + (size_t)j * ldb], ukk);
+    __syncthreads();
+    const int rows = m - 1 - k;
+    for (int e = threadIdx.x; e < rows * n; e += NT) {
+      const int i = k + 1 + e % rows, j = e / rows;
+      B[i + (size_t)j * ldb] = sub(B[i + (size_t)j * ldb], mul(U[k + (size_t)i * ldu], B[k + (size_t)j * ldb]));
+    }
+    __syncthreads();
+  }
+}
+
+// B := L^-T B, L non-unit lower (Rtrsm.cpp:174-186): row i needs b_{i+1} for its first
+// term, so each column is one chain; one lane per column
+static __device__ __noinline__ void trsm_lltn(int m, int n, const uint32_t* L, int ldl, uint32_t* B, int ldb) {
+  for (int j = threadIdx.x; j < n; j += NT) {
+    uint32_t* b = B + (size_t)j * ldb;
+    for (int i = m - 1; i >= 0; --i) {
+      uint32_t t = b[i];
+      for (int k = i + 1; k < m; ++k) t = sub(t, mul(L[k + (size_t)i * ldl], b[k]));
+      b[i] = div(t, L[i + (size_t)i * ldl]);
+    }
+  }
+  __syncthreads();
+}
+
+// B := B L^-T, B m x n, L non-unit lower n x n (Rtrsm.cpp:263-285)
+static __device__ __noinline__ void trsm_rltn(int m, int n, const uint32_t* L, int ldl, uint32_t* B, int ldb) {
+  for (int k = 0; k < n; ++k) {
+    const uint32_t r = div(P_ONE, L[k + (size_t)k * ldl]);
+    for (int i = threadIdx.x; i < m; i += NT) B[i + (size_t)k * ldb] = mul(r, B[i + (size_t)k * ldb]);
+    __syncthreads();
+    const int cols = n - 1 - k;
+    for (int e = threadIdx.x; e < m * cols; e += NT) {
+      const int i = e % m, j = k + 1 + e / m;
+      const uint32_t t = L[j + (size_t)k * ldl];
+      if (t != 0) B[i + (size_t)j * ldb] = sub(B[i + (size_t)j * ldb], mul(t, B[i + (size_t)k * ldb]));
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// LU (Rgetrf_opencl / Rgetrf2)
+// ---------------------------------------------------------------------------------
+// First index of the largest |x_i| in signed pattern order, i < n (iRamax.cpp:31-88)
+static __device__ int iamax(int n, const uint32_t* x, Shared& sh) {
+  const int tid = threadIdx.x;
+  int bv = INT32_MIN, bi = INT32_MAX;  // bi == INT32_MAX: no element yet
+  for (int i = tid; i < n; i += NT) {
+    const int v = (int)abs_p(x[i]);
+    if (bi == INT32_MAX || v > bv) {
+      bv = v;
+      bi = i;
+    }
+  }
+  // the largest value wins, ties go to the smaller index: the sequential first maximum
+  auto take = [&](int ov, int oi) {
+    if (oi != INT32_MAX && (bi == INT32_MAX || ov > bv || (ov == bv && oi < bi))) {
+      bv = ov;
+      bi = oi;
+    }
+  };
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const int ov = __shfl_xor(bv, off), oi = __shfl_xor(bi, off);
+    take(ov, oi);
+  }
+  if ((tid & 63) == 0) {
+    sh.red_v[tid >> 6] = bv;
+    sh.red_i[tid >> 6] = bi;
+  }
+  __syncthreads();
+  bv = sh.red_v[0];
+  bi = sh.red_i[0];
+#pragma unroll
+  for (int w = 1; w < NT / 64; ++w) take(sh.red_v[w], sh.red_i[w]);
+  __syncthreads();
+  return bi;
+}
+
+// Leaf of Rgetrf2 for column j, rows j..n-1 (Rgetrf2.cpp:98-132)
+static __device__ void lu_leaf(int n, uint32_t* a, int lda, int j, int* ipiv, int& info, Shared& sh) {
+  uint32_t* col = at(a, lda, 0, j);
+  const int p = j + iamax(n - j, col + j, sh);
+  const uint32_t ap = col[p];
+  if (threadIdx.x == 0) ipiv[j] = p + 1;
+  if (p != j) {
+    // the pivot column itself is swapped only for a nonzero pivot (Rgetrf2.cpp:109-117);
+    // every other column follows ipiv (the Rlaswp calls of Rgetrf2 / Rgetrf_opencl)
+    for (int c = threadIdx.x; c < n; c += NT) {
+      if (c == j && ap == 0) continue;
+      uint32_t* x = at(a, lda, 0, c);
+      const uint32_t t = x[j];
+      x[j] = x[p];
+      x[p] = t;
+    }
+  }
+  __syncthreads();
+  if (ap != 0) {
+    // sfmin = FLT_MIN is below minpos, so every pivot but NaR takes the Rscal branch
+    if (!lt(abs_p(ap), P_MINPOS)) {
+      const uint32_t r = div(P_ONE, ap);
+      for (int i = j + 1 + threadIdx.x; i < n; i += NT) col[i] = mul(r, col[i]);
+    } else {
+      for (int i = j + 1 + threadIdx.x; i < n; i += NT) col[i] = div(col[i], ap);
+    }
+  } else if (info == 0) {
+    info = j + 1;
+  }
+  __syncthreads();
+}
+
+// The node of a Rgetrf2 / Rpotrf2 recursion on w columns (split n1 = len / 2) whose left
+// half ends at local column jr; false if jr ends no left half.
+static __device__ __forceinline__ bool node_after(int jr, int w, int& s, int& mid, int& e) {
+  s = 0;
+  e = w;
+  while (e - s > 1) {
+    mid = s + (e - s) / 2;
+    if (jr + 1 == mid) return true;
+    if (jr < mid) e = mid;
+    else s = mid;
+  }
+  return false;
+}
+
+// Rgetrf2 on columns p0..p0+w-1, rows p0..n-1 of the n x n matrix (rows >= columns at
+// every node, so n1 = columns / 2)
+static __device__ void lu_panel(int n, uint32_t* a, int lda, int p0, int w, int* ipiv, int& info, Shared& sh) {
+  for (int jr = 0; jr < w; ++jr) {
+    lu_leaf(n, a, lda, p0 + jr, ipiv, info, sh);
+    int s, mid, e;
+    if (node_after(jr, w, s, mid, e)) {
+      // Rgetrf2.cpp:151-159: A12 := L11^-1 A12; A22 -= A21 A12
+      const int n1 = mid - s, n2 = e - mid;
+      uint32_t* a12 = at(a, lda, p0 + s, p0 + mid);
+      trsm_llnu(n1, n2, at(a, lda, p0 + s, p0 + s), lda, a12, lda);
+      gemm<false, false, false>(n - p0 - mid, n2, n1, at(a, lda, p0 + mid, p0 + s), lda, a12, lda,
+                                at(a, lda, p0 + mid, p0 + mid), lda, 0, sh);
+    }
+  }
+}
+
+// Rgetrf_opencl on an n x n matrix with block size nb (Rgetrf_opencl.cpp:93-146)
+static __device__ void getrf(int n, uint32_t* a, int lda, int* ipiv, int nb, int& info, Shared& sh) {
+  if (nb <= 1 || nb >= n) {
+    lu_panel(n, a, lda, 0, n, ipiv, info, sh);
+    return;
+  }
+  for (int j = 0; j < n; j += nb) {
+    const int jb = min(nb, n - j);
+    lu_panel(n, a, lda, j, jb, ipiv, info, sh);
+    if (j + jb < n) {
+      uint32_t* a12 = at(a, lda, j, j + jb);
+      trsm_llnu(jb, n - j - jb, at(a, lda, j, j), lda, a12, lda);
+      gemm<false, false, false>(n - j - jb, n - j - jb, jb, at(a, lda, j + jb, j), lda, a12, lda,
+                                at(a, lda, j + jb, j + jb), lda, 0, sh);
+    }
+  }
+}
+
+// Rgetrs 'N' (Rgetrs.cpp:86-100): Rlaswp on B, then L and U solves
+static __device__ void getrs(int n, int nrhs, uint32_t* a, int lda, const int* ipiv, uint32_t* b, int ldb, Shared& sh) {
+  // the interchanges b_i <-> b_ipiv(i), i ascending, composed into one gather
+  if (threadIdx.x == 0) {
+    for (int r = 0; r < n; ++r) sh.perm[r] = r;
+    for (int i = 0; i < n; ++i) {
+      const int p = ipiv[i] - 1;
+      if (p != i && p >= 0 && p < n) {
+        const int t = sh.perm[i];
+        sh.perm[i] = sh.perm[p];
+        sh.perm[p] = t;
+      }
+    }
+  }
+  __syncthreads();
+  for (int c = 0; c < nrhs; ++c) {
+    uint32_t* bc = b + (size_t)c * ldb;
+    uint32_t v[MAX_N / NT];
+#pragma unroll
+    for (int q = 0; q < MAX_N / NT; ++q) {
+      const int r = threadIdx.x + q * NT;
+      if (r < n) v[q] = bc[sh.perm[r]];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < MAX_N / NT; ++q) {
+      const int r = threadIdx.x + q * NT;
+      if (r < n) bc[r] = v[q];
+    }
+    __syncthreads();
+  }
+  trsm_llnu(n, nrhs, a, lda, b, ldb);
+  trsm_lunn(n, nrhs, a, lda, b, ldb);
+}
+
+// ---------------------------------------------------------------------------------
+// Cholesky (Rpotrf_opencl / Rpotrf2); a failing step returns its 1-based column and
+// leaves the matrix as the reference leaves it
+// ---------------------------------------------------------------------------------
+template <bool UPPER>
+static __device__ int potrf2(int n, uint32_t* a, int lda, Shared& sh) {
+  for (int j = 0; j < n; ++j) {
+    const uint32_t v = a[j + (size_t)j * lda];
+    if (le(v, 0)) return j + 1;  // Rpotrf2.cpp:90 (NaR is below zero in pattern order)
+    __syncthreads();
+    if (threadIdx.x == 0) a[j + (size_t)j * lda] = pga::sqrt(v);
+    __syncthreads();
+    int s, mid, e;
+    if (node_after(j, n, s, mid, e)) {
+      const int n1 = mid - s, n2 = e - mid;
+      if (UPPER) {  // Rpotrf2.cpp:115-120
+        uint32_t* a12 = at(a, lda, s, mid);
+        trsm_lutn(n1, n2, at(a, lda, s, s), lda, a12, lda);
+        gemm<true, false, false>(n2, n2, n1, a12, lda, a12, lda, at(a, lda, mid, mid), lda, 2, sh);
+      } else {  // Rpotrf2.cpp:132-137
+        uint32_t* a21 = at(a, lda, mid, s);
+        trsm_rltn(n2, n1, at(a, lda, s, s), lda, a21, lda);
+        gemm<false, true, true>(n2, n2, n1, a21, lda, a21, lda, at(a, lda, mid, mid), lda, 1, sh);
+      }
+    }
+  }
+  return 0;
+}
+
+template <bool UPPER>
+static __device__ int potrf(int n, uint32_t* a, int lda, int nb, Shared& sh) {
+  if (nb <= 1 || nb >= n) return potrf2<UPPER>(n, a, lda, sh);
+  for (int j = 0; j < n; j += nb) {
+    const int jb = min(nb, n - j);
+    if (UPPER) {  // Rpotrf_opencl.cpp:100-123
+      gemm<true, false, false>(jb, jb, j, at(a, lda, 0, j), lda, at(a, lda, 0, j), lda, at(a, lda, j, j), lda, 2,
+                               sh);
+      const int f = potrf2<true>(jb, at(a, lda, j, j), lda, sh);
+      if (f) return f + j;
+      if (j + jb < n) {
+        gemm<true, false, false>(jb, n - j - jb, j, at(a, lda, 0, j), lda, at(a, lda, 0, j + jb), lda,
+                                 at(a, lda, j, j + jb), lda, 0, sh);
+        trsm_lutn(jb, n - j - jb, at(a, lda, j, j), lda, at(a, lda, j, j + jb), lda);
+      }
+    } else {  // Rpotrf_opencl.cpp:130-153
+      gemm<false, true, true>(jb, jb, j, at(a, lda, j, 0), lda, at(a, lda, j, 0), lda, at(a, lda, j, j), lda, 1,
+                              sh);
+      const int f = potrf2<false>(jb, at(a, lda, j, j), lda, sh);
+      if (f) return f + j;
+      if (j + jb < n) {
+        gemm<false, true, false>(n - j - jb, jb, j, at(a, lda, j + jb, 0), lda, at(a, lda, j, 0), lda,
+                                 at(a, lda, j + jb, j), lda, 0, sh);
+        trsm_rltn(n - j - jb, jb, at(a, lda, j, j), lda, at(a, lda, j + jb, j), lda);
+      }
+    }
+  }
+  return 0;
+}
+
+// Rpotrs (Rpotrs.cpp:86-104)
+static __device__ void potrs(bool upper, int n, int nrhs, const uint32_t* a, int lda, uint32_t* b, int ldb) {
+  if (upper) {
+    trsm_lutn(n, nrhs, a, lda, b, ldb);
+    trsm_lunn(n, nrhs, a, lda, b, ldb);
+  } else {
+    trsm_llnn(n, nrhs, a, lda, b, ldb);
+    trsm_lltn(n, nrhs, a, lda, b, ldb);
+  }
+}
+
+}  // namespace wg
+}  // namespace pga

@@ -46,6 +46,44 @@ def test_trsm(pga, oracle, side, uplo, transa, diag, alpha):
     _same(got, want)
 
 
+@pytest.mark.parametrize("diag", ["N", "U"])
+@pytest.mark.parametrize("special", [False, True])
+def test_trsm_left_lower_tall(pga, oracle, diag, special):
+    """Left lower solves with many rows (solved in row blocks with GEMM updates)."""
+    m, n = 150, 37
+    A = _tri(oracle, 72, m, False)
+    B = posit_matrix(oracle, 73, m, n)
+    B[40, :] = 0  # zero b_k skip their step
+    B[3, ::2] = 0
+    if special:  # NaR in L under skipped and unskipped multipliers
+        A[90, 40] = NAR
+        A[120, 3] = NAR
+        A[100, 7] = NAR
+    for alpha in (ONE, 0x3A000000):
+        want = B.copy(order="F")
+        assert oracle.rtrsm("L", "L", "N", diag, m, n, alpha, A, want) == 0
+        got = B.copy(order="F")
+        pga.Rtrsm("L", "L", "N", diag, m, n, alpha, A, m, got, m)
+        _same(got, want)
+
+
+@pytest.mark.parametrize("uplo", ["U", "L"])
+@pytest.mark.parametrize("diag", ["N", "U"])
+def test_trsm_left_wide(pga, oracle, uplo, diag):
+    """Left no-transpose solves with many right-hand sides (several columns per workgroup)."""
+    m, n = 40, 2304
+    A = _tri(oracle, 70, m, uplo == "U")
+    B = posit_matrix(oracle, 71, m, n)
+    B[5, ::7] = 0  # zero b_k entries skip their step
+    B[30, 1::5] = 0
+    for alpha in (ONE, 0x3A000000):
+        want = B.copy(order="F")
+        assert oracle.rtrsm("L", uplo, "N", diag, m, n, alpha, A, want) == 0
+        got = B.copy(order="F")
+        pga.Rtrsm("L", uplo, "N", diag, m, n, alpha, A, m, got, m)
+        _same(got, want)
+
+
 @pytest.mark.parametrize("uplo", ["U", "L"])
 @pytest.mark.parametrize("trans", ["N", "T"])
 @pytest.mark.parametrize("alpha,beta", [(MONE, ONE), (0x3C000000, 0), (0, 0x44000000)])
