@@ -45,6 +45,7 @@ __device__ void run_task(const FactorTask& t, Shared& sh) {
 // its loops, and one workgroup's barrier-bound steps overlap the other's tile work
 __global__ __launch_bounds__(NT, 2) void k_factor_tasks(const FactorTask* __restrict__ tasks, int ntasks,
                                                      int* __restrict__ counter) {
+  PGA_KERNEL_VGPR_FLOOR();
   __shared__ Shared sh;
   for (;;) {
     if (threadIdx.x == 0) sh.bcast = atomicAdd(counter, 1);
@@ -57,6 +58,7 @@ __global__ __launch_bounds__(NT, 2) void k_factor_tasks(const FactorTask* __rest
 }
 
 __global__ void k_reset_counter(int* counter) {
+  PGA_KERNEL_VGPR_FLOOR();
   if (threadIdx.x == 0) *counter = 0;
 }
 

@@ -35,6 +35,7 @@ hipError_t la_potrs(hipStream_t s, bool upper, int n, int nrhs, const uint32_t* 
 namespace {
 
 __global__ void k_set_int(int* p, int v) {
+  PGA_KERNEL_VGPR_FLOOR();
   if (threadIdx.x == 0) *p = v;
 }
 

@@ -30,6 +30,7 @@ __global__ __launch_bounds__(WG) void k_trsm_left(int m, int n, uint32_t alpha, 
                                                   const uint32_t* __restrict__ A, int lda,
                                                   uint32_t* __restrict__ B, int ldb,
                                                   uint32_t* __restrict__ work, const int* stop) {
+  PGA_KERNEL_VGPR_FLOOR();
   const int j = blockIdx.x;
   if (j >= n || stopped(stop)) return;
   uint32_t* b = B + (size_t)j * ldb;
@@ -115,6 +116,7 @@ __global__ __launch_bounds__(WG) void k_trsm_left_cols(int m, int n, uint32_t al
                                                        const uint32_t* __restrict__ A, int lda,
                                                        uint32_t* __restrict__ B, int ldb, int cpw,
                                                        const int* stop) {
+  PGA_KERNEL_VGPR_FLOOR();
   const int c0 = blockIdx.x * cpw;
   if (c0 >= n || stopped(stop)) return;
   const int C = min(cpw, n - c0);
@@ -153,6 +155,7 @@ __global__ __launch_bounds__(WG) void k_trsm_right(int m, int n, uint32_t alpha,
                                                    const uint32_t* __restrict__ A, int lda,
                                                    uint32_t* __restrict__ B, int ldb,
                                                    const int* stop) {
+  PGA_KERNEL_VGPR_FLOOR();
   const int i = blockIdx.x * WG + threadIdx.x;
   if (i >= m || stopped(stop)) return;
   auto a = [&](int r, int c) { return A[r + (size_t)c * lda]; };
@@ -217,6 +220,7 @@ __global__ __launch_bounds__(WG) void k_trsm_right_sweep(int m, int n, uint32_t 
                                                          const uint32_t* __restrict__ A, int lda,
                                                          uint32_t* __restrict__ B, int ldb, int rows,
                                                          const int* stop) {
+  PGA_KERNEL_VGPR_FLOOR();
   const int r0 = blockIdx.x * rows;
   if (r0 >= m || stopped(stop)) return;
   const int R = min(rows, m - r0);
@@ -245,6 +249,7 @@ __global__ __launch_bounds__(WG) void k_trsm_right_sweep(int m, int n, uint32_t 
 
 // B = 0 (alpha == 0, Rtrsm.cpp:101-108)
 __global__ __launch_bounds__(WG) void k_zero(int m, int n, uint32_t* B, int ldb, const int* stop) {
+  PGA_KERNEL_VGPR_FLOOR();
   const int i = blockIdx.x * WG + threadIdx.x;
   if (i >= m || stopped(stop)) return;
   for (int j = 0; j < n; ++j) B[i + (size_t)j * ldb] = 0;
@@ -257,6 +262,7 @@ __global__ __launch_bounds__(WG) void k_gemv(int trans, int m, int n, uint32_t a
                                              const uint32_t* __restrict__ A, int lda,
                                              const uint32_t* __restrict__ x, int incx, uint32_t beta,
                                              uint32_t* __restrict__ y, int incy) {
+  PGA_KERNEL_VGPR_FLOOR();
   const int leny = trans ? n : m, lenx = trans ? m : n;
   const int r = blockIdx.x * WG + threadIdx.x;
   if (r >= leny) return;
@@ -279,6 +285,7 @@ __global__ __launch_bounds__(WG) void k_gemv(int trans, int m, int n, uint32_t a
 
 // SCAL (Rscal.cpp:62-119)
 __global__ __launch_bounds__(WG) void k_scal(int n, uint32_t da, uint32_t* x, int incx) {
+  PGA_KERNEL_VGPR_FLOOR();
   for (long i = blockIdx.x * (long)WG + threadIdx.x; i < n; i += (long)gridDim.x * WG)
     x[i * incx] = mul(da, x[i * incx]);
 }
@@ -286,7 +293,9 @@ __global__ __launch_bounds__(WG) void k_scal(int n, uint32_t da, uint32_t* x, in
 // IAMAX (iRamax.cpp:31-88): first index of the largest |x| in signed pattern order.
 // The largest value wins, ties go to the smaller index (a total order, so the reduction
 // order does not matter): the sequential first maximum.
-__device__ void block_iamax(int n, const uint32_t* x, long incx, int* out_idx) {
+// With out_val, the element at the index (its pattern) is broadcast too, read before the
+// closing barrier, so callers may overwrite it once the call returns.
+__device__ void block_iamax(int n, const uint32_t* x, long incx, int* out_idx, uint32_t* out_val = nullptr) {
   __shared__ int sv[WG / 64], si[WG / 64];
   const int tid = threadIdx.x;
   int bv = INT32_MIN, bi = INT32_MAX;  // bi == INT32_MAX: nothing seen
@@ -314,11 +323,15 @@ __device__ void block_iamax(int n, const uint32_t* x, long incx, int* out_idx) {
   bi = si[0];
 #pragma unroll
   for (int w = 1; w < WG / 64; ++w) take(sv[w], si[w]);
-  if (tid == 0) *out_idx = (bi == INT32_MAX) ? -1 : bi;  // 0-based, -1 when n == 0
+  if (tid == 0) {
+    *out_idx = (bi == INT32_MAX) ? -1 : bi;  // 0-based, -1 when n == 0
+    if (out_val) *out_val = (bi == INT32_MAX) ? 0u : x[(long)bi * incx];
+  }
   __syncthreads();
 }
 
 __global__ __launch_bounds__(WG) void k_iamax(int n, const uint32_t* x, int incx, int* result) {
+  PGA_KERNEL_VGPR_FLOOR();
   __shared__ int idx;
   if (n < 1 || incx <= 0) {
     if (threadIdx.x == 0) *result = 0;
@@ -332,6 +345,7 @@ __global__ __launch_bounds__(WG) void k_iamax(int n, const uint32_t* x, int incx
 // (1-based) of the block starting at `a`.
 __global__ __launch_bounds__(WG) void k_laswp(int n, uint32_t* a, int lda, int k1, int k2,
                                               const int* ipiv, int incx, int ipoff, const int* stop) {
+  PGA_KERNEL_VGPR_FLOOR();
   const int col = blockIdx.x * WG + threadIdx.x;
   if (col >= n || stopped(stop)) return;
   int ix0, i1, i2, inc;
@@ -365,6 +379,7 @@ __global__ __launch_bounds__(WG) void k_laswp(int n, uint32_t* a, int lda, int k
 // ---------------------------------------------------------------------------------
 __global__ __launch_bounds__(WG) void k_getrf2_row(uint32_t* a, int* ipiv, int* info, int row_off,
                                                    int col_off) {
+  PGA_KERNEL_VGPR_FLOOR();
   if (threadIdx.x != 0) return;
   ipiv[0] = row_off + 1;
   if (a[0] == 0 && *info == 0) *info = col_off + 1;
@@ -385,12 +400,16 @@ constexpr int LEAF_TRSM_N1 = 16, LEAF_TRSM_N2 = 32;
 __global__ __launch_bounds__(WG) void k_getrf2_leaf(int m_root, int w_root, uint32_t* root, int lda, int j,
                                                     int* ipiv, int* info, int row_off, int col_off, int ts,
                                                     int tm, int te) {
+  PGA_KERNEL_VGPR_FLOOR();
   __shared__ int piv;
+  __shared__ uint32_t pval;
   __shared__ uint32_t sL[LEAF_TRSM_N1][LEAF_TRSM_N1], sB[LEAF_TRSM_N1][LEAF_TRSM_N2];
   uint32_t* col = root + (size_t)j * lda;
-  block_iamax(m_root - j, col + j, 1, &piv);
+  // the pivot's value comes from the search, not from col[p]: the lane that swaps the
+  // pivot column may overwrite col[p] before a later wave would have read it
+  block_iamax(m_root - j, col + j, 1, &piv, &pval);
   const int p = j + piv;
-  const uint32_t ap = col[p];
+  const uint32_t ap = pval;
   if (threadIdx.x == 0) ipiv[j] = row_off + p + 1;
   if (p != j) {
     for (int c = threadIdx.x; c < w_root; c += WG) {
@@ -434,6 +453,7 @@ __global__ __launch_bounds__(WG) void k_getrf2_leaf(int m_root, int w_root, uint
 // Rpotrf2 leaf (Rpotrf2.cpp:86-100): not positive (or NaR) -> info, else sqrt
 // ---------------------------------------------------------------------------------
 __global__ void k_potrf2_leaf(uint32_t* a, int* info, int col_off) {
+  PGA_KERNEL_VGPR_FLOOR();
   if (threadIdx.x != 0 || *info) return;
   uint32_t v = a[0];
   if (le(v, 0)) {
@@ -453,6 +473,7 @@ constexpr int POTRF2_ONE_WG_MAX = 64;
 
 template <bool UPPER>
 __global__ __launch_bounds__(WG, 2) void k_potrf2_wg(int n, uint32_t* a, int lda, int* info, int col_off) {
+  PGA_KERNEL_VGPR_FLOOR();
   __shared__ wg::Shared sh;
   __shared__ uint32_t blk[POTRF2_ONE_WG_MAX * POTRF2_ONE_WG_MAX];
   if (*info) return;  // every lane reads the same flag
@@ -493,6 +514,7 @@ static hipError_t trsm(const Ctx& c, bool left, bool upper, bool trans, bool nou
 // B := alpha * scale; elementwise (the first step of every Rtrsm branch with alpha != 1)
 __global__ __launch_bounds__(WG) void k_scale_block(int m, int n, uint32_t alpha, uint32_t* B, int ldb,
                                                     const int* stop) {
+  PGA_KERNEL_VGPR_FLOOR();
   if (stopped(stop)) return;
   for (long e = blockIdx.x * (long)WG + threadIdx.x; e < (long)m * n; e += (long)gridDim.x * WG)
     B[e % m + (size_t)(e / m) * ldb] = mul(alpha, B[e % m + (size_t)(e / m) * ldb]);

@@ -84,3 +84,11 @@ void* scratch(int slot, size_t bytes, pga_status* st);
     hipError_t _e = (call);                                      \
     if (_e != hipSuccess) return ::pga::from_hip(_e);            \
   } while (0)
+
+// First statement of every kernel: the empty asm marks v31 as written, so each wave
+// allocates at least 32 VGPRs.  On MI355X (gfx950), waves of kernels with smaller
+// allocations (16 or 24 VGPRs) intermittently computed wrong values while long-running
+// compute kernels of another stream or process shared their SIMDs; the same code with
+// 32 or more VGPRs allocated computed correct values in every run.  Occupancy is not
+// affected (32 VGPRs still allow the maximum number of waves per SIMD).
+#define PGA_KERNEL_VGPR_FLOOR() asm volatile("" ::: "v31")

@@ -31,6 +31,7 @@ __device__ double atomic_max_nonneg(double* addr, double v) {
 __global__ __launch_bounds__(WG) void k_residual(int n, const uint32_t* __restrict__ A, int lda,
                                                  const uint32_t* __restrict__ x, const uint32_t* __restrict__ b,
                                                  double* out /* [0]=max|r|, [1]=||A||, [2]=||x|| */) {
+  PGA_KERNEL_VGPR_FLOOR();
   int i = blockIdx.x * WG + threadIdx.x;
   if (i >= n) return;
   double r = -to_double(b[i]), arow = 0.0;
@@ -45,6 +46,7 @@ __global__ __launch_bounds__(WG) void k_residual(int n, const uint32_t* __restri
 }
 
 __global__ void k_zero3(double* p) {
+  PGA_KERNEL_VGPR_FLOOR();
   if (threadIdx.x < 3) p[threadIdx.x] = 0.0;
 }
 

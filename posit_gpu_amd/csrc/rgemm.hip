@@ -19,6 +19,7 @@ __global__ __launch_bounds__((BM / TM) * (BN / TN)) void rgemm_tiled(
     int m, int n, int k, uint32_t alpha, const uint32_t* __restrict__ A, int lda,
     const uint32_t* __restrict__ B, int ldb, uint32_t beta, uint32_t* __restrict__ C, int ldc,
     int tiles_m, int tri, const int* __restrict__ stop) {
+  PGA_KERNEL_VGPR_FLOOR();
   // tiles in column-major order of the tile grid: consecutive workgroups share B slices
   const int bid = blockIdx.x;
   const int i0 = (bid % tiles_m) * BM;
@@ -38,6 +39,7 @@ template <int BM, int BN, int BK, int TM, int TN, bool TA, bool TB>
 __global__ __launch_bounds__((BM / TM) * (BN / TN)) void rgemm_batched_persistent(
     const BatchProblem* __restrict__ probs, const int* __restrict__ tile_start, int nprob, int total_tiles,
     int* __restrict__ counter) {
+  PGA_KERNEL_VGPR_FLOOR();
   __shared__ int s_tile;
   for (;;) {
     if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1);
@@ -61,6 +63,7 @@ __global__ __launch_bounds__((BM / TM) * (BN / TN)) void rgemm_batched_persisten
 }
 
 __global__ void reset_counter(int* counter) {
+  PGA_KERNEL_VGPR_FLOOR();
   if (threadIdx.x == 0) *counter = 0;
 }
 
@@ -74,6 +77,7 @@ __global__ __launch_bounds__(256) void rgemm_exact(int m, int n, int k, uint32_t
                                                    const uint32_t* __restrict__ B, int ldb,
                                                    uint32_t beta, uint32_t* __restrict__ C,
                                                    int ldc, int tri, const int* __restrict__ stop) {
+  PGA_KERNEL_VGPR_FLOOR();
   int i = blockIdx.x * 16 + (threadIdx.x % 16);
   int j = blockIdx.y * 16 + (threadIdx.x / 16);
   if (i >= m || j >= n) return;
@@ -102,6 +106,7 @@ __global__ __launch_bounds__(256) void rgemm_exact(int m, int n, int k, uint32_t
 // alpha == 0 (Rgemm.cpp:142-157): C = beta==0 ? 0 : beta*C
 __global__ __launch_bounds__(256) void scale_c(int m, int n, uint32_t beta, uint32_t* C, int ldc,
                                                int j0, int tri, const int* stop) {
+  PGA_KERNEL_VGPR_FLOOR();
   int i = blockIdx.x * 256 + threadIdx.x;
   int j = blockIdx.y;
   if (i >= m) return;

@@ -65,15 +65,18 @@ void* scratch(int slot, size_t bytes, pga_status* st) {
 // elementwise kernels
 // ---------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_from_f64(const double* x, uint32_t* p, size_t n) {
+  PGA_KERNEL_VGPR_FLOOR();
   for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
     p[i] = pga::from_double(x[i]);
 }
 __global__ __launch_bounds__(256) void k_to_f64(const uint32_t* p, double* x, size_t n) {
+  PGA_KERNEL_VGPR_FLOOR();
   for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
     x[i] = pga::to_double(p[i]);
 }
 __global__ __launch_bounds__(256) void k_elementwise(int op, const uint32_t* a, const uint32_t* b,
                                                      uint32_t* o, size_t n) {
+  PGA_KERNEL_VGPR_FLOOR();
   for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
     uint32_t x = a[i];
     uint32_t y = (op == 4) ? 0u : b[i];
@@ -101,6 +104,7 @@ __device__ __forceinline__ double uniform01(uint64_t seed, uint64_t i) {
 
 __global__ __launch_bounds__(256) void k_fill_uniform(uint32_t* p, size_t n, uint64_t seed,
                                                       double lo, double hi) {
+  PGA_KERNEL_VGPR_FLOOR();
   for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
     p[i] = pga::from_double(lo + (hi - lo) * uniform01(seed, i));
 }

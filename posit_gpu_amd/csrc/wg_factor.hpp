@@ -41,6 +41,7 @@ struct Shared {
   int red_v[NT / 64];
   int red_i[NT / 64];
   int bcast;
+  uint32_t pivot;  // the pattern at iamax's index
 };
 
 static __device__ __forceinline__ uint32_t* at(uint32_t* a, int lda, int i, int j) {
@@ -219,7 +220,8 @@ static __device__ __noinline__ void trsm_rltn(int m, int n, const uint32_t* L, i
 // ---------------------------------------------------------------------------------
 // LU (Rgetrf_opencl / Rgetrf2)
 // ---------------------------------------------------------------------------------
-// First index of the largest |x_i| in signed pattern order, i < n (iRamax.cpp:31-88)
+// First index of the largest |x_i| in signed pattern order, i < n (iRamax.cpp:31-88);
+// x at that index is left in sh.pivot, read before the closing barrier.
 static __device__ int iamax(int n, const uint32_t* x, Shared& sh) {
   const int tid = threadIdx.x;
   int bv = INT32_MIN, bi = INT32_MAX;  // bi == INT32_MAX: no element yet
@@ -251,6 +253,7 @@ static __device__ int iamax(int n, const uint32_t* x, Shared& sh) {
   bi = sh.red_i[0];
 #pragma unroll
   for (int w = 1; w < NT / 64; ++w) take(sh.red_v[w], sh.red_i[w]);
+  if (threadIdx.x == 0) sh.pivot = (bi == INT32_MAX) ? 0u : x[bi];
   __syncthreads();
   return bi;
 }
@@ -259,7 +262,9 @@ static __device__ int iamax(int n, const uint32_t* x, Shared& sh) {
 static __device__ void lu_leaf(int n, uint32_t* a, int lda, int j, int* ipiv, int& info, Shared& sh) {
   uint32_t* col = at(a, lda, 0, j);
   const int p = j + iamax(n - j, col + j, sh);
-  const uint32_t ap = col[p];
+  // not col[p]: the lane that swaps the pivot column may overwrite it before a later
+  // wave would read it
+  const uint32_t ap = sh.pivot;
   if (threadIdx.x == 0) ipiv[j] = p + 1;
   if (p != j) {
     // the pivot column itself is swapped only for a nonzero pivot (Rgetrf2.cpp:109-117);
