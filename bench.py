@@ -79,10 +79,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--config", default="gemm",
-                    choices=["gemm", "lu", "cholesky", "batched", "batched_factor", "serve", "copycal"],
+                    choices=["gemm", "lu", "cholesky", "batched", "batched_factor", "blocked", "serve", "copycal"],
                     help="gemm: trailing-update GEMMs (headline); lu / cholesky: blocked "
                          "factorization + solve at n=8192; batched: 4096 small GEMMs; batched_factor: "
-                         "2048 LU + 2048 Cholesky factorizations of n in [32, 512]; serve: Poisson "
+                         "2048 LU + 2048 Cholesky factorizations of n in [32, 512]; blocked: LU + "
+                         "upper Cholesky solves of n in [513, 4096] in one blocked batch; serve: Poisson "
                          "stream of solve requests, served vs plain per-request; copycal: a "
                          "streaming kernel of known size (2 x 1 GiB read, 1 GiB written) that "
                          "calibrates the FETCH_SIZE / WRITE_SIZE counters")
@@ -117,6 +118,8 @@ def main():
         return bench_batched_factor(pga, info, args)
     if args.config == "serve":
         return bench_serve(pga, info, args)
+    if args.config == "blocked":
+        return bench_blocked(pga, info, args)
 
     bufs = []
     for i, n in enumerate(SHAPES):
@@ -456,6 +459,94 @@ def bench_batched_factor(pga, info, args):
                    "bitwise_mismatches_vs_per_call": mismatches, "device": info["name"]},
     }
     print(json.dumps(out), flush=True)
+
+
+def bench_blocked(pga, info, args):
+    """Factor-and-solve tasks too large for one workgroup (LU and upper Cholesky, n
+    log-uniform in [513, 4096], 1-16 right-hand sides) in one blocked batch, against the
+    per-call device drivers one problem after another; every factor and solution compared
+    bit for bit."""
+    rng = np.random.default_rng(4096)
+    count = max(2, args.requests // 4)
+    N = 4096
+    G = pga.DeviceArray((N, N))
+    pga.fill_uniform(G, 77)
+    S = _spd_device(pga, N, 78)
+    Gh, Sh = G.to_host(), S.to_host()
+    items = []
+    for i in range(count):
+        n = int(round(math.exp(rng.uniform(math.log(513), math.log(4096)))))
+        kind = "gesv" if i % 2 == 0 else "posv"
+        nrhs = int(rng.integers(1, 17))
+        a0 = pga.DeviceArray.from_host(np.asfortranarray((Gh if kind == "gesv" else Sh)[:n, :n]))
+        b0 = pga.DeviceArray.from_host(np.asfortranarray(Gh[:n, N - nrhs:]))
+        items.append(dict(kind=kind, n=n, nrhs=nrhs, a0=a0, b0=b0, a=pga.DeviceArray((n, n)), b=pga.DeviceArray((n, nrhs)),
+                          ipiv=pga.DeviceArray(n, np.int32, "C") if kind == "gesv" else None,
+                          info=pga.DeviceArray(1, np.int32, "C")))
+    pga.synchronize()
+    log(f"inputs ready: {count} problems, n median {int(np.median([it['n'] for it in items]))}")
+
+    def restore():
+        for it in items:
+            it["a"].copy_from(it["a0"])
+            it["b"].copy_from(it["b0"])
+        pga.synchronize()
+
+    ops = sum((2.0 / 3.0 if it["kind"] == "gesv" else 1.0 / 3.0) * it["n"] ** 3 for it in items)
+    tasks = [dict(kind=it["kind"], n=it["n"], a=it["a"], lda=it["n"], ipiv=it["ipiv"], info=it["info"], uplo="U", nb=64,
+                  b=it["b"], ldb=it["n"], nrhs=it["nrhs"]) for it in items]
+    plan = pga.FactorPlan(tasks)
+    times = []
+    for _ in range(max(1, min(args.steps, 3))):
+        restore()
+        times.append(_timed(pga, plan.execute))
+    t_plan = min(times)
+    log(f"blocked batch {times}")
+    got = [(it["a"].to_host(), it["b"].to_host(), int(it["info"].to_host()[0])) for it in items]
+    restore()
+
+    def per_call():
+        for it in items:
+            n = it["n"]
+            if it["kind"] == "gesv":
+                pga.getrf_device(n, n, it["a"], n, it["ipiv"], it["info"], 64)
+                pga.getrs_device("N", n, it["nrhs"], it["a"], n, it["ipiv"], it["b"], n)
+            else:
+                pga.potrf_device("U", n, it["a"], n, it["info"], 64)
+                pga.potrs_device("U", n, it["nrhs"], it["a"], n, it["b"], n)
+
+    t_plain = _timed(pga, per_call)
+    log(f"per-call {t_plain:.3f} s")
+    mism = sum(0 if (np.array_equal(a, it["a"].to_host()) and np.array_equal(b, it["b"].to_host()) and
+                     inf == int(it["info"].to_host()[0])) else 1 for it, (a, b, inf) in zip(items, got))
+    out = {
+        "metric": "blocked batched posit32 solves/s", "value": round(count / t_plan, 2), "unit": "solves/s",
+        "n_gpus": 1, "higher_is_better": True, "dtype": "posit32 (integer VALU)",
+        "data": "synthetic: leading blocks of a uniform[-1,1) matrix and of an SPD M*M^T/n + I matrix, posit32",
+        "config": {"workload": f"{count} factor-and-solve tasks, alternating Rgetrf_opencl+Rgetrs and upper "
+                               "Rpotrf_opencl+Rpotrs (nb 64), n log-uniform in [513, 4096], 1-16 RHS",
+                   "plan_s": round(t_plan, 4), "per_call_s": round(t_plain, 4),
+                   "speedup_vs_per_call": round(t_plain / t_plan, 2), "gops": round(ops / t_plan / 1e9, 2),
+                   "bitwise_mismatches_vs_per_call": mism, "device": info["name"]},
+    }
+    print(json.dumps(out), flush=True)
+
+
+def _spd_device(pga, N, seed):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from datagen import uniform_doubles
+    M = uniform_doubles(seed, N * N).reshape(N, N)
+    S = M @ M.T
+    S /= N
+    S[np.diag_indices(N)] += 1.0
+    del M
+    dS = pga.DeviceArray.from_host(np.asfortranarray(S))
+    del S
+    P = pga.DeviceArray((N, N))
+    pga._check(pga._lib.pga_p32_from_f64_device(dS.ptr, P.ptr, N * N, None), "from_f64")
+    pga.synchronize()
+    dS.free()
+    return P
 
 
 def _serve_requests(pga, count, seed):

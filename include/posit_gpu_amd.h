@@ -188,11 +188,15 @@ pga_status pga_batch_plan_destroy(pga_batch_plan *plan);
  *   GETRS  = pga_getrs_device 'N' on an earlier GETRF's factors (Rgetrs.cpp:86-100)
  *   POTRS  = pga_potrs_device  (Rpotrs.cpp:86-104)
  *   GESV / POSV = the factorization, then the solve when info == 0.
- * Tasks are square, n <= PGA_FACTOR_TASK_MAX_N; a, ipiv, b, info are device pointers
- * (info: one int, written with the factorization's info, 0 for a solve).  Tasks run
- * most expensive first; tasks must not share output memory.  The reference has no
- * batched form: this extends Rgetrf_opencl / Rpotrf_opencl / Rgetrs / Rpotrs. */
+ * Tasks are square, n <= PGA_BLOCKED_TASK_MAX_N; a, ipiv, b, info are device pointers
+ * (info: one int, written with the factorization's info, 0 for a solve).  Tasks with
+ * n <= PGA_FACTOR_TASK_MAX_N run one workgroup each in one persistent launch, most
+ * expensive first; larger tasks advance together one column block at a time (a few
+ * launches per block for all of them).  Tasks must not share output memory.  The
+ * reference has no batched form: this extends Rgetrf_opencl / Rpotrf_opencl / Rgetrs /
+ * Rpotrs. */
 #define PGA_FACTOR_TASK_MAX_N 512
+#define PGA_BLOCKED_TASK_MAX_N 16384
 enum {
   PGA_TASK_GETRF = 0,
   PGA_TASK_POTRF = 1,
@@ -230,8 +234,11 @@ pga_status pga_factor_plan_destroy(pga_factor_plan *plan);
  * later re-solves stay resident and are evicted least-recently-used when memory is
  * short.  In PGA_SERVER_BATCHED mode every admitted request with n <= batch_max_n joins
  * the next persistent task launch (pga_factor_plan semantics) on one of batch_slots
- * streams as soon as a slot is free, while larger requests run the per-call drivers on
- * `workers` threads with a stream each; all of it overlaps on the device.  In
+ * streams as soon as a slot is free, while larger requests are taken by `workers`
+ * threads with a stream each: a worker runs every large request ready at that moment as
+ * one blocked batch (pga_factor_plan semantics for orders above PGA_FACTOR_TASK_MAX_N;
+ * orders above PGA_BLOCKED_TASK_MAX_N run alone through the per-call drivers); all of it
+ * overlaps on the device.  In
  * PGA_SERVER_PLAIN mode one worker runs each request alone through the per-call drivers
  * (pga_rgetrf_device + pga_rgetrs_device, pga_rpotrf_device + pga_rpotrs_device), one
  * after another.  Both modes give identical bits: a factor-and-solve request computes

@@ -534,10 +534,11 @@ class BatchPlan:
 
 
 # ---------------------------------------------------------------------------------
-# batched factorizations / solves (one persistent launch)
+# batched factorizations / solves
 # ---------------------------------------------------------------------------------
 TASK_GETRF, TASK_POTRF, TASK_GETRS, TASK_POTRS, TASK_GESV, TASK_POSV = range(6)
-FACTOR_TASK_MAX_N = 512
+FACTOR_TASK_MAX_N = 512        # one workgroup per task up to this order
+BLOCKED_TASK_MAX_N = 16384     # larger tasks advance together one column block at a time
 _TASK_KINDS = {"getrf": TASK_GETRF, "potrf": TASK_POTRF, "getrs": TASK_GETRS, "potrs": TASK_POTRS,
                "gesv": TASK_GESV, "posv": TASK_POSV}
 
@@ -558,7 +559,9 @@ _sig("pga_factor_plan_destroy", [_P])
 
 
 class FactorPlan:
-    """Many independent factorizations / solves in one persistent launch.
+    """Many independent factorizations / solves: orders up to FACTOR_TASK_MAX_N in one
+    persistent launch (a workgroup per task), larger ones (up to BLOCKED_TASK_MAX_N)
+    together one column block at a time.
 
     tasks: list of dicts with keys kind ("getrf", "potrf", "getrs", "potrs", "gesv",
     "posv" or the TASK_* value), n, a, lda and, as the kind needs, uplo, nb, ipiv, b,

@@ -69,11 +69,14 @@ __global__ void k_reset_counter(int* counter) {
 // Plans (C ABI)
 // ===================================================================================
 struct pga_factor_plan {
-  int count = 0;
+  int count = 0;                       // tasks run one workgroup each
   int grid = 0;
   std::vector<pga::FactorTask> host;  // scheduling order
   pga::FactorTask* d_tasks = nullptr;
   int* d_counter = nullptr;
+  pga::BlockedPlan* blocked = nullptr;  // tasks larger than one workgroup handles
+  int blocked_count = 0;
+  void* d_blocked = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 };
@@ -101,7 +104,11 @@ hipError_t factor_tasks_launch(const FactorTask* d_tasks, int count, int* d_coun
 namespace {
 
 hipError_t run(pga_factor_plan& p, hipStream_t s) {
-  return factor_tasks_launch(p.d_tasks, p.count, p.d_counter, p.grid, s);
+  if (p.count) {
+    hipError_t e = factor_tasks_launch(p.d_tasks, p.count, p.d_counter, p.grid, s);
+    if (e != hipSuccess) return e;
+  }
+  return p.blocked ? blocked_plan_launch(p.blocked, s) : hipSuccess;
 }
 
 // relative cost of a task, for scheduling the longest first
@@ -135,7 +142,7 @@ pga_status pga_factor_plan_create(pga_factor_plan** out, int count, const pga_fa
     return PGA_ERROR_ILLEGAL_ARGUMENT;
   }
   auto* p = new pga_factor_plan();
-  std::vector<FactorTask> v;
+  std::vector<FactorTask> v, big;
   for (int i = 0; i < count; ++i) {
     const pga_factor_task& q = tasks[i];
     FactorTask t{};
@@ -154,7 +161,7 @@ pga_status pga_factor_plan_create(pga_factor_plan** out, int count, const pga_fa
     const bool solve = (t.kind == PGA_TASK_GETRS || t.kind == PGA_TASK_POTRS || t.kind == PGA_TASK_GESV ||
                         t.kind == PGA_TASK_POSV);
     bool ok = t.kind >= PGA_TASK_GETRF && t.kind <= PGA_TASK_POSV;
-    ok = ok && t.n >= 0 && t.n <= PGA_FACTOR_TASK_MAX_N && t.lda >= std::max(1, t.n) && t.a;
+    ok = ok && t.n >= 0 && t.n <= PGA_BLOCKED_TASK_MAX_N && t.lda >= std::max(1, t.n) && t.a;
     ok = ok && (lu ? t.ipiv != nullptr : t.uplo != 0);
     if (solve) ok = ok && t.nrhs >= 0 && (t.nrhs == 0 || (t.b && t.ldb >= std::max(1, t.n)));
     if (!ok) {
@@ -163,7 +170,7 @@ pga_status pga_factor_plan_create(pga_factor_plan** out, int count, const pga_fa
       return PGA_ERROR_ILLEGAL_ARGUMENT;
     }
     if (!solve) t.nrhs = 0;
-    v.push_back(t);
+    (t.n <= PGA_FACTOR_TASK_MAX_N ? v : big).push_back(t);
   }
   std::vector<int> idx(v.size());
   for (size_t i = 0; i < idx.size(); ++i) idx[i] = (int)i;
@@ -175,9 +182,18 @@ pga_status pga_factor_plan_create(pga_factor_plan** out, int count, const pga_fa
   if (e == hipSuccess && p->count) e = hipMalloc(&p->d_tasks, sizeof(FactorTask) * p->count);
   if (e == hipSuccess && p->count)
     e = hipMemcpy(p->d_tasks, p->host.data(), sizeof(FactorTask) * p->count, hipMemcpyHostToDevice);
+  if (e == hipSuccess && !big.empty()) {
+    p->blocked = blocked_plan_build(big.data(), (int)big.size());
+    p->blocked_count = (int)big.size();
+    e = hipMalloc(&p->d_blocked, blocked_plan_bytes(p->blocked));
+    if (e == hipSuccess) e = blocked_plan_upload(p->blocked, p->d_blocked, runtime().stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(runtime().stream);
+  }
   if (e != hipSuccess) {
     if (p->d_counter) (void)hipFree(p->d_counter);
     if (p->d_tasks) (void)hipFree(p->d_tasks);
+    if (p->d_blocked) (void)hipFree(p->d_blocked);
+    if (p->blocked) blocked_plan_destroy(p->blocked);
     delete p;
     return from_hip(e);
   }
@@ -188,7 +204,7 @@ pga_status pga_factor_plan_create(pga_factor_plan** out, int count, const pga_fa
 pga_status pga_factor_plan_execute(pga_factor_plan* p, void* stream) {
   PGA_CHECK_INIT();
   if (!p) return PGA_ERROR_ILLEGAL_ARGUMENT;
-  if (p->count == 0) return PGA_SUCCESS;
+  if (p->count == 0 && !p->blocked) return PGA_SUCCESS;
   PGA_HIP(run(*p, stream ? (hipStream_t)stream : runtime().stream));
   return PGA_SUCCESS;
 }
@@ -204,7 +220,7 @@ pga_status pga_factor_plan_capture_graph(pga_factor_plan* p, void* stream) {
     p->graph = nullptr;
   }
   PGA_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-  hipError_t e = p->count ? run(*p, s) : hipSuccess;
+  hipError_t e = (p->count || p->blocked) ? run(*p, s) : hipSuccess;
   hipError_t e2 = hipStreamEndCapture(s, &p->graph);
   if (e != hipSuccess) return from_hip(e);
   PGA_HIP(e2);
@@ -221,7 +237,7 @@ pga_status pga_factor_plan_launch_graph(pga_factor_plan* p, void* stream) {
 
 pga_status pga_factor_plan_info(const pga_factor_plan* p, int* count, int* grid) {
   if (!p) return PGA_ERROR_ILLEGAL_ARGUMENT;
-  if (count) *count = p->count;
+  if (count) *count = p->count + p->blocked_count;
   if (grid) *grid = p->grid;
   return PGA_SUCCESS;
 }
@@ -233,6 +249,8 @@ pga_status pga_factor_plan_destroy(pga_factor_plan* p) {
   if (p->graph) (void)hipGraphDestroy(p->graph);
   if (p->d_tasks) (void)hipFree(p->d_tasks);
   if (p->d_counter) (void)hipFree(p->d_counter);
+  if (p->d_blocked) (void)hipFree(p->d_blocked);
+  if (p->blocked) blocked_plan_destroy(p->blocked);
   delete p;
   return PGA_SUCCESS;
 }

@@ -1,0 +1,462 @@
+// blocked_batch.hip -- batched LU / Cholesky factorizations and solves of problems too
+// large for one workgroup.
+//
+// Every problem follows the blocking of the per-call driver it mirrors
+// (Rgetrf_opencl.cpp:93-146, Rpotrf_opencl.cpp:86-160) and all problems advance together,
+// one column block per step.  A step is at most five launches for the whole batch:
+//   1. the Cholesky diagonal-block SYRKs (one persistent GEMM launch over all their tiles),
+//   2. the panels: one workgroup per problem runs Rgetrf2 on its column block (rows
+//      interchanged within the block as each pivot is chosen) or Rpotrf2 on its diagonal
+//      block (wg_factor.hpp, the code of the one-workgroup task kernel),
+//   3. the LU row-block triangular solves (U12 := L11^-1 A12), a few columns per
+//      workgroup, each column first taking the block's row interchanges (Rlaswp),
+//   4. the trailing GEMMs of every problem (LU A22 -= A21 U12; Cholesky panel updates),
+//   5. the Cholesky panel triangular solves.
+// The columns left of each LU block take that block's interchanges at the end, in one
+// pass (nothing reads them in between, as in the reference).  The solves of all problems
+// follow, one workgroup per right-hand side.  Each element
+// receives the reference routines' operations in their order (the GEMM tiles are the
+// plain kernel's), so every factor, pivot, info code and solution equals the per-call
+// result bit for bit.  A failing Cholesky block sets info, which stops every later step
+// of that problem exactly where the reference stops.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "pga_internal.hpp"
+#include "wg_factor.hpp"
+
+namespace pga {
+namespace bb {
+
+constexpr int NT = wg::NT;
+constexpr int TRSM_COLS = 4;   // columns per workgroup of the row-block solves (63 x 4 pairs)
+constexpr int TRSM_ROWS = 64;  // rows per workgroup of the lower-Cholesky panel solves
+
+// A GEMM of one problem at one step: C = alpha op(A) op(B) + beta C in Rgemm's order.
+struct Gemm {
+  int m, n, k, lda, ldb, ldc;
+  int ta, tb, skip0, tri;
+  uint32_t alpha, beta;
+  const uint32_t* a;
+  const uint32_t* b;
+  uint32_t* c;
+  const int* stop;  // nonzero: the problem stopped (a Cholesky block failed)
+  int task;         // owning task (the plan points stop at its info when uploaded)
+};
+
+// A unit of panel / solve work: task, column block (j, jb), and a column (or row) range.
+struct Item {
+  int task, j, jb, off, cnt;
+};
+
+__device__ __forceinline__ bool is_lu(int kind) {
+  return kind == PGA_TASK_GETRF || kind == PGA_TASK_GESV || kind == PGA_TASK_GETRS;
+}
+
+// info := 0 for every task (the per-call drivers start that way; a solve reports 0), and
+// the persistent-launch counters of the whole plan := 0
+__global__ void k_bb_init(const FactorTask* __restrict__ tasks, int ntasks, int* __restrict__ counters,
+                          int ncounters) {
+  PGA_KERNEL_VGPR_FLOOR();
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < ntasks) *tasks[i].info = 0;
+  if (i < ncounters) counters[i] = 0;
+}
+
+// Panels: Rgetrf2 on columns j..j+jb-1 (rows j..n-1), or Rpotrf2 on the diagonal block.
+__global__ __launch_bounds__(NT, 2) void k_bb_panel(const FactorTask* __restrict__ tasks,
+                                                    const Item* __restrict__ items) {
+  PGA_KERNEL_VGPR_FLOOR();
+  __shared__ wg::Shared sh;
+  const Item it = items[blockIdx.x];
+  const FactorTask& t = tasks[it.task];
+  if (threadIdx.x == 0) sh.bcast = *t.info;
+  __syncthreads();
+  int info = sh.bcast;
+  __syncthreads();
+  if (is_lu(t.kind)) {
+    wg::lu_panel(t.n, t.a, t.lda, it.j, it.jb, t.ipiv, info, sh, true);
+    if (threadIdx.x == 0) *t.info = info;
+    return;
+  }
+  if (info) return;  // an earlier block failed: the reference has stopped
+  uint32_t* d = t.a + it.j + (size_t)it.j * t.lda;
+  const int f = t.uplo == 'U' ? wg::potrf2<true>(it.jb, d, t.lda, sh) : wg::potrf2<false>(it.jb, d, t.lda, sh);
+  if (f && threadIdx.x == 0) *t.info = it.j + f;
+}
+
+// Triangular solves on a block: LU U12 := L11^-1 U12 (Rtrsm LLNU, columns), upper
+// Cholesky U12 := U11^-T U12 (LUTN, columns), lower Cholesky L21 := L21 L11^-T (RLTN, rows).
+__global__ __launch_bounds__(NT) void k_bb_trsm(const FactorTask* __restrict__ tasks,
+                                                const Item* __restrict__ items) {
+  PGA_KERNEL_VGPR_FLOOR();
+  const Item it = items[blockIdx.x];
+  const FactorTask& t = tasks[it.task];
+  const int lda = t.lda, j = it.j, jb = it.jb;
+  uint32_t* d = t.a + j + (size_t)j * lda;
+  if (is_lu(t.kind)) {
+    // Rlaswp(k1 = j+1, k2 = j+jb) on these columns (Rgetrf_opencl.cpp:125-127)
+    if (threadIdx.x < it.cnt) {
+      uint32_t* c = t.a + (size_t)(j + jb + it.off + threadIdx.x) * lda;
+      for (int k = j; k < j + jb; ++k) {
+        const int p = t.ipiv[k] - 1;
+        if (p != k) {
+          const uint32_t x = c[k];
+          c[k] = c[p];
+          c[p] = x;
+        }
+      }
+    }
+    __syncthreads();
+    wg::trsm_llnu(jb, it.cnt, d, lda, d + (size_t)(jb + it.off) * lda, lda);
+    return;
+  }
+  if (*t.info) return;
+  if (t.uplo == 'U') wg::trsm_lutn(jb, it.cnt, d, lda, d + (size_t)(jb + it.off) * lda, lda);
+  else wg::trsm_rltn(it.cnt, jb, d, lda, d + jb + it.off, lda);
+}
+
+// The interchanges of every later block on LU columns [off, off + cnt), one lane per
+// column (Rgetrf_opencl.cpp:123 applies them block by block to the columns on the left;
+// item.j is the task's block size).
+__global__ __launch_bounds__(NT) void k_bb_laswp_left(const FactorTask* __restrict__ tasks,
+                                                      const Item* __restrict__ items) {
+  PGA_KERNEL_VGPR_FLOOR();
+  const Item it = items[blockIdx.x];
+  const FactorTask& t = tasks[it.task];
+  if (threadIdx.x >= it.cnt) return;
+  const int c = it.off + threadIdx.x, nb = it.j;
+  uint32_t* col = t.a + (size_t)c * t.lda;
+  for (int k = (c / nb + 1) * nb; k < t.n; ++k) {
+    const int p = t.ipiv[k] - 1;
+    if (p != k) {
+      const uint32_t x = col[k];
+      col[k] = col[p];
+      col[p] = x;
+    }
+  }
+}
+
+// Persistent GEMM over the tiles of many problems: T x T tiles (64: 4 x 4 elements per
+// lane; 32: 2 x 2; 16: 1 x 1), the size chosen per launch so that the tiles fill the
+// device (a step of a few problems has few large tiles; K is never split).
+template <int T, bool TA, bool TB, bool SKIP0>
+__device__ __forceinline__ void bb_tile(const Gemm& p, int i0, int j0, uint32_t* lds) {
+  gemm::gemm_tile_in<T, T, 16, T / 16, T / 16, TA, TB, SKIP0>(p.m, p.n, p.k, p.alpha, p.a, p.lda, p.b, p.ldb, p.beta,
+                                                              p.c, p.ldc, i0, j0, p.tri, lds);
+}
+
+template <int T>
+__global__ __launch_bounds__(NT) void k_bb_gemm(const Gemm* __restrict__ probs, const int* __restrict__ tile_start,
+                                                int nprob, int total, int* __restrict__ counter) {
+  PGA_KERNEL_VGPR_FLOOR();
+  __shared__ uint32_t lds[gemm::TileLds<T, T, 16>::words];
+  __shared__ int s_tile;
+  for (;;) {
+    if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1);
+    __syncthreads();
+    const int t = s_tile;
+    __syncthreads();
+    if (t >= total) return;  // every workgroup ends here once the tiles are taken
+    int lo = 0, hi = nprob - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (tile_start[mid] <= t) lo = mid;
+      else hi = mid - 1;
+    }
+    const Gemm& p = probs[lo];
+    if (p.stop && *p.stop) continue;  // the whole workgroup reads the same flag
+    const int tm = (p.m + T - 1) / T;
+    const int i0 = ((t - tile_start[lo]) % tm) * T, j0 = ((t - tile_start[lo]) / tm) * T;
+    if (p.tri == 1 && i0 + T - 1 < j0) continue;
+    if (p.tri == 2 && j0 + T - 1 < i0) continue;
+    if (!p.ta && !p.tb) bb_tile<T, false, false, false>(p, i0, j0, lds);
+    else if (p.ta && !p.tb) bb_tile<T, true, false, false>(p, i0, j0, lds);
+    else if (p.skip0) bb_tile<T, false, true, true>(p, i0, j0, lds);
+    else bb_tile<T, false, true, false>(p, i0, j0, lds);
+  }
+}
+
+// Solves, one workgroup per right-hand side, with the column staged in LDS:
+// Rgetrs 'N' (Rlaswp, L, U), Rpotrs upper (U^T, U) or lower (L, L^T).
+__global__ __launch_bounds__(NT) void k_bb_solve(const FactorTask* __restrict__ tasks,
+                                                 const Item* __restrict__ items) {
+  PGA_KERNEL_VGPR_FLOOR();
+  extern __shared__ uint32_t col[];
+  const Item it = items[blockIdx.x];
+  const FactorTask& t = tasks[it.task];
+  const bool factor = (t.kind == PGA_TASK_GESV || t.kind == PGA_TASK_POSV);
+  if (factor && *t.info) return;  // factor-and-solve tasks solve only after success
+  const int n = t.n;
+  uint32_t* b = t.b + (size_t)it.off * t.ldb;
+  for (int i = threadIdx.x; i < n; i += NT) col[i] = b[i];
+  __syncthreads();
+  if (is_lu(t.kind)) {
+    if (threadIdx.x == 0) {  // Rlaswp(1, n, ipiv, 1): the interchanges in order
+      for (int i = 0; i < n; ++i) {
+        const int p = t.ipiv[i] - 1;
+        if (p != i) {
+          const uint32_t x = col[i];
+          col[i] = col[p];
+          col[p] = x;
+        }
+      }
+    }
+    __syncthreads();
+    wg::trsm_llnu(n, 1, t.a, t.lda, col, n);
+    wg::trsm_lunn(n, 1, t.a, t.lda, col, n);
+  } else if (t.uplo == 'U') {
+    wg::trsm_lutn(n, 1, t.a, t.lda, col, n);
+    wg::trsm_lunn(n, 1, t.a, t.lda, col, n);
+  } else {
+    wg::trsm_llnn(n, 1, t.a, t.lda, col, n);
+    wg::trsm_lltn(n, 1, t.a, t.lda, col, n);
+  }
+  for (int i = threadIdx.x; i < n; i += NT) b[i] = col[i];
+}
+
+}  // namespace bb
+
+// ===================================================================================
+// Host side: the plan image (descriptors of every step) and the launch sequence
+// ===================================================================================
+struct BlockedPlan {
+  struct Step {
+    size_t syrk, syrk_tiles, panel, trsm_lu, gemm, gemm_tiles, trsm_ch;  // byte offsets in the image
+    int n_syrk = 0, syrk_total = 0, n_panel = 0, n_trsm_lu = 0, n_gemm = 0, gemm_total = 0, n_trsm_ch = 0;
+    int syrk_tile = 64, gemm_tile = 64;
+  };
+  std::vector<unsigned char> image;
+  std::vector<Step> steps;
+  size_t tasks = 0, solve = 0, counters = 0, spare_info = 0, swaps = 0;
+  int ntasks = 0, n_solve = 0, ncounters = 0, max_solve_n = 0, n_swaps = 0;
+  int gemm_grid[3] = {1, 1, 1};  // persistent workgroups for tiles of 64, 32, 16
+  unsigned char* dev = nullptr;  // device copy of the image
+};
+
+namespace {
+
+size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
+
+template <class T>
+size_t put(std::vector<unsigned char>& img, const std::vector<T>& v) {
+  const size_t off = align16(img.size());
+  img.resize(off + v.size() * sizeof(T));
+  if (!v.empty()) std::memcpy(img.data() + off, v.data(), v.size() * sizeof(T));
+  return off;
+}
+
+int gemm_tiles(int m, int n, int t) { return ((m + t - 1) / t) * ((n + t - 1) / t); }
+
+// largest-first order of a step's GEMMs, the tile size (the largest that still gives
+// every CU two tiles, as the per-call GEMM picks it) and the tile prefix sums
+size_t put_gemms(std::vector<unsigned char>& img, std::vector<bb::Gemm>& g, size_t& tiles_off, int& total, int& tile) {
+  std::stable_sort(g.begin(), g.end(), [](const bb::Gemm& x, const bb::Gemm& y) {
+    return (double)x.m * x.n * x.k > (double)y.m * y.n * y.k;
+  });
+  long t64 = 0, t32 = 0;
+  for (const bb::Gemm& x : g) {
+    t64 += gemm_tiles(x.m, x.n, 64);
+    t32 += gemm_tiles(x.m, x.n, 32);
+  }
+  const long want = 2L * runtime().cus;
+  tile = t64 >= want ? 64 : (t32 >= want ? 32 : 16);
+  std::vector<int> start(g.size());
+  total = 0;
+  for (size_t i = 0; i < g.size(); ++i) {
+    start[i] = total;
+    total += gemm_tiles(g[i].m, g[i].n, tile);
+  }
+  const size_t off = put(img, g);
+  tiles_off = put(img, start);
+  return off;
+}
+
+int grid_index(int tile) { return tile == 64 ? 0 : (tile == 32 ? 1 : 2); }
+
+template <class K>
+int occupancy(K kernel) {
+  int v = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, kernel, bb::NT, 0) != hipSuccess || v < 1) v = 1;
+  return v;
+}
+
+// one persistent GEMM launch with the step's tile size
+hipError_t launch_gemm(const BlockedPlan* p, int tile, int total, int nprob, const unsigned char* probs,
+                       const unsigned char* starts, int* counter, hipStream_t s) {
+  const unsigned grid = (unsigned)std::min(total, p->gemm_grid[grid_index(tile)]);
+  const bb::Gemm* g = reinterpret_cast<const bb::Gemm*>(probs);
+  const int* st = reinterpret_cast<const int*>(starts);
+  if (tile == 64) hipLaunchKernelGGL(bb::k_bb_gemm<64>, dim3(grid), dim3(bb::NT), 0, s, g, st, nprob, total, counter);
+  else if (tile == 32)
+    hipLaunchKernelGGL(bb::k_bb_gemm<32>, dim3(grid), dim3(bb::NT), 0, s, g, st, nprob, total, counter);
+  else hipLaunchKernelGGL(bb::k_bb_gemm<16>, dim3(grid), dim3(bb::NT), 0, s, g, st, nprob, total, counter);
+  return hipGetLastError();
+}
+
+bool factor_kind(int k) {
+  return k == PGA_TASK_GETRF || k == PGA_TASK_POTRF || k == PGA_TASK_GESV || k == PGA_TASK_POSV;
+}
+bool lu_kind(int k) { return k == PGA_TASK_GETRF || k == PGA_TASK_GESV || k == PGA_TASK_GETRS; }
+
+}  // namespace
+
+BlockedPlan* blocked_plan_build(const FactorTask* tasks, int count) {
+  auto* p = new BlockedPlan();
+  p->ntasks = count;
+  std::vector<FactorTask> tv(tasks, tasks + count);
+  p->tasks = put(p->image, tv);
+  // column blocks of every task: unblocked (one block of n columns) as Rgetrf_opencl /
+  // Rpotrf_opencl do when nb <= 1 or nb >= n
+  int nsteps = 0;
+  std::vector<int> nbt(count), steps_of(count);
+  for (int i = 0; i < count; ++i) {
+    const FactorTask& t = tasks[i];
+    const int nb = t.nb > 0 ? t.nb : 64;
+    nbt[i] = (nb <= 1 || nb >= t.n) ? t.n : nb;
+    steps_of[i] = factor_kind(t.kind) && t.n > 0 ? (t.n + nbt[i] - 1) / nbt[i] : 0;
+    nsteps = std::max(nsteps, steps_of[i]);
+  }
+  const uint32_t ONE = 0x40000000u, MONE = 0xC0000000u;
+  for (int s = 0; s < nsteps; ++s) {
+    std::vector<bb::Gemm> syrk, gemm;
+    std::vector<bb::Item> panel, trsm_lu, trsm_ch;
+    for (int i = 0; i < count; ++i) {
+      if (s >= steps_of[i]) continue;
+      const FactorTask& t = tasks[i];
+      const int n = t.n, lda = t.lda, j = s * nbt[i], jb = std::min(nbt[i], n - j);
+      auto A = [&](int r, int c) { return t.a + r + (size_t)c * lda; };
+      const int rest = n - j - jb;
+      if (lu_kind(t.kind)) {
+        panel.push_back({i, j, jb, 0, 0});
+        if (rest > 0) {
+          for (int c = 0; c < rest; c += bb::TRSM_COLS) trsm_lu.push_back({i, j, jb, c, std::min(bb::TRSM_COLS, rest - c)});
+          gemm.push_back({rest, rest, jb, lda, lda, lda, 0, 0, 0, 0, MONE, ONE, A(j + jb, j), A(j, j + jb),
+                          A(j + jb, j + jb), nullptr, -1});
+        }
+        continue;
+      }
+      const bool upper = (t.uplo == 'U');
+      if (j > 0) {  // Rpotrf_opencl.cpp:103-106 / 133-136 (Rsyrk, K = j)
+        if (upper)
+          syrk.push_back({jb, jb, j, lda, lda, lda, 1, 0, 0, 2, MONE, ONE, A(0, j), A(0, j), A(j, j), nullptr, i});
+        else
+          syrk.push_back({jb, jb, j, lda, lda, lda, 0, 1, 1, 1, MONE, ONE, A(j, 0), A(j, 0), A(j, j), nullptr, i});
+      }
+      panel.push_back({i, j, jb, 0, 0});
+      if (rest > 0) {
+        if (j > 0) {  // Rgemm with K = j (Rgemm quick-returns on K = 0)
+          if (upper)
+            gemm.push_back({jb, rest, j, lda, lda, lda, 1, 0, 0, 0, MONE, ONE, A(0, j), A(0, j + jb), A(j, j + jb),
+                            nullptr, i});
+          else
+            gemm.push_back({rest, jb, j, lda, lda, lda, 0, 1, 0, 0, MONE, ONE, A(j + jb, 0), A(j, 0), A(j + jb, j),
+                            nullptr, i});
+        }
+        if (upper) {
+          for (int c = 0; c < rest; c += bb::TRSM_COLS) trsm_ch.push_back({i, j, jb, c, std::min(bb::TRSM_COLS, rest - c)});
+        } else {
+          for (int r = 0; r < rest; r += bb::TRSM_ROWS) trsm_ch.push_back({i, j, jb, r, std::min(bb::TRSM_ROWS, rest - r)});
+        }
+      }
+    }
+    BlockedPlan::Step st;
+    st.syrk = put_gemms(p->image, syrk, st.syrk_tiles, st.syrk_total, st.syrk_tile);
+    st.n_syrk = (int)syrk.size();
+    st.panel = put(p->image, panel);
+    st.n_panel = (int)panel.size();
+    st.trsm_lu = put(p->image, trsm_lu);
+    st.n_trsm_lu = (int)trsm_lu.size();
+    st.gemm = put_gemms(p->image, gemm, st.gemm_tiles, st.gemm_total, st.gemm_tile);
+    st.n_gemm = (int)gemm.size();
+    st.trsm_ch = put(p->image, trsm_ch);
+    st.n_trsm_ch = (int)trsm_ch.size();
+    p->steps.push_back(st);
+  }
+  // the left columns' interchanges of the blocked LU tasks
+  std::vector<bb::Item> swaps;
+  for (int i = 0; i < count; ++i) {
+    const FactorTask& t = tasks[i];
+    if (!lu_kind(t.kind) || steps_of[i] < 2) continue;
+    const int last = (steps_of[i] - 1) * nbt[i];  // columns of the last block take nothing
+    for (int c = 0; c < last; c += bb::NT) swaps.push_back({i, nbt[i], 0, c, std::min(bb::NT, last - c)});
+  }
+  p->swaps = put(p->image, swaps);
+  p->n_swaps = (int)swaps.size();
+  std::vector<bb::Item> solve;
+  for (int i = 0; i < count; ++i) {
+    const FactorTask& t = tasks[i];
+    const bool solves = !(t.kind == PGA_TASK_GETRF || t.kind == PGA_TASK_POTRF);
+    if (!solves || t.n == 0) continue;
+    for (int c = 0; c < t.nrhs; ++c) solve.push_back({i, 0, 0, c, 1});
+    p->max_solve_n = std::max(p->max_solve_n, t.n);
+  }
+  p->solve = put(p->image, solve);
+  p->n_solve = (int)solve.size();
+  p->ncounters = 2 * (int)p->steps.size();
+  p->counters = put(p->image, std::vector<int>(std::max(1, p->ncounters), 0));
+  // info of tasks given none: the steps read it as their stop flag
+  p->spare_info = put(p->image, std::vector<int>(std::max(1, count), 0));
+  static const int per_cu[3] = {occupancy(bb::k_bb_gemm<64>), occupancy(bb::k_bb_gemm<32>), occupancy(bb::k_bb_gemm<16>)};
+  for (int i = 0; i < 3; ++i) p->gemm_grid[i] = per_cu[i] * runtime().cus;
+  return p;
+}
+
+size_t blocked_plan_bytes(const BlockedPlan* p) { return align16(p->image.size()); }
+
+hipError_t blocked_plan_upload(BlockedPlan* p, void* dev, hipStream_t s) {
+  p->dev = static_cast<unsigned char*>(dev);
+  FactorTask* t = reinterpret_cast<FactorTask*>(p->image.data() + p->tasks);
+  for (int i = 0; i < p->ntasks; ++i)
+    if (!t[i].info) t[i].info = reinterpret_cast<int*>(p->dev + p->spare_info) + i;
+  // the Cholesky GEMMs stop with their task
+  for (const BlockedPlan::Step& st : p->steps) {
+    for (auto [off, cnt] : {std::pair<size_t, int>(st.syrk, st.n_syrk), std::pair<size_t, int>(st.gemm, st.n_gemm)}) {
+      bb::Gemm* g = reinterpret_cast<bb::Gemm*>(p->image.data() + off);
+      for (int k = 0; k < cnt; ++k) g[k].stop = g[k].task >= 0 ? t[g[k].task].info : nullptr;
+    }
+  }
+  return hipMemcpyAsync(dev, p->image.data(), p->image.size(), hipMemcpyHostToDevice, s);
+}
+
+hipError_t blocked_plan_launch(const BlockedPlan* p, hipStream_t s) {
+  using namespace bb;
+  unsigned char* d = p->dev;
+  const FactorTask* tasks = reinterpret_cast<const FactorTask*>(d + p->tasks);
+  int* counters = reinterpret_cast<int*>(d + p->counters);
+  const int init_n = std::max(p->ntasks, p->ncounters);
+  if (init_n > 0)
+    hipLaunchKernelGGL(k_bb_init, dim3((init_n + 255) / 256), dim3(256), 0, s, tasks, p->ntasks, counters,
+                       p->ncounters);
+  for (size_t i = 0; i < p->steps.size(); ++i) {
+    const BlockedPlan::Step& st = p->steps[i];
+    if (st.syrk_total > 0)
+      (void)launch_gemm(p, st.syrk_tile, st.syrk_total, st.n_syrk, d + st.syrk, d + st.syrk_tiles, counters + 2 * i, s);
+    if (st.n_panel > 0)
+      hipLaunchKernelGGL(k_bb_panel, dim3(st.n_panel), dim3(NT), 0, s, tasks,
+                         reinterpret_cast<const Item*>(d + st.panel));
+    if (st.n_trsm_lu > 0)
+      hipLaunchKernelGGL(k_bb_trsm, dim3(st.n_trsm_lu), dim3(NT), 0, s, tasks,
+                         reinterpret_cast<const Item*>(d + st.trsm_lu));
+    if (st.gemm_total > 0)
+      (void)launch_gemm(p, st.gemm_tile, st.gemm_total, st.n_gemm, d + st.gemm, d + st.gemm_tiles, counters + 2 * i + 1,
+                        s);
+    if (st.n_trsm_ch > 0)
+      hipLaunchKernelGGL(k_bb_trsm, dim3(st.n_trsm_ch), dim3(NT), 0, s, tasks,
+                         reinterpret_cast<const Item*>(d + st.trsm_ch));
+  }
+  if (p->n_swaps > 0)
+    hipLaunchKernelGGL(k_bb_laswp_left, dim3(p->n_swaps), dim3(NT), 0, s, tasks,
+                       reinterpret_cast<const Item*>(d + p->swaps));
+  if (p->n_solve > 0)
+    hipLaunchKernelGGL(k_bb_solve, dim3(p->n_solve), dim3(NT), (size_t)p->max_solve_n * 4, s, tasks,
+                       reinterpret_cast<const Item*>(d + p->solve));
+  return hipGetLastError();
+}
+
+void blocked_plan_destroy(BlockedPlan* p) { delete p; }
+
+}  // namespace pga

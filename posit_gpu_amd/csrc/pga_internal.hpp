@@ -45,6 +45,16 @@ struct FactorTask {
 int factor_tasks_grid(int count);
 hipError_t factor_tasks_launch(const FactorTask* d_tasks, int count, int* d_counter, int grid, hipStream_t s);
 
+// Blocked lockstep batches of tasks larger than one workgroup handles (blocked_batch.hip).
+// build: host descriptors (device pointers); upload: the plan image into a device buffer
+// of blocked_plan_bytes(); launch: the whole sequence on one stream (graph-capturable).
+struct BlockedPlan;
+BlockedPlan* blocked_plan_build(const FactorTask* tasks, int count);
+size_t blocked_plan_bytes(const BlockedPlan* p);
+hipError_t blocked_plan_upload(BlockedPlan* p, void* dev, hipStream_t s);
+hipError_t blocked_plan_launch(const BlockedPlan* p, hipStream_t s);
+void blocked_plan_destroy(BlockedPlan* p);
+
 // Per-call device drivers of lapack.hip (stream-ordered; arguments already validated)
 hipError_t la_getrf(hipStream_t s, int m, int n, uint32_t* a, int lda, int* ipiv, int* info, int nb);
 hipError_t la_getrs(hipStream_t s, bool trans, int n, int nrhs, const uint32_t* a, int lda, const int* ipiv,

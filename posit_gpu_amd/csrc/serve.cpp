@@ -7,16 +7,18 @@
 // slots is free, every admitted small request is launched together as one persistent
 // task launch (batched_factor.hip) on that slot's stream, so requests that arrive while
 // others run join the next launch and the launches overlap on the device.  Larger
-// requests run the per-call device drivers (lapack.hip) on worker threads, one stream
-// each, overlapping with everything else.  Both paths execute the reference routines'
-// operations in the reference order, so every result equals the per-request plain
-// result bit for bit.  In plain mode one worker runs every request alone through the
-// per-call drivers, one after the other: the baseline the batched form is measured
-// against.
+// requests are taken by worker threads (a stream each): a worker runs every large
+// request ready at that moment as one blocked batch (blocked_batch.hip: all of them
+// advance one column block per step, a few launches per step for the whole batch), and
+// requests that arrive meanwhile go to the next free worker.  All paths execute the
+// reference routines' operations in the reference order, so every result equals the
+// per-request plain result bit for bit.  In plain mode one worker runs every request
+// alone through the per-call drivers, one after the other: the baseline the batched
+// form is measured against.
 //
 // All device memory of requests and factors comes from one arena allocated at creation
-// (its size is the budget), so admission is exact and no allocation or free runs while
-// kernels are in flight.
+// (its size is the budget), so admission is exact.  Only the workers' small plan
+// descriptor buffers are allocated later (grown when a larger batch needs more).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -342,25 +344,121 @@ pga_status run_alone(pga_server* S, Request* q, hipStream_t s, int* info_out) {
   return PGA_SUCCESS;
 }
 
+// Device buffer of a worker's blocked-batch plans (grown, never shrunk while running).
+struct PlanBuf {
+  void* dev = nullptr;
+  size_t bytes = 0;
+};
+
+// Run several large requests together as one blocked batch (blocked_batch.hip) on
+// stream s: inputs in, all factorizations advancing one column block at a time, the
+// solves, solutions and infos out.  infos[i] receives request i's info.
+pga_status run_blocked(pga_server* S, const std::vector<Request*>& batch, hipStream_t s, PlanBuf& buf,
+                       std::vector<int>& infos) {
+  const int cnt = (int)batch.size();
+  std::vector<FactorTask> tasks(cnt);
+  for (int i = 0; i < cnt; ++i) {
+    Request* q = batch[i];
+    const pga_request& r = q->r;
+    Factor& f = *q->f;
+    const bool fresh = is_new(r);
+    if (fresh) PGA_HIP(hipMemcpyAsync(factor_a(S, f), r.a, (size_t)r.n * r.n * 4, hipMemcpyHostToDevice, s));
+    if (r.nrhs > 0)
+      PGA_HIP(hipMemcpyAsync(S->dev(q->sol_off), r.b, (size_t)r.n * r.nrhs * 4, hipMemcpyHostToDevice, s));
+    FactorTask& t = tasks[i];
+    t.kind = f.lu ? (fresh ? PGA_TASK_GESV : PGA_TASK_GETRS) : (fresh ? PGA_TASK_POSV : PGA_TASK_POTRS);
+    t.uplo = f.uplo;
+    t.n = r.n;
+    t.nrhs = r.nrhs;
+    t.nb = 64;
+    t.lda = t.ldb = std::max(1, r.n);
+    t.a = factor_a(S, f);
+    t.ipiv = f.lu ? factor_ipiv(S, f) : nullptr;
+    t.b = S->dev(q->sol_off);
+    t.info = fresh ? factor_info(S, f) : nullptr;  // a re-solve must not touch its factor's info
+  }
+  BlockedPlan* p = blocked_plan_build(tasks.data(), cnt);
+  const size_t need = blocked_plan_bytes(p);
+  hipError_t e = hipSuccess;
+  if (need > buf.bytes) {
+    if (buf.dev) {
+      (void)hipStreamSynchronize(s);
+      (void)hipFree(buf.dev);
+      buf.dev = nullptr;
+      buf.bytes = 0;
+    }
+    const size_t want = std::max(2 * need, (size_t)1 << 20);
+    e = hipMalloc(&buf.dev, want);
+    if (e == hipSuccess) buf.bytes = want;
+  }
+  if (e == hipSuccess) e = blocked_plan_upload(p, buf.dev, s);
+  if (e == hipSuccess) e = blocked_plan_launch(p, s);
+  for (int i = 0; i < cnt && e == hipSuccess; ++i) {
+    Request* q = batch[i];
+    if (is_new(q->r))
+      e = hipMemcpyAsync(S->h_info + (q->id % S->info_slots), factor_info(S, *q->f), sizeof(int),
+                         hipMemcpyDeviceToHost, s);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  // solutions of the requests whose solve ran (factor-and-solve requests solve only
+  // when their factorization succeeded)
+  infos.assign(cnt, 0);
+  for (int i = 0; i < cnt && e == hipSuccess; ++i) {
+    Request* q = batch[i];
+    const pga_request& r = q->r;
+    infos[i] = is_new(r) ? S->h_info[q->id % S->info_slots] : 0;
+    if (infos[i] == 0 && r.nrhs > 0)
+      e = hipMemcpy(r.x, S->dev(q->sol_off), (size_t)r.n * r.nrhs * 4, hipMemcpyDeviceToHost);
+  }
+  blocked_plan_destroy(p);
+  return e == hipSuccess ? PGA_SUCCESS : from_hip(e);
+}
+
+// Larger requests.  Plain mode: one request at a time through the per-call drivers.
+// Batched mode: every ready large request (up to max_batch) runs as one blocked batch;
+// requests beyond the blocked path's largest order run alone.
 void worker_main(pga_server* S) {
   hipStream_t s = nullptr;
   (void)hipSetDevice(runtime().device);
   (void)hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+  PlanBuf buf;
+  std::vector<int> infos;
   std::unique_lock<std::mutex> lk(S->mu);
   for (;;) {
     S->cv_work.wait(lk, [&] { return S->stop || !S->large_ready.empty(); });
     if (S->stop) break;
-    Request* q = S->large_ready.front();
-    S->large_ready.pop_front();
-    q->state = RUNNING;
+    Request* first = S->large_ready.front();
+    if (S->cfg.mode == PGA_SERVER_PLAIN || first->r.n > PGA_BLOCKED_TASK_MAX_N) {
+      S->large_ready.pop_front();
+      first->state = RUNNING;
+      lk.unlock();
+      int info = 0;
+      pga_status st = run_alone(S, first, s, &info);
+      lk.lock();
+      finish(S, first, st == PGA_SUCCESS ? 0 : PGA_REQ_FAILED, info);
+      continue;
+    }
+    std::vector<Request*> batch;
+    for (auto it = S->large_ready.begin(); it != S->large_ready.end() && (int)batch.size() < S->cfg.max_batch;) {
+      if ((*it)->r.n <= PGA_BLOCKED_TASK_MAX_N) {
+        (*it)->state = RUNNING;
+        batch.push_back(*it);
+        it = S->large_ready.erase(it);
+      } else {
+        ++it;
+      }
+    }
+    S->stats.batches++;
+    S->stats.batched_requests += (int64_t)batch.size();
     lk.unlock();
-    int info = 0;
-    pga_status st = run_alone(S, q, s, &info);
+    pga_status st = run_blocked(S, batch, s, buf, infos);
     lk.lock();
-    finish(S, q, st == PGA_SUCCESS ? 0 : PGA_REQ_FAILED, info);
+    for (size_t i = 0; i < batch.size(); ++i)
+      finish(S, batch[i], st == PGA_SUCCESS ? 0 : PGA_REQ_FAILED, st == PGA_SUCCESS ? infos[i] : 0);
   }
   lk.unlock();
   (void)hipStreamSynchronize(s);
+  if (buf.dev) (void)hipFree(buf.dev);
   (void)hipStreamDestroy(s);
 }
 

@@ -259,7 +259,11 @@ static __device__ int iamax(int n, const uint32_t* x, Shared& sh) {
 }
 
 // Leaf of Rgetrf2 for column j, rows j..n-1 (Rgetrf2.cpp:98-132)
-static __device__ void lu_leaf(int n, uint32_t* a, int lda, int j, int* ipiv, int& info, Shared& sh) {
+// Rows are interchanged in columns [c_lo, c_hi) (default: all n); the caller applies
+// the interchanges to the other columns later, in the same order.
+static __device__ void lu_leaf(int n, uint32_t* a, int lda, int j, int* ipiv, int& info, Shared& sh, int c_lo = 0,
+                               int c_hi = -1) {
+  if (c_hi < 0) c_hi = n;
   uint32_t* col = at(a, lda, 0, j);
   const int p = j + iamax(n - j, col + j, sh);
   // not col[p]: the lane that swaps the pivot column may overwrite it before a later
@@ -269,7 +273,7 @@ static __device__ void lu_leaf(int n, uint32_t* a, int lda, int j, int* ipiv, in
   if (p != j) {
     // the pivot column itself is swapped only for a nonzero pivot (Rgetrf2.cpp:109-117);
     // every other column follows ipiv (the Rlaswp calls of Rgetrf2 / Rgetrf_opencl)
-    for (int c = threadIdx.x; c < n; c += NT) {
+    for (int c = c_lo + threadIdx.x; c < c_hi; c += NT) {
       if (c == j && ap == 0) continue;
       uint32_t* x = at(a, lda, 0, c);
       const uint32_t t = x[j];
@@ -307,10 +311,13 @@ static __device__ __forceinline__ bool node_after(int jr, int w, int& s, int& mi
 }
 
 // Rgetrf2 on columns p0..p0+w-1, rows p0..n-1 of the n x n matrix (rows >= columns at
-// every node, so n1 = columns / 2)
-static __device__ void lu_panel(int n, uint32_t* a, int lda, int p0, int w, int* ipiv, int& info, Shared& sh) {
+// every node, so n1 = columns / 2); with panel_swaps_only the row interchanges reach only
+// the panel's columns
+static __device__ void lu_panel(int n, uint32_t* a, int lda, int p0, int w, int* ipiv, int& info, Shared& sh,
+                                bool panel_swaps_only = false) {
   for (int jr = 0; jr < w; ++jr) {
-    lu_leaf(n, a, lda, p0 + jr, ipiv, info, sh);
+    if (panel_swaps_only) lu_leaf(n, a, lda, p0 + jr, ipiv, info, sh, p0, p0 + w);
+    else lu_leaf(n, a, lda, p0 + jr, ipiv, info, sh);
     int s, mid, e;
     if (node_after(jr, w, s, mid, e)) {
       // Rgetrf2.cpp:151-159: A12 := L11^-1 A12; A22 -= A21 A12

@@ -219,9 +219,73 @@ def test_largest_order(pga, oracle):
     _check_plan(pga, cases)
 
 
+def test_blocked_tasks_equal_per_call(pga, oracle):
+    """Tasks larger than one workgroup handles advance together one column block at a
+    time; every kind, both triangles, several block sizes (1 and >= n: unblocked), mixed
+    with one-workgroup tasks in the same plan."""
+    rng = np.random.default_rng(21)
+    cases = []
+    specs = [("gesv", 700, "L", 0), ("posv", 600, "U", 0), ("posv", 530, "L", 64), ("getrf", 513, "L", 100),
+             ("potrf", 777, "U", 48), ("potrf", 520, "L", 0), ("gesv", 530, "L", 1), ("posv", 515, "U", 515),
+             ("gesv", 90, "L", 16), ("posv", 200, "U", 0)]
+    for i, (kind, n, uplo, nb) in enumerate(specs):
+        A = posit_matrix(oracle, 1000 + i, n, n) if kind in ("getrf", "gesv") else _spd(oracle, 1000 + i, n)
+        B = posit_matrix(oracle, 1100 + i, n, int(rng.integers(1, 4))) if kind in ("gesv", "posv") else None
+        cases.append(Case(pga, kind, A, uplo, nb, B))
+    _check_plan(pga, cases)
+
+
+def test_blocked_solve_only_and_failures(pga, oracle):
+    """Solve-only tasks on given factors; a zero column (LU info, factorization goes on)
+    and a failing Cholesky block (info, every later step skipped) at sizes of the
+    blocked path."""
+    n = 600
+    cases = []
+    A = posit_matrix(oracle, 1200, n, n)
+    F = pga.DeviceArray.from_host(A.copy(order="F"))
+    ip = pga.DeviceArray(n, np.int32, "C")
+    inf = pga.DeviceArray(1, np.int32, "C")
+    pga.getrf_device(n, n, F, n, ip, inf, 64)
+    pga.synchronize()
+    cases.append(Case(pga, "getrs", A, B=posit_matrix(oracle, 1201, n, 2), factored=(F.to_host(), ip.to_host())))
+    for uplo in "LU":
+        S = _spd(oracle, 1202, n)
+        G = pga.DeviceArray.from_host(S.copy(order="F"))
+        pga.potrf_device(uplo, n, G, n, inf, 64)
+        pga.synchronize()
+        cases.append(Case(pga, "potrs", S, uplo, B=posit_matrix(oracle, 1203, n, 2), factored=(G.to_host(), None)))
+    Z = posit_matrix(oracle, 1204, n, n)
+    Z[:, 300] = 0
+    Z[17, 5] = NAR
+    cases.append(Case(pga, "gesv", Z, nb=64, B=posit_matrix(oracle, 1205, n, 1)))
+    for uplo, bad in (("U", 200), ("L", 450), ("U", 64)):
+        P = _spd(oracle, 1206, n)
+        P[bad, bad] = oracle.from_double(-2.0)
+        cases.append(Case(pga, "posv", P, uplo, 64, B=posit_matrix(oracle, 1207, n, 2)))
+    _check_plan(pga, cases)
+
+
+def test_blocked_graph_replays(pga, oracle):
+    cases = [Case(pga, "gesv", posit_matrix(oracle, 1300, 640, 640), nb=0, B=posit_matrix(oracle, 1301, 640, 2)),
+             Case(pga, "posv", _spd(oracle, 1302, 560), "U", 0, B=posit_matrix(oracle, 1303, 560, 1))]
+    plan, bufs = _check_plan(pga, cases, graph=True)
+    for rep in range(2):
+        for c, d in zip(cases, bufs):
+            d["a"].copy_from_host(c.A.copy(order="F"))
+            d["b"].copy_from_host(c.B.copy(order="F"))
+        pga.synchronize()
+        plan.launch_graph()
+        pga.synchronize()
+        for i, (c, d) in enumerate(zip(cases, bufs)):
+            got, want = c.read(d), c.per_call()
+            assert got["info"] == want["info"]
+            for key in ("a", "b"):
+                _same(got[key], want[key], f"replay {rep} task {i} {key}")
+
+
 def test_rejects_bad_tasks(pga):
     a = pga.DeviceArray((4, 4))
     with pytest.raises(pga.PositError):
         pga.FactorPlan([{"kind": "getrf", "n": 4, "a": a, "lda": 4}])  # no ipiv
     with pytest.raises(pga.PositError):
-        pga.FactorPlan([{"kind": "potrf", "n": 2000, "a": a, "lda": 2000, "uplo": "L"}])  # too large
+        pga.FactorPlan([{"kind": "potrf", "n": 20000, "a": a, "lda": 20000, "uplo": "L"}])  # too large
