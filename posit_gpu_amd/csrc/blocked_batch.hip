@@ -25,6 +25,7 @@
 #include <cstring>
 #include <vector>
 
+#include "lu_leaf.hpp"
 #include "pga_internal.hpp"
 #include "wg_factor.hpp"
 
@@ -32,6 +33,7 @@ namespace pga {
 namespace bb {
 
 constexpr int NT = wg::NT;
+constexpr int RECUR_W = 64;    // LU panels of this width are walked leaf by leaf for all problems
 constexpr int TRSM_COLS = 4;   // columns per workgroup of the row-block solves (63 x 4 pairs)
 constexpr int TRSM_ROWS = 64;  // rows per workgroup of the lower-Cholesky panel solves
 
@@ -117,6 +119,55 @@ __global__ __launch_bounds__(NT) void k_bb_trsm(const FactorTask* __restrict__ t
   if (*t.info) return;
   if (t.uplo == 'U') wg::trsm_lutn(jb, it.cnt, d, lda, d + (size_t)(jb + it.off) * lda, lda);
   else wg::trsm_rltn(it.cnt, jb, d, lda, d + jb + it.off, lda);
+}
+
+// LU panels of RECUR_W columns, all problems at once: leaf jr of every problem's panel
+// (iamax, interchange within the panel, scaling), then, when jr ends the left half of a
+// recursion node [ts, te) split at tm, that node's Rtrsm in LDS (lu_leaf.hpp; the
+// per-call driver runs the same leaf one problem at a time).
+__global__ __launch_bounds__(NT) void k_bb_leaf(const FactorTask* __restrict__ tasks, const Item* __restrict__ items,
+                                                int jr, int ts, int tm, int te) {
+  PGA_KERNEL_VGPR_FLOOR();
+  const Item it = items[blockIdx.x];
+  const FactorTask& t = tasks[it.task];
+  uint32_t* root = t.a + it.j + (size_t)it.j * t.lda;
+  leaf::getrf2_leaf<RECUR_W / 2, RECUR_W / 2>(t.n - it.j, it.jb, root, t.lda, jr, t.ipiv + it.j, t.info, it.j, it.j,
+                                              ts, tm, te);
+}
+
+// The node's Rgemm (A22 -= A21 A12, Rgetrf2.cpp:155-159) of every problem's panel in one
+// persistent launch of T x T tiles.
+template <int T>
+__global__ __launch_bounds__(NT) void k_bb_node_gemm(const FactorTask* __restrict__ tasks,
+                                                     const Item* __restrict__ items,
+                                                     const int* __restrict__ tile_start, int nitems, int total,
+                                                     int* __restrict__ counter, int s, int mid, int e) {
+  PGA_KERNEL_VGPR_FLOOR();
+  __shared__ uint32_t lds[gemm::TileLds<T, T, 16>::words];
+  __shared__ int s_tile;
+  for (;;) {
+    if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1);
+    __syncthreads();
+    const int t = s_tile;
+    __syncthreads();
+    if (t >= total) return;
+    int lo = 0, hi = nitems - 1;
+    while (lo < hi) {
+      const int md = (lo + hi + 1) >> 1;
+      if (tile_start[md] <= t) lo = md;
+      else hi = md - 1;
+    }
+    const Item it = items[lo];
+    const FactorTask& tk = tasks[it.task];
+    const int lda = tk.lda, j = it.j;
+    const int m = tk.n - j - mid, n2 = e - mid, k = mid - s;
+    const uint32_t* A = tk.a + (j + mid) + (size_t)(j + s) * lda;
+    const uint32_t* B = tk.a + (j + s) + (size_t)(j + mid) * lda;
+    uint32_t* C = tk.a + (j + mid) + (size_t)(j + mid) * lda;
+    const int tmr = (m + T - 1) / T, tl = t - tile_start[lo];
+    gemm::gemm_tile_in<T, T, 16, T / 16, T / 16, false, false, false>(m, n2, k, 0xC0000000u, A, lda, B, lda, 0x40000000u,
+                                                                    C, lda, (tl % tmr) * T, (tl / tmr) * T, 0, lds);
+  }
 }
 
 // The interchanges of every later block on LU columns [off, off + cnt), one lane per
@@ -224,15 +275,20 @@ __global__ __launch_bounds__(NT) void k_bb_solve(const FactorTask* __restrict__ 
 // Host side: the plan image (descriptors of every step) and the launch sequence
 // ===================================================================================
 struct BlockedPlan {
+  struct Node {  // a recursion node of the RECUR_W-wide LU panels: its Rgemm launch
+    int jr, s, mid, e, tile, total, counter;
+    size_t tiles;
+  };
   struct Step {
-    size_t syrk, syrk_tiles, panel, trsm_lu, gemm, gemm_tiles, trsm_ch;  // byte offsets in the image
+    size_t syrk, syrk_tiles, panel, trsm_lu, gemm, gemm_tiles, trsm_ch, recur;  // byte offsets in the image
     int n_syrk = 0, syrk_total = 0, n_panel = 0, n_trsm_lu = 0, n_gemm = 0, gemm_total = 0, n_trsm_ch = 0;
-    int syrk_tile = 64, gemm_tile = 64;
+    int syrk_tile = 64, gemm_tile = 64, n_recur = 0;
+    std::vector<Node> nodes;
   };
   std::vector<unsigned char> image;
   std::vector<Step> steps;
   size_t tasks = 0, solve = 0, counters = 0, spare_info = 0, swaps = 0;
-  int ntasks = 0, n_solve = 0, ncounters = 0, max_solve_n = 0, n_swaps = 0;
+  int ntasks = 0, n_solve = 0, ncounters = 0, max_solve_n = 0, n_swaps = 0, node_counter_base = 0;
   int gemm_grid[3] = {1, 1, 1};  // persistent workgroups for tiles of 64, 32, 16
   unsigned char* dev = nullptr;  // device copy of the image
 };
@@ -297,6 +353,33 @@ hipError_t launch_gemm(const BlockedPlan* p, int tile, int total, int nprob, con
   return hipGetLastError();
 }
 
+// the node of a recursion on w columns whose left half ends at column jr (the split of
+// Rgetrf2.cpp:137 with rows >= columns: n1 = columns / 2)
+bool node_after_host(int jr, int w, int& s, int& mid, int& e) {
+  s = 0;
+  e = w;
+  while (e - s > 1) {
+    mid = s + (e - s) / 2;
+    if (jr + 1 == mid) return true;
+    if (jr < mid) e = mid;
+    else s = mid;
+  }
+  return false;
+}
+
+hipError_t launch_node_gemm(const BlockedPlan::Node& nd, const FactorTask* tasks, const bb::Item* items, int nitems,
+                            const int* starts, int* counter, unsigned grid, hipStream_t s) {
+  const unsigned g = std::min<unsigned>(grid, (unsigned)nd.total);
+#define PGA_NODE(T)                                                                                          \
+  hipLaunchKernelGGL(bb::k_bb_node_gemm<T>, dim3(g), dim3(bb::NT), 0, s, tasks, items, starts, nitems, nd.total, \
+                     counter, nd.s, nd.mid, nd.e)
+  if (nd.tile == 64) PGA_NODE(64);
+  else if (nd.tile == 32) PGA_NODE(32);
+  else PGA_NODE(16);
+#undef PGA_NODE
+  return hipGetLastError();
+}
+
 bool factor_kind(int k) {
   return k == PGA_TASK_GETRF || k == PGA_TASK_POTRF || k == PGA_TASK_GESV || k == PGA_TASK_POSV;
 }
@@ -311,7 +394,7 @@ BlockedPlan* blocked_plan_build(const FactorTask* tasks, int count) {
   p->tasks = put(p->image, tv);
   // column blocks of every task: unblocked (one block of n columns) as Rgetrf_opencl /
   // Rpotrf_opencl do when nb <= 1 or nb >= n
-  int nsteps = 0;
+  int nsteps = 0, node_counters = 0;
   std::vector<int> nbt(count), steps_of(count);
   for (int i = 0; i < count; ++i) {
     const FactorTask& t = tasks[i];
@@ -323,7 +406,7 @@ BlockedPlan* blocked_plan_build(const FactorTask* tasks, int count) {
   const uint32_t ONE = 0x40000000u, MONE = 0xC0000000u;
   for (int s = 0; s < nsteps; ++s) {
     std::vector<bb::Gemm> syrk, gemm;
-    std::vector<bb::Item> panel, trsm_lu, trsm_ch;
+    std::vector<bb::Item> panel, trsm_lu, trsm_ch, recur;
     for (int i = 0; i < count; ++i) {
       if (s >= steps_of[i]) continue;
       const FactorTask& t = tasks[i];
@@ -331,7 +414,9 @@ BlockedPlan* blocked_plan_build(const FactorTask* tasks, int count) {
       auto A = [&](int r, int c) { return t.a + r + (size_t)c * lda; };
       const int rest = n - j - jb;
       if (lu_kind(t.kind)) {
-        panel.push_back({i, j, jb, 0, 0});
+        // full-width panels of a blocked factorization go leaf by leaf with all others
+        if (jb == bb::RECUR_W && nbt[i] < n) recur.push_back({i, j, jb, 0, 0});
+        else panel.push_back({i, j, jb, 0, 0});
         if (rest > 0) {
           for (int c = 0; c < rest; c += bb::TRSM_COLS) trsm_lu.push_back({i, j, jb, c, std::min(bb::TRSM_COLS, rest - c)});
           gemm.push_back({rest, rest, jb, lda, lda, lda, 0, 0, 0, 0, MONE, ONE, A(j + jb, j), A(j, j + jb),
@@ -374,6 +459,31 @@ BlockedPlan* blocked_plan_build(const FactorTask* tasks, int count) {
     st.n_gemm = (int)gemm.size();
     st.trsm_ch = put(p->image, trsm_ch);
     st.n_trsm_ch = (int)trsm_ch.size();
+    st.recur = put(p->image, recur);
+    st.n_recur = (int)recur.size();
+    if (!recur.empty()) {
+      for (int jr = 0; jr < bb::RECUR_W; ++jr) {
+        int ns, nmid, ne;
+        if (!node_after_host(jr, bb::RECUR_W, ns, nmid, ne)) continue;
+        BlockedPlan::Node nd{jr, ns, nmid, ne, 16, 0, 0, 0};
+        long t64 = 0, t32 = 0;
+        for (const bb::Item& it : recur) {
+          const int m = tasks[it.task].n - it.j - nmid;
+          t64 += gemm_tiles(m, ne - nmid, 64);
+          t32 += gemm_tiles(m, ne - nmid, 32);
+        }
+        const long want = 2L * runtime().cus;
+        nd.tile = t64 >= want ? 64 : (t32 >= want ? 32 : 16);
+        std::vector<int> start(recur.size());
+        for (size_t q = 0; q < recur.size(); ++q) {
+          start[q] = nd.total;
+          nd.total += gemm_tiles(tasks[recur[q].task].n - recur[q].j - nmid, ne - nmid, nd.tile);
+        }
+        nd.tiles = put(p->image, start);
+        nd.counter = node_counters++;
+        st.nodes.push_back(nd);
+      }
+    }
     p->steps.push_back(st);
   }
   // the left columns' interchanges of the blocked LU tasks
@@ -396,7 +506,8 @@ BlockedPlan* blocked_plan_build(const FactorTask* tasks, int count) {
   }
   p->solve = put(p->image, solve);
   p->n_solve = (int)solve.size();
-  p->ncounters = 2 * (int)p->steps.size();
+  p->node_counter_base = 2 * (int)p->steps.size();
+  p->ncounters = p->node_counter_base + node_counters;
   p->counters = put(p->image, std::vector<int>(std::max(1, p->ncounters), 0));
   // info of tasks given none: the steps read it as their stop flag
   p->spare_info = put(p->image, std::vector<int>(std::max(1, count), 0));
@@ -438,6 +549,20 @@ hipError_t blocked_plan_launch(const BlockedPlan* p, hipStream_t s) {
     if (st.n_panel > 0)
       hipLaunchKernelGGL(k_bb_panel, dim3(st.n_panel), dim3(NT), 0, s, tasks,
                          reinterpret_cast<const Item*>(d + st.panel));
+    if (st.n_recur > 0) {
+      const Item* ri = reinterpret_cast<const Item*>(d + st.recur);
+      size_t nn = 0;
+      for (int jr = 0; jr < RECUR_W; ++jr) {
+        const BlockedPlan::Node* nd = (nn < st.nodes.size() && st.nodes[nn].jr == jr) ? &st.nodes[nn++] : nullptr;
+        // the node's Rtrsm rides on the leaf that ends its left half
+        hipLaunchKernelGGL(k_bb_leaf, dim3(st.n_recur), dim3(NT), 0, s, tasks, ri, jr, nd ? nd->s : 0,
+                           nd ? nd->mid : 0, nd ? nd->e : 0);
+        if (nd && nd->total > 0)
+          (void)launch_node_gemm(*nd, tasks, ri, st.n_recur, reinterpret_cast<const int*>(d + nd->tiles),
+                                 counters + p->node_counter_base + nd->counter,
+                                 (unsigned)p->gemm_grid[grid_index(nd->tile)], s);
+      }
+    }
     if (st.n_trsm_lu > 0)
       hipLaunchKernelGGL(k_bb_trsm, dim3(st.n_trsm_lu), dim3(NT), 0, s, tasks,
                          reinterpret_cast<const Item*>(d + st.trsm_lu));
