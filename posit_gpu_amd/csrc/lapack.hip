@@ -9,6 +9,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
+#include <vector>
 
 #include "pga_internal.hpp"
 #include "posit32_core.hpp"
@@ -661,9 +663,108 @@ static hipError_t potrf2(const Ctx& c, bool upper, int n, uint32_t* a, int lda, 
   return potrf2(c, upper, n2, a22, lda, info, col_off + n1);
 }
 
+// B := A on an m x n block; with a stop flag set, nothing (the panel update of a block
+// whose diagonal factorization failed is not applied: the reference stops before it)
+__global__ __launch_bounds__(WG) void k_copy_block(int m, int n, const uint32_t* A, int lda, uint32_t* B, int ldb,
+                                                   const int* stop) {
+  PGA_KERNEL_VGPR_FLOOR();
+  if (stopped(stop)) return;
+  for (long e = blockIdx.x * (long)WG + threadIdx.x; e < (long)m * n; e += (long)gridDim.x * WG)
+    B[e % m + (size_t)(e / m) * ldb] = A[e % m + (size_t)(e / m) * lda];
+}
+
+static hipError_t copy_block(hipStream_t s, int m, int n, const uint32_t* A, int lda, uint32_t* B, int ldb,
+                             const int* stop) {
+  const long total = (long)m * n;
+  if (total <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_copy_block, dim3((unsigned)std::min<long>((total + WG - 1) / WG, 4096)), dim3(WG), 0, s, m, n,
+                     A, lda, B, ldb, stop);
+  return hipGetLastError();
+}
+
+// A second stream with two events for one factorization at a time (kept for reuse; the
+// pool grows to the number of factorizations running at once).
+struct Aux {
+  hipStream_t s = nullptr;
+  hipEvent_t join = nullptr, done = nullptr;
+};
+static std::mutex g_aux_mu;
+static std::vector<Aux> g_aux_free;
+
+static hipError_t aux_take(Aux& x) {
+  {
+    std::lock_guard<std::mutex> lk(g_aux_mu);
+    if (!g_aux_free.empty()) {
+      x = g_aux_free.back();
+      g_aux_free.pop_back();
+      return hipSuccess;
+    }
+  }
+  PGA_TRY_E(hipStreamCreateWithFlags(&x.s, hipStreamNonBlocking));
+  PGA_TRY_E(hipEventCreateWithFlags(&x.join, hipEventDisableTiming));
+  return hipEventCreateWithFlags(&x.done, hipEventDisableTiming);
+}
+static void aux_give(const Aux& x) {
+  std::lock_guard<std::mutex> lk(g_aux_mu);
+  g_aux_free.push_back(x);
+}
+
+// Orders from which the panel GEMM of each block runs beside the block's diagonal steps
+constexpr int POTRF_OVERLAP_MIN_N = 1024;
+
+// Rpotrf_opencl with the panel GEMM of block j on a second stream, into a workspace W:
+// it needs only the columns before j, so it runs while the same block's Rsyrk and
+// Rpotrf2 run on the caller's stream.  W is copied over the panel once Rpotrf2 has
+// succeeded (info doubles as the stop flag), then the panel's Rtrsm runs.  Every
+// operation sees the same operands as in the sequential order, so the bits are the same.
+static hipError_t potrf_overlap(const Ctx& c, bool upper, int n, uint32_t* a, int lda, int* info, int nb) {
+  auto A = [&](int r, int col) { return a + r + (size_t)col * lda; };
+  Aux x;
+  PGA_TRY_E(aux_take(x));
+  uint32_t* W = nullptr;
+  hipError_t e = hipMallocAsync((void**)&W, (size_t)(n - nb) * nb * sizeof(uint32_t), c.s);
+  bool pending = false;  // the aux stream has work that c.s has not joined
+  for (int j = 0; j < n && e == hipSuccess; j += nb) {
+    const int jb = (n - j < nb) ? n - j : nb, rest = n - j - jb;
+    const int wm = upper ? jb : rest, wn = upper ? rest : jb;  // W's shape (the panel)
+    uint32_t* P = upper ? A(j, j + jb) : A(j + jb, j);
+    const bool gemm = rest > 0 && j > 0;  // Rgemm quick-returns on K = 0
+    if (gemm) {
+      if ((e = hipEventRecord(x.join, c.s)) != hipSuccess) break;
+      if ((e = hipStreamWaitEvent(x.s, x.join, 0)) != hipSuccess) break;
+      if ((e = copy_block(x.s, wm, wn, P, lda, W, wm, nullptr)) != hipSuccess) break;
+      e = upper ? rgemm_launch(true, false, jb, rest, j, P_MONE, A(0, j), lda, A(0, j + jb), lda, P_ONE, W, wm, x.s,
+                               c.impl)
+                : rgemm_launch(false, true, rest, jb, j, P_MONE, A(j + jb, 0), lda, A(j, 0), lda, P_ONE, W, wm, x.s,
+                               c.impl);
+      if (e != hipSuccess || (e = hipEventRecord(x.done, x.s)) != hipSuccess) break;
+      pending = true;
+    }
+    if (upper) e = syrk(c, true, true, jb, j, P_MONE, A(0, j), lda, P_ONE, A(j, j), lda, info);
+    else e = syrk(c, false, false, jb, j, P_MONE, A(j, 0), lda, P_ONE, A(j, j), lda, info);
+    if (e == hipSuccess) e = potrf2(c, upper, jb, A(j, j), lda, info, j);
+    if (e != hipSuccess || rest == 0) break;
+    if (gemm) {
+      if ((e = hipStreamWaitEvent(c.s, x.done, 0)) != hipSuccess) break;
+      pending = false;
+      if ((e = copy_block(c.s, wm, wn, W, wm, P, lda, info)) != hipSuccess) break;
+    }
+    e = upper ? trsm(c, true, true, true, true, jb, rest, P_ONE, A(j, j), lda, P, lda, info)
+              : trsm(c, false, false, true, true, rest, jb, P_ONE, A(j, j), lda, P, lda, info);
+  }
+  if (pending) (void)hipStreamWaitEvent(c.s, x.done, 0);
+  if (W) {
+    hipError_t f = hipFreeAsync(W, c.s);
+    if (e == hipSuccess) e = f;
+  }
+  aux_give(x);
+  return e;
+}
+
 // Rpotrf_opencl (Rpotrf_opencl.cpp:32-166): blocked left-looking Cholesky
 static hipError_t potrf(const Ctx& c, bool upper, int n, uint32_t* a, int lda, int* info, int nb) {
   if (nb <= 1 || nb >= n) return potrf2(c, upper, n, a, lda, info, 0);
+  if (n >= POTRF_OVERLAP_MIN_N) return potrf_overlap(c, upper, n, a, lda, info, nb);
   auto A = [&](int r, int col) { return a + r + (size_t)col * lda; };
   for (int j = 0; j < n; j += nb) {
     const int jb = (n - j < nb) ? n - j : nb;

@@ -287,3 +287,23 @@ def test_cholesky_1024_device(pga, oracle):
     assert oracle.rpotrf("L", want, nb=0, nthreads=THREADS) == (0, 0)
     assert int(di.to_host()[0]) == 0
     _same(np.tril(dA.to_host()), np.tril(want))
+
+
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("bad", [0, 700])
+def test_cholesky_panel_overlap(pga, oracle, uplo, bad):
+    """Orders from 1024 run each block's panel GEMM on a second stream beside the block's
+    Rsyrk / Rpotrf2; the factor, and after a failing block the whole matrix the reference
+    leaves behind, must stay the same bits."""
+    n = 1100
+    A = _spd(oracle, 62, n)
+    if bad:
+        A[bad, bad] = oracle.from_double(-2.0)
+    want = A.copy(order="F")
+    _, winfo = oracle.rpotrf(uplo, want, nb=64, nthreads=THREADS)
+    dA = pga.DeviceArray.from_host(A)
+    di = pga.DeviceArray(1, np.int32, "C")
+    pga.potrf_device(uplo, n, dA, n, di, 64)
+    pga.synchronize()
+    assert int(di.to_host()[0]) == winfo == (bad + 1 if bad else 0)
+    _same(dA.to_host(), want)
