@@ -15,6 +15,7 @@
 #include "pga_internal.hpp"
 #include "posit32_core.hpp"
 #include "rgemm_kernels.hpp"
+#include "rgemm_tile.hpp"
 #include "lu_leaf.hpp"
 #include "wg_factor.hpp"
 
@@ -761,9 +762,143 @@ static hipError_t potrf_overlap(const Ctx& c, bool upper, int n, uint32_t* a, in
   return e;
 }
 
+// The lower Cholesky's trailing update by one finished block of kb columns, L (rows
+// from the region's origin down):  C(i, c) -= L(i, l) * L(c, l) for l = 0 .. kb-1 in
+// order, on the lower trapezoid i >= c of an m x ncols region whose origin lies on the
+// nb-block grid.  An element whose row and column fall in one nb x nb diagonal block
+// receives the Rsyrk 'L','N' terms (zero multipliers L(c, l) skipped, Rsyrk.cpp:149-169),
+// any other element the RgemmG 'N','T' terms (no skip, Rpotrf_opencl.cpp:146) -- the same
+// per-element steps as the left-looking order's Rsyrk on the diagonal block and RgemmG on
+// the panel below it.
+// T divides nb, so each T x T tile is wholly one or the other.  One workgroup per tile of
+// the lower trapezoid, tiles numbered column by column.
+template <int T>
+__global__ __launch_bounds__(WG) void k_potrf_trail(int m, int ncols, int kb, const uint32_t* __restrict__ L,
+                                                    int lda, uint32_t* __restrict__ C, int nb, const int* stop) {
+  PGA_KERNEL_VGPR_FLOOR();
+  __shared__ uint32_t lds[gemm::TileLds<T, T, 16>::words];
+  if (stopped(stop)) return;
+  const int tm = (m + T - 1) / T, tc = (ncols + T - 1) / T, t = blockIdx.x;
+  // column q holds tiles q .. tm-1: it starts at q*tm - q*(q-1)/2
+  int lo = 0, hi = tc - 1;
+  while (lo < hi) {
+    const int q = (lo + hi + 1) >> 1;
+    if (q * tm - q * (q - 1) / 2 <= t) lo = q;
+    else hi = q - 1;
+  }
+  const int tj = lo, ti = tj + (t - (tj * tm - tj * (tj - 1) / 2));
+  const int i0 = ti * T, j0 = tj * T;
+  if (i0 / nb == j0 / nb)
+    gemm::gemm_tile_in<T, T, 16, T / 16, T / 16, false, true, true>(m, ncols, kb, P_MONE, L, lda, L, lda, P_ONE, C,
+                                                                   lda, i0, j0, 1, lds);
+  else
+    gemm::gemm_tile_in<T, T, 16, T / 16, T / 16, false, true, false>(m, ncols, kb, P_MONE, L, lda, L, lda, P_ONE, C,
+                                                                    lda, i0, j0, 0, lds);
+}
+
+// Rpotrf_opencl returns at the first failing diagonal block (Rpotrf_opencl.cpp:137-139):
+// the panel below that block and every later column keep their input values.  Once a
+// right-looking factorization has failed, put the saved inputs back there: columns c from
+// the failed block's start J, rows from max(c, J + nb).
+__global__ __launch_bounds__(WG) void k_potrf_restore(int n, uint32_t* __restrict__ a, int lda,
+                                                      const uint32_t* __restrict__ save, int nb,
+                                                      const int* __restrict__ info) {
+  PGA_KERNEL_VGPR_FLOOR();
+  const int f = *info;
+  if (f == 0) return;
+  const int J = (f - 1) / nb * nb, r0 = min(n, J + nb);
+  for (int c = J + blockIdx.y; c < n; c += gridDim.y)
+    for (int i = max(c, r0) + blockIdx.x * WG + threadIdx.x; i < n; i += gridDim.x * WG)
+      a[i + (size_t)c * lda] = save[i + (size_t)c * n];
+}
+
+// Tile edge for a trailing update of about m x ncols / 2 elements: the largest of 64, 32,
+// 16 that divides nb and still gives every CU two tiles.
+static int trail_tile(long m, long ncols, int nb) {
+  const long want = 2L * runtime().cus;
+  for (int T : {64, 32}) {
+    if (nb % T) continue;
+    const long tm = (m + T - 1) / T, tc = (ncols + T - 1) / T;
+    if (tc * tm - tc * (tc - 1) / 2 >= want) return T;
+  }
+  return 16;
+}
+
+static hipError_t potrf_trail(hipStream_t s, int m, int ncols, int kb, const uint32_t* L, int lda, uint32_t* C,
+                              int nb, const int* stop) {
+  if (m <= 0 || ncols <= 0 || kb <= 0) return hipSuccess;
+  const int T = trail_tile(m, ncols, nb);
+  const long tm = (m + T - 1) / T, tc = (ncols + T - 1) / T;
+  const unsigned grid = (unsigned)(tc * tm - tc * (tc - 1) / 2);
+  if (T == 64)
+    hipLaunchKernelGGL(k_potrf_trail<64>, dim3(grid), dim3(WG), 0, s, m, ncols, kb, L, lda, C, nb, stop);
+  else if (T == 32)
+    hipLaunchKernelGGL(k_potrf_trail<32>, dim3(grid), dim3(WG), 0, s, m, ncols, kb, L, lda, C, nb, stop);
+  else hipLaunchKernelGGL(k_potrf_trail<16>, dim3(grid), dim3(WG), 0, s, m, ncols, kb, L, lda, C, nb, stop);
+  return hipGetLastError();
+}
+
+// Lower Rpotrf_opencl in right-looking order with one block of lookahead.  Each element
+// still receives its terms in ascending l with the Rsyrk or Rgemm form the left-looking
+// order gives it (k_potrf_trail), then Rpotrf2 or the panel's Rtrsm, so the factor is
+// the same bits.  Per block j, on the caller's stream: the update of block column j+1 by
+// block j, then Rpotrf2 and the panel Rtrsm of block j+1; meanwhile a second stream
+// applies block j to every later column -- the bulk of the work, as large GEMM launches.
+// A failing block leaves info set (every later launch then does nothing) and the saved
+// input is copied back over what Rpotrf_opencl never touches.  The (upper) 'T' forms sum
+// each element's terms before adding them, so they cannot be split by block this way.
+static hipError_t potrf_right_lower(const Ctx& c, int n, uint32_t* a, int lda, int* info, int nb) {
+  auto A = [&](int r, int col) { return a + r + (size_t)col * lda; };
+  uint32_t* save = nullptr;
+  PGA_TRY_E(hipMallocAsync((void**)&save, (size_t)n * n * sizeof(uint32_t), c.s));
+  Aux x;
+  hipError_t e = aux_take(x);
+  if (e != hipSuccess) {
+    (void)hipFreeAsync(save, c.s);
+    return e;
+  }
+  e = copy_block(c.s, n, n, a, lda, save, n, nullptr);
+  const int jb0 = std::min(nb, n);
+  if (e == hipSuccess) e = potrf2(c, false, jb0, a, lda, info, 0);
+  if (e == hipSuccess && jb0 < n) e = trsm(c, false, false, true, true, n - jb0, jb0, P_ONE, a, lda, A(jb0, 0), lda, info);
+  bool recorded = false;  // the aux stream has recorded x.done at least once
+  for (int j = 0; j + nb < n && e == hipSuccess; j += nb) {
+    const int c0 = j + nb, nb1 = std::min(nb, n - c0), c1 = c0 + nb1;
+    // block j is final: the aux stream applies it to the columns from c1 on
+    if (c1 < n) {
+      if ((e = hipEventRecord(x.join, c.s)) != hipSuccess) break;
+      if ((e = hipStreamWaitEvent(x.s, x.join, 0)) != hipSuccess) break;
+      if ((e = potrf_trail(x.s, n - c1, n - c1, nb, A(c1, j), lda, A(c1, c1), nb, info)) != hipSuccess) break;
+    }
+    // block column j+1 has had every earlier block's terms once the previous aux launch ends
+    if (recorded && (e = hipStreamWaitEvent(c.s, x.done, 0)) != hipSuccess) break;
+    if (c1 < n) {
+      if ((e = hipEventRecord(x.done, x.s)) != hipSuccess) break;
+      recorded = true;
+    }
+    if ((e = potrf_trail(c.s, n - c0, nb1, nb, A(c0, j), lda, A(c0, c0), nb, info)) != hipSuccess) break;
+    if ((e = potrf2(c, false, nb1, A(c0, c0), lda, info, c0)) != hipSuccess) break;
+    if (c1 < n) e = trsm(c, false, false, true, true, n - c1, nb1, P_ONE, A(c0, c0), lda, A(c1, c0), lda, info);
+  }
+  if (recorded) {
+    hipError_t w = hipStreamWaitEvent(c.s, x.done, 0);
+    if (e == hipSuccess) e = w;
+  }
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_potrf_restore, dim3((unsigned)std::min(16, (n + WG - 1) / WG), (unsigned)std::min(n, 256)),
+                       dim3(WG), 0, c.s, n, a, lda, save, nb, info);
+    e = hipGetLastError();
+  }
+  hipError_t f = hipFreeAsync(save, c.s);
+  if (e == hipSuccess) e = f;
+  aux_give(x);
+  return e;
+}
+
 // Rpotrf_opencl (Rpotrf_opencl.cpp:32-166): blocked left-looking Cholesky
 static hipError_t potrf(const Ctx& c, bool upper, int n, uint32_t* a, int lda, int* info, int nb) {
   if (nb <= 1 || nb >= n) return potrf2(c, upper, n, a, lda, info, 0);
+  if (!upper && n >= POTRF_OVERLAP_MIN_N && nb % 16 == 0) return potrf_right_lower(c, n, a, lda, info, nb);
   if (n >= POTRF_OVERLAP_MIN_N) return potrf_overlap(c, upper, n, a, lda, info, nb);
   auto A = [&](int r, int col) { return a + r + (size_t)col * lda; };
   for (int j = 0; j < n; j += nb) {

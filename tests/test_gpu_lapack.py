@@ -289,21 +289,23 @@ def test_cholesky_1024_device(pga, oracle):
     _same(np.tril(dA.to_host()), np.tril(want))
 
 
-@pytest.mark.parametrize("uplo", ["L", "U"])
-@pytest.mark.parametrize("bad", [0, 700])
-def test_cholesky_panel_overlap(pga, oracle, uplo, bad):
-    """Orders from 1024 run each block's panel GEMM on a second stream beside the block's
-    Rsyrk / Rpotrf2; the factor, and after a failing block the whole matrix the reference
+@pytest.mark.parametrize("uplo,n,nb,bad", [("L", 1100, 64, 0), ("L", 1100, 64, 700), ("U", 1100, 64, 0),
+                                           ("U", 1100, 64, 700), ("L", 1300, 128, 0), ("L", 1300, 128, 5),
+                                           ("L", 1300, 128, 1290), ("L", 1111, 48, 1063), ("L", 1040, 80, 0),
+                                           ("L", 1040, 80, 959)])
+def test_cholesky_panel_overlap(pga, oracle, uplo, n, nb, bad):
+    """Orders from 1024 run on two streams: the upper factor with each block's panel GEMM
+    beside the block's Rsyrk / Rpotrf2, the lower one right-looking with one block of
+    lookahead.  The factor, and after a failing block the whole matrix the reference
     leaves behind, must stay the same bits."""
-    n = 1100
     A = _spd(oracle, 62, n)
     if bad:
         A[bad, bad] = oracle.from_double(-2.0)
     want = A.copy(order="F")
-    _, winfo = oracle.rpotrf(uplo, want, nb=64, nthreads=THREADS)
+    _, winfo = oracle.rpotrf(uplo, want, nb=nb, nthreads=THREADS)
     dA = pga.DeviceArray.from_host(A)
     di = pga.DeviceArray(1, np.int32, "C")
-    pga.potrf_device(uplo, n, dA, n, di, 64)
+    pga.potrf_device(uplo, n, dA, n, di, nb)
     pga.synchronize()
     assert int(di.to_host()[0]) == winfo == (bad + 1 if bad else 0)
     _same(dA.to_host(), want)

@@ -326,3 +326,62 @@ def test_cholesky_not_pd_info(oracle):
     F = A.copy(order="F")
     rc, info = oracle.rpotrf("L", F, nb=16)
     assert rc == 0 and info == 41
+
+
+def _right_looking_lower(oracle, A, nb):
+    """Lower Cholesky in right-looking order, from the oracle's Rpotrf2 / Rtrsm / Rsyrk /
+    Rgemm: once block j is factored, its columns update every later column at once
+    (Rsyrk 'L','N' on each diagonal block, Rgemm 'N','T' below it, K = block j only);
+    after a failing block the saved input is put back over the panel below it and every
+    later column.  This is the order of the device's lower driver for large n."""
+    n = A.shape[0]
+    save = A.copy(order="F")
+    info = 0
+    for j in range(0, n, nb):
+        jb = min(nb, n - j)
+        D = np.asfortranarray(A[j:j + jb, j:j + jb])
+        _, f = oracle.rpotrf2("L", D)
+        A[j:j + jb, j:j + jb] = D
+        if f:
+            info = j + f
+            break
+        if j + jb == n:
+            break
+        P = np.asfortranarray(A[j + jb:, j:j + jb])
+        oracle.rtrsm("R", "L", "T", "N", n - j - jb, jb, ONE, D, P)
+        A[j + jb:, j:j + jb] = P
+        for c in range(j + jb, n, nb):
+            cb = min(nb, n - c)
+            Lc = np.asfortranarray(A[c:c + cb, j:j + jb])
+            C = np.asfortranarray(A[c:c + cb, c:c + cb])
+            oracle.rsyrk("L", "N", cb, jb, MONE, Lc, ONE, C)
+            A[c:c + cb, c:c + cb] = C
+            if c + cb < n:
+                Lr = np.asfortranarray(A[c + cb:, j:j + jb])
+                C = np.asfortranarray(A[c + cb:, c:c + cb])
+                oracle.rgemm("N", "T", n - c - cb, cb, jb, MONE, Lr, Lc, ONE, C)
+                A[c + cb:, c:c + cb] = C
+    if info:
+        J = (info - 1) // nb * nb
+        r0 = min(n, J + nb)
+        for c in range(J, n):
+            A[max(c, r0):, c] = save[max(c, r0):, c]
+    return info
+
+
+@pytest.mark.parametrize("n,nb,bad", [(150, 32, None), (150, 32, 0), (150, 32, 40), (150, 32, 100),
+                                      (150, 32, 149), (130, 48, None), (130, 48, 97)])
+def test_cholesky_lower_right_looking_order(oracle, n, nb, bad):
+    """The lower factor splits by K block: each element gets the same terms in the same
+    order whether its columns are applied left-looking (Rpotrf_opencl) or one finished
+    block at a time, and after a failure the matrix the reference leaves is recovered from
+    the saved input."""
+    A = spd_matrix(oracle, 54, n)
+    if bad is not None:
+        A[bad, bad] = oracle.from_double(-2.0)
+    F = A.copy(order="F")
+    _, want = oracle.rpotrf("L", F, nb=nb)
+    G = A.copy(order="F")
+    got = _right_looking_lower(oracle, G, nb)
+    assert got == want == (0 if bad is None else bad + 1)
+    assert np.array_equal(G, F)
