@@ -584,11 +584,94 @@ static hipError_t getrf2(const Ctx& c, int m, int n, uint32_t* a, int lda, int* 
   return getrf2_node(c, m, n, a, lda, ipiv, info, row_off, col_off, 0, n, 0, 0, 0);
 }
 
+// A second stream with two events for one factorization at a time (kept for reuse; the
+// pool grows to the number of factorizations running at once).
+struct Aux {
+  hipStream_t s = nullptr;
+  hipEvent_t join = nullptr, done = nullptr;
+};
+static std::mutex g_aux_mu;
+static std::vector<Aux> g_aux_free;
+
+static hipError_t aux_take(Aux& x) {
+  {
+    std::lock_guard<std::mutex> lk(g_aux_mu);
+    if (!g_aux_free.empty()) {
+      x = g_aux_free.back();
+      g_aux_free.pop_back();
+      return hipSuccess;
+    }
+  }
+  PGA_TRY_E(hipStreamCreateWithFlags(&x.s, hipStreamNonBlocking));
+  PGA_TRY_E(hipEventCreateWithFlags(&x.join, hipEventDisableTiming));
+  return hipEventCreateWithFlags(&x.done, hipEventDisableTiming);
+}
+static void aux_give(const Aux& x) {
+  std::lock_guard<std::mutex> lk(g_aux_mu);
+  g_aux_free.push_back(x);
+}
+
+// The update of columns [c0, c0 + w) by block j of an LU (Rgetrf_opencl.cpp:123-146 on
+// those columns): the block's row interchanges, the Rtrsm of its jb rows, the Rgemm of the
+// rows below.  Columns are independent here, so any split of the right-hand columns
+// gives the same bits.
+static hipError_t getrf_update(const Ctx& c, int m, uint32_t* a, int lda, const int* ipiv, int j, int jb, int c0,
+                               int w) {
+  if (w <= 0) return hipSuccess;
+  uint32_t* ajj = a + j + (size_t)j * lda;
+  uint32_t* a12 = a + j + (size_t)c0 * lda;
+  PGA_TRY(laswp(c, w, a + (size_t)c0 * lda, lda, j + 1, j + jb, ipiv, 1, 0, nullptr));
+  PGA_TRY(trsm(c, true, false, false, false, jb, w, P_ONE, ajj, lda, a12, lda, nullptr));
+  if (j + jb < m)
+    PGA_TRY(rgemm_launch(false, false, m - j - jb, w, jb, P_MONE, ajj + jb, lda, a12, lda, P_ONE, a12 + jb, lda,
+                         c.s, c.impl));
+  return hipSuccess;
+}
+
+// Orders from which the LU runs with one panel of lookahead
+constexpr int GETRF_LOOKAHEAD_MIN = 1024;
+
+// Rgetrf_opencl with one panel of lookahead.  After panel j, the caller's stream updates
+// only block column j+1 and factors it as panel j+1, while a second stream updates the
+// columns from j+2 on (the bulk of the work, one large GEMM) and applies panel j's
+// interchanges to the columns left of it.  Each element sees the same operations in the
+// same order as in the sequential driver: block column j+1 waits for the second stream's
+// update by block j-1, and the left interchanges of panel j run after that update has
+// read panel j-1.
+static hipError_t getrf_lookahead(const Ctx& c, int m, int n, uint32_t* a, int lda, int* ipiv, int* info,
+                                  int nb) {
+  const int mn = m < n ? m : n;
+  Aux x;
+  PGA_TRY_E(aux_take(x));
+  const Ctx cx{x.s, c.impl};
+  hipError_t e = getrf2(c, m, std::min(nb, mn), a, lda, ipiv, info, 0, 0);
+  bool recorded = false;  // the aux stream has recorded x.done at least once
+  for (int j = 0; j < mn && e == hipSuccess; j += nb) {
+    const int jb = std::min(nb, mn - j), c0 = j + jb, w1 = std::min(nb, n - c0), c1 = c0 + std::max(w1, 0);
+    if ((e = hipEventRecord(x.join, c.s)) != hipSuccess) break;
+    if ((e = hipStreamWaitEvent(x.s, x.join, 0)) != hipSuccess) break;
+    if (j > 0 && (e = laswp(cx, j, a, lda, j + 1, j + jb, ipiv, 1, 0, nullptr)) != hipSuccess) break;
+    if ((e = getrf_update(cx, m, a, lda, ipiv, j, jb, c1, n - c1)) != hipSuccess) break;
+    if (recorded && (e = hipStreamWaitEvent(c.s, x.done, 0)) != hipSuccess) break;
+    if ((e = hipEventRecord(x.done, x.s)) != hipSuccess) break;
+    recorded = true;
+    if ((e = getrf_update(c, m, a, lda, ipiv, j, jb, c0, w1)) != hipSuccess) break;
+    if (c0 < mn) e = getrf2(c, m - c0, std::min(nb, mn - c0), a + c0 + (size_t)c0 * lda, lda, ipiv + c0, info, c0, c0);
+  }
+  if (recorded) {
+    hipError_t w = hipStreamWaitEvent(c.s, x.done, 0);
+    if (e == hipSuccess) e = w;
+  }
+  aux_give(x);
+  return e;
+}
+
 // Rgetrf_opencl (Rgetrf_opencl.cpp:32-154): blocked right-looking LU, nb columns per
 // panel; ipiv holds global 1-based rows, info the first zero pivot column.
 static hipError_t getrf(const Ctx& c, int m, int n, uint32_t* a, int lda, int* ipiv, int* info, int nb) {
   const int mn = m < n ? m : n;
   if (nb <= 1 || nb >= mn) return getrf2(c, m, n, a, lda, ipiv, info, 0, 0);
+  if (mn >= GETRF_LOOKAHEAD_MIN) return getrf_lookahead(c, m, n, a, lda, ipiv, info, nb);
   for (int j = 0; j < mn; j += nb) {
     const int jb = (mn - j < nb) ? mn - j : nb;
     uint32_t* ajj = a + j + (size_t)j * lda;
@@ -681,33 +764,6 @@ static hipError_t copy_block(hipStream_t s, int m, int n, const uint32_t* A, int
   hipLaunchKernelGGL(k_copy_block, dim3((unsigned)std::min<long>((total + WG - 1) / WG, 4096)), dim3(WG), 0, s, m, n,
                      A, lda, B, ldb, stop);
   return hipGetLastError();
-}
-
-// A second stream with two events for one factorization at a time (kept for reuse; the
-// pool grows to the number of factorizations running at once).
-struct Aux {
-  hipStream_t s = nullptr;
-  hipEvent_t join = nullptr, done = nullptr;
-};
-static std::mutex g_aux_mu;
-static std::vector<Aux> g_aux_free;
-
-static hipError_t aux_take(Aux& x) {
-  {
-    std::lock_guard<std::mutex> lk(g_aux_mu);
-    if (!g_aux_free.empty()) {
-      x = g_aux_free.back();
-      g_aux_free.pop_back();
-      return hipSuccess;
-    }
-  }
-  PGA_TRY_E(hipStreamCreateWithFlags(&x.s, hipStreamNonBlocking));
-  PGA_TRY_E(hipEventCreateWithFlags(&x.join, hipEventDisableTiming));
-  return hipEventCreateWithFlags(&x.done, hipEventDisableTiming);
-}
-static void aux_give(const Aux& x) {
-  std::lock_guard<std::mutex> lk(g_aux_mu);
-  g_aux_free.push_back(x);
 }
 
 // Orders from which the panel GEMM of each block runs beside the block's diagonal steps

@@ -276,6 +276,33 @@ def test_lu_1024_nb256_device(pga, oracle):
     _same(dB.to_host(), B)
 
 
+@pytest.mark.parametrize("m,n,nb,kind", [(1100, 1100, 64, None), (1300, 1100, 128, None), (1100, 1300, 96, None),
+                                         (1100, 1100, 64, "zero_column"), (1040, 1040, 256, "nar")])
+def test_lu_lookahead_device(pga, oracle, m, n, nb, kind):
+    """From min(m, n) = 1024 the LU factors panel j+1 while a second stream updates the
+    columns beyond it; pivots, info and every element must stay the same bits, for tall
+    and wide shapes, a zero column and NaR entries."""
+    A = posit_matrix(oracle, 64 + m + n, m, n)
+    if kind == "zero_column":
+        A[:, 700] = 0
+    elif kind == "nar":
+        A[5, 3] = NAR
+        A[900, 600] = NAR
+    mn = min(m, n)
+    dA = pga.DeviceArray.from_host(A)
+    dp = pga.DeviceArray(mn, np.int32, "C")
+    di = pga.DeviceArray(1, np.int32, "C")
+    pga.getrf_device(m, n, dA, m, dp, di, nb)
+    pga.synchronize()
+    want = A.copy(order="F")
+    _, wpiv, winfo = oracle.rgetrf(want, nb=nb, nthreads=THREADS)
+    assert int(di.to_host()[0]) == winfo
+    if kind == "zero_column":
+        assert winfo == 701
+    _same(dp.to_host(), wpiv)
+    _same(dA.to_host(), want)
+
+
 def test_cholesky_1024_device(pga, oracle):
     n = 1024
     A = _spd(oracle, 61, n)
