@@ -79,8 +79,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--config", default="gemm",
-                    choices=["gemm", "lu", "cholesky", "batched", "batched_factor", "blocked", "serve", "copycal"],
-                    help="gemm: trailing-update GEMMs (headline); lu / cholesky: blocked "
+                    choices=["gemm", "dropin", "lu", "cholesky", "batched", "batched_factor", "blocked", "serve",
+                             "copycal"],
+                    help="gemm: trailing-update GEMMs (headline); dropin: Rgemm 512^3 (alpha=1, beta=0) "
+                         "through the host-pointer RgemmG drop-in and device-resident; lu / cholesky: blocked "
                          "factorization + solve at n=8192; batched: 4096 small GEMMs; batched_factor: "
                          "2048 LU + 2048 Cholesky factorizations of n in [32, 512]; blocked: LU + "
                          "upper Cholesky solves of n in [513, 4096] in one blocked batch; serve: Poisson "
@@ -106,6 +108,8 @@ def main():
     pga.init(0)
     info = pga.device_info()
     log(f"device: {info['name']} {info['arch']} CUs={info['compute_units']} clock={info['clock_khz']} kHz")
+    if args.config == "dropin":
+        return bench_dropin(pga, info, args)
     if args.config == "lu":
         return bench_lu(pga, info, args)
     if args.config == "cholesky":
@@ -235,6 +239,57 @@ def _oracle():
     return Oracle()
 
 
+def bench_dropin(pga, info, args):
+    """Plain Rgemm C = A*B, M = N = K = 512, alpha = 1, beta = 0, through the host-pointer
+    drop-in RgemmG (copies in, kernel, copies out, synchronous like libRpack's RgemmG),
+    timed wall-clock around each call as main_gemm.cpp:128-140 does; beside it the same GEMM
+    on device-resident operands (HIP events).  value is the device-resident rate."""
+    n = 512
+    dA, dB, dC = pga.DeviceArray((n, n)), pga.DeviceArray((n, n)), pga.DeviceArray((n, n))
+    pga.fill_uniform(dA, 5101)
+    pga.fill_uniform(dB, 5102)
+    pga.fill_uniform(dC, 5103)
+    pga.synchronize()
+    A, B = dA.to_host(), dB.to_host()
+    C = dC.to_host()
+    ops = 2.0 * n ** 3
+    reps = max(args.steps, 5)
+    for _ in range(max(args.warmup, 1)):
+        pga.RgemmG("N", "N", n, n, n, P_ONE, A, n, B, n, 0, C, n)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        pga.RgemmG("N", "N", n, n, n, P_ONE, A, n, B, n, 0, C, n)
+    t_e2e = (time.perf_counter() - t0) / reps
+    pga.rgemm_device("N", "N", n, n, n, P_ONE, dA, n, dB, n, 0, dC, n)
+    pga.synchronize()
+    t_dev = _timed(pga, lambda: [pga.rgemm_device("N", "N", n, n, n, P_ONE, dA, n, dB, n, 0, dC, n)
+                                 for _ in range(reps)]) / reps
+    same = bool(np.array_equal(dC.to_host(), C))
+    cpu = None
+    if not args.no_cpu_baseline:
+        # the oracle's Rgemm (port of Rgemm.cpp NN) on the same operands
+        O = _oracle()
+        cores = min(16, os.cpu_count() or 1)
+        Cc = np.zeros((n, n), np.uint32, order="F")
+        t0 = time.perf_counter()
+        O.rgemm("N", "N", n, n, n, P_ONE, np.asfortranarray(A), np.asfortranarray(B), 0, Cc, nthreads=cores)
+        t_cpu = time.perf_counter() - t0
+        cpu = {"value": round(ops / t_cpu / 1e9, 4), "unit": "Gop/s", "cores": cores, "kind": "port",
+               "sample": f"oracle Rgemm NN 512^3, {t_cpu:.2f} s", "bitwise_equal": bool(np.array_equal(Cc, C))}
+    out = {
+        "metric": "posit32 GEMM Gop/s (2MNK/t)", "value": round(ops / t_dev / 1e9, 2), "unit": "Gop/s",
+        "n_gpus": 1, "steps": reps, "ms_per_step": round(t_dev * 1e3, 4), "higher_is_better": True,
+        "dtype": "posit32 (integer VALU)",
+        "data": "synthetic: seeded uniform[-1,1) FP64 -> posit32, generated on the device",
+        "config": {"workload": "Rgemm NN M=N=K=512, alpha=1, beta=0 (plain drop-in case)",
+                   "dropin_ms_per_call": round(t_e2e * 1e3, 4),
+                   "dropin_gops_including_copies": round(ops / t_e2e / 1e9, 2),
+                   "dropin_equals_device_resident": same, "device": info["name"]},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def bench_lu(pga, info, args):
     """Rgetrf_opencl (nb=256) + Rgetrs on the device, b = A*ones by the posit GEMV;
     block-size invariance (nb=64 gives the same bits) as the full-size parity check."""
@@ -325,6 +380,20 @@ def bench_cholesky(pga, info, args):
     same = bool(np.array_equal(L1, np.tril(dA.to_host())))
     del L1
     ops = n ** 3 / 3.0
+    cpu = None
+    if not args.no_cpu_baseline:
+        # the oracle's blocked lower Cholesky (port of Rpotrf_opencl / Rpotrf2) at n_cpu
+        O = _oracle()
+        cores = min(16, os.cpu_count() or 1)
+        n_cpu = 1024
+        Mc = uniform_doubles(4301, n_cpu * n_cpu).reshape(n_cpu, n_cpu)
+        Sc = Mc @ Mc.T / n_cpu + np.eye(n_cpu)
+        Ac = np.asfortranarray(O.from_double_v(Sc.ravel(order="F")).reshape(n_cpu, n_cpu, order="F"))
+        t0 = time.perf_counter()
+        O.rpotrf("L", Ac, nb=64, nthreads=cores)
+        t_cpu = time.perf_counter() - t0
+        cpu = {"value": round(n_cpu ** 3 / 3.0 / t_cpu / 1e9, 4), "unit": "Gop/s", "cores": cores,
+               "kind": "port", "sample": f"oracle Rpotrf lower n={n_cpu} nb=64, {t_cpu:.2f} s"}
     out = {
         "metric": "posit32 Cholesky Gop/s ((1/3)n^3/t)", "value": round(ops / t_f / 1e9, 2), "unit": "Gop/s",
         "n_gpus": 1, "higher_is_better": True, "dtype": "posit32 (integer VALU)",
@@ -332,6 +401,7 @@ def bench_cholesky(pga, info, args):
         "config": {"workload": f"Rpotrf_opencl lower n={n} nb=64 + Rpotrs", "factor_s": round(t_f, 4),
                    "solve_s": round(t_s, 4), "info": info_v, "backward_error": berr,
                    "lower_nb64_equals_nb128": same, "device": info["name"]},
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 

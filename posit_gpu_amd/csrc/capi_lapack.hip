@@ -54,16 +54,12 @@ pga_status illegal(int i) {
 std::recursive_mutex g_host_mutex;
 
 // Host<->device copies of a rows x cols column-major block with leading dimension ld
-// into a packed device buffer (leading dimension rows).
+// into a packed device buffer (leading dimension rows); synchronous.
 hipError_t put(uint32_t* d, const uint32_t* h, int rows, int cols, int ld, hipStream_t s) {
-  if (rows <= 0 || cols <= 0) return hipSuccess;
-  const size_t w = sizeof(uint32_t);
-  return hipMemcpy2DAsync(d, rows * w, h, (size_t)ld * w, rows * w, cols, hipMemcpyHostToDevice, s);
+  return copy_h2d_block(d, h, rows, cols, ld, s);
 }
 hipError_t get(uint32_t* h, const uint32_t* d, int rows, int cols, int ld, hipStream_t s) {
-  if (rows <= 0 || cols <= 0) return hipSuccess;
-  const size_t w = sizeof(uint32_t);
-  return hipMemcpy2DAsync(h, (size_t)ld * w, d, rows * w, rows * w, cols, hipMemcpyDeviceToHost, s);
+  return copy_d2h_block(h, d, rows, cols, ld, s);
 }
 
 // Rtrsm's checks (Rtrsm.cpp:57-88)

@@ -29,11 +29,12 @@ __device__ __forceinline__ bool stopped(const int* stop) { return stop && *stop;
 // ---------------------------------------------------------------------------------
 // TRSM, left side: one workgroup per column of B (Rtrsm.cpp:115-188)
 // ---------------------------------------------------------------------------------
+// (the not-transposed forms and the upper transposed one; k_trsm_lltn is the fourth)
 template <bool UPPER, bool TRANS>
 __global__ __launch_bounds__(WG) void k_trsm_left(int m, int n, uint32_t alpha, int nounit,
                                                   const uint32_t* __restrict__ A, int lda,
-                                                  uint32_t* __restrict__ B, int ldb,
-                                                  uint32_t* __restrict__ work, const int* stop) {
+                                                  uint32_t* __restrict__ B, int ldb, const int* stop) {
+  static_assert(UPPER || !TRANS, "the lower transposed form is k_trsm_lltn");
   PGA_KERNEL_VGPR_FLOOR();
   const int j = blockIdx.x;
   if (j >= n || stopped(stop)) return;
@@ -63,7 +64,7 @@ __global__ __launch_bounds__(WG) void k_trsm_left(int m, int n, uint32_t alpha, 
       if (active && nounit && tid == 0) b[k] = bk;
     }
     __syncthreads();
-  } else if (UPPER) {
+  } else {
     // temp_i = alpha*b_i - sum_{k<i} a_ki*b_k in ascending k (Rtrsm.cpp:161-173); b_k is
     // final before any term that needs it, so the sums advance as a sweep over k
     for (int i = tid; i < m; i += WG) b[i] = mul(alpha, b[i]);
@@ -76,39 +77,94 @@ __global__ __launch_bounds__(WG) void k_trsm_left(int m, int n, uint32_t alpha, 
       if (nounit && tid == 0) b[k] = bk;
     }
     __syncthreads();
-  } else {
-    // temp_i = alpha*b_i - sum_{k>i} a_ki*b_k in ascending k, i descending
-    // (Rtrsm.cpp:174-186): the first term of row i needs b_{i+1}, so the sums form one
-    // chain.  The negated products of a row are formed and decoded in parallel; one lane
-    // runs the chain of additions (exact in every case, see gemm::add_checked).
-    uint32_t* pm = work + (size_t)j * 3 * m;
-    int* ps = (int*)(pm + m);
-    uint32_t* pg = pm + 2 * (size_t)m;
-    for (int i = m - 1; i >= 0; --i) {
-      for (int k = i + 1 + tid; k < m; k += WG) {
-        dec::Val v = dec::decode(neg(mul(a(k, i), b[k])));
-        pm[k] = v.m;
-        ps[k] = v.sc;
-        pg[k] = v.sg;
-      }
-      __syncthreads();
-      if (tid == 0) {
-        dec::Val t = dec::decode(mul(alpha, b[i]));
-        int k = i + 1;
-        for (; k + 4 <= m; k += 4) {
-          dec::Val q[4];
+  }
+}
+
+// Left, lower, transposed (Rtrsm.cpp:174-186), one right-hand side per workgroup:
+// temp_i = alpha*b_i - sum_{k>i} a_ki*b_k in ascending k, i descending.  Row i's first
+// term needs b_{i+1}, so the rows form one chain of m^2/2 rounded additions.  Per row the
+// lanes form the negated products, decoded, in LDS (CHAIN_CHUNK terms at a time); wave 0
+// then runs the additions with every lane computing the same values, eight terms a group
+// while the next group's terms load.  A group whose fast additions left the fast range
+// (or met NaR) is redone from its start term by term in the exact form.
+constexpr int CHAIN_CHUNK = 4096;
+
+struct ChainGroup {
+  static constexpr int G = 8;
+  uint32_t m[G], g[G];
+  int s[G];
+  __device__ __forceinline__ void load(const uint32_t* sm, const int* ss, const uint32_t* sg, int k) {
 #pragma unroll
-          for (int u = 0; u < 4; ++u) q[u] = dec::Val{pm[k + u], ps[k + u], pg[k + u]};
+    for (int u = 0; u < G; ++u) m[u] = sm[k + u], s[u] = ss[k + u], g[u] = sg[k + u];
+  }
+  // t <- t + the group's terms in order: the fast form, or the exact one when any fast
+  // sum left the fast range
+  __device__ __forceinline__ void apply(dec::Val& t) {
+    // keep the terms in VGPRs (the chain runs on the VALU; uniform copies would be moved
+    // to SGPRs and back for every select)
 #pragma unroll
-          for (int u = 0; u < 4; ++u) gemm::add_checked(t, q[u]);
-        }
-        for (; k < m; ++k) gemm::add_checked(t, dec::Val{pm[k], ps[k], pg[k]});
-        uint32_t tp = dec::encode(t);
-        if (nounit) tp = div(tp, a(i, i));
-        b[i] = tp;
-      }
-      __syncthreads();
+    for (int u = 0; u < G; ++u) asm volatile("" : "+v"(m[u]), "+v"(s[u]), "+v"(g[u]));
+    const dec::Val save = t;
+    bool ok = true;
+#pragma unroll
+    for (int u = 0; u < G; ++u) ok &= gemm::add_fast(t, m[u], s[u], g[u]);
+    if (__builtin_amdgcn_readfirstlane((int)!ok)) {
+      t = save;
+#pragma unroll
+      for (int u = 0; u < G; ++u) gemm::add_checked(t, dec::Val{m[u], s[u], g[u]});
     }
+  }
+};
+
+// Two groups in flight: one loads from LDS while the other is added.
+__device__ __forceinline__ void chain_terms(dec::Val& t, const uint32_t* __restrict__ sm,
+                                            const int* __restrict__ ss, const uint32_t* __restrict__ sg,
+                                            int kn) {
+  constexpr int G = ChainGroup::G;
+  const int groups = kn / G;
+  ChainGroup x, y;
+  if (groups > 0) x.load(sm, ss, sg, 0);
+  for (int g = 0; g < groups; g += 2) {
+    if (g + 1 < groups) y.load(sm, ss, sg, (g + 1) * G);
+    x.apply(t);
+    if (g + 1 >= groups) break;
+    if (g + 2 < groups) x.load(sm, ss, sg, (g + 2) * G);
+    y.apply(t);
+  }
+  for (int k = groups * G; k < kn; ++k) gemm::add_checked(t, dec::Val{sm[k], ss[k], sg[k]});
+}
+
+__global__ __launch_bounds__(WG) void k_trsm_lltn(int m, int n, uint32_t alpha, int nounit,
+                                                  const uint32_t* __restrict__ A, int lda,
+                                                  uint32_t* __restrict__ B, int ldb, const int* stop) {
+  PGA_KERNEL_VGPR_FLOOR();
+  __shared__ uint32_t sm[CHAIN_CHUNK], sg[CHAIN_CHUNK];
+  __shared__ int ss[CHAIN_CHUNK];
+  const int j = blockIdx.x;
+  if (j >= n || stopped(stop)) return;
+  uint32_t* b = B + (size_t)j * ldb;
+  const int tid = threadIdx.x;
+  for (int i = m - 1; i >= 0; --i) {
+    dec::Val t{};
+    if (tid < 64) t = dec::decode(mul(alpha, b[i]));
+    for (int k0 = i + 1; k0 < m; k0 += CHAIN_CHUNK) {
+      const int kn = min(CHAIN_CHUNK, m - k0);
+      __syncthreads();  // wave 0 has used the previous chunk
+      for (int u = tid; u < kn; u += WG) {
+        const dec::Val v = dec::decode(neg(mul(A[k0 + u + (size_t)i * lda], b[k0 + u])));
+        sm[u] = v.m;
+        ss[u] = v.sc;
+        sg[u] = v.sg;
+      }
+      __syncthreads();
+      if (tid < 64) chain_terms(t, sm, ss, sg, kn);
+    }
+    if (tid == 0) {
+      uint32_t tp = dec::encode(t);
+      if (nounit) tp = div(tp, A[i + (size_t)i * lda]);
+      b[i] = tp;
+    }
+    __syncthreads();  // b_i is final before row i-1 forms its products
   }
 }
 
@@ -481,27 +537,16 @@ static hipError_t trsm(const Ctx& c, bool left, bool upper, bool trans, bool nou
     return hipGetLastError();
   }
   if (left) {
-    // the left-lower-transposed form keeps decoded products in a work buffer of 3*m
-    // words per column: stream-ordered, so calls on different streams never share it
-    uint32_t* work = nullptr;
-    if (trans && !upper) {
-      hipError_t e = hipMallocAsync((void**)&work, (size_t)3 * m * n * sizeof(uint32_t), c.s);
-      if (e != hipSuccess) return e;
-    }
 #define PGA_TL(U, T)                                                                               \
   hipLaunchKernelGGL((k_trsm_left<U, T>), dim3(n), dim3(WG), 0, c.s, m, n, alpha, (int)nounit, A,  \
-                     lda, B, ldb, work, stop)
+                     lda, B, ldb, stop)
     if (upper && !trans) PGA_TL(true, false);
     else if (!upper && !trans) PGA_TL(false, false);
     else if (upper && trans) PGA_TL(true, true);
-    else PGA_TL(false, true);
+    else
+      hipLaunchKernelGGL(k_trsm_lltn, dim3(n), dim3(WG), 0, c.s, m, n, alpha, (int)nounit, A, lda, B, ldb, stop);
 #undef PGA_TL
-    hipError_t e = hipGetLastError();
-    if (work) {
-      hipError_t f = hipFreeAsync(work, c.s);
-      if (e == hipSuccess) e = f;
-    }
-    return e;
+    return hipGetLastError();
   } else if (trans) {
     // enough row blocks for every CU twice where m allows, at least 4 rows each
     const int rows = (int)std::min<long>(256, std::max<long>(4, (m + 2L * runtime().cus - 1) / (2L * runtime().cus)));

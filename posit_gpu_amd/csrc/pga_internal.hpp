@@ -78,6 +78,10 @@ pga_status from_hip(hipError_t e);
 
 // Growable device scratch buffers used by the host-pointer entry points.
 void* scratch(int slot, size_t bytes, pga_status* st);
+// Synchronous host <-> device copies of a column-major block through pinned staging
+// (host leading dimension ld, device leading dimension rows).
+hipError_t copy_h2d_block(uint32_t* d, const uint32_t* h, int rows, int cols, int ld, hipStream_t s);
+hipError_t copy_d2h_block(uint32_t* h, const uint32_t* d, int rows, int cols, int ld, hipStream_t s);
 
 }  // namespace pga
 

@@ -156,6 +156,33 @@ PGA_D void add_checked(Val& c, const Val& p) {
   c = dec::decode(pga::add(dec::encode(c), dec::encode(p)));
 }
 
+// c <- round(c + p) on the fast decoded path only: returns false unless the sum's scale
+// lies in [FAST_MIN, FAST_MAX], where the decoded arithmetic rounds exactly.  Every other
+// case -- a NaR operand (its scale carries the sum far above the range), a zero sum (far
+// below), a result too large or too small to round here -- returns false with a
+// meaningless c, and the caller redoes the step exactly.  A zero operand needs no case of
+// its own: the other operand comes back unchanged.
+PGA_D bool add_fast(Val& c, uint32_t pm, int ps, uint32_t pg) {
+  int64_t kp = (int64_t)(((uint64_t)(uint32_t)ps << 32) | pm);
+  int64_t kc = (int64_t)(((uint64_t)(uint32_t)c.sc << 32) | c.m);
+  bool pbig = kp > kc;
+  uint32_t mb = pbig ? pm : c.m;
+  uint32_t ms = pbig ? c.m : pm;
+  int sb = pbig ? ps : c.sc, ss = pbig ? c.sc : ps;
+  uint32_t gb = pbig ? pg : c.sg;
+  uint32_t d = min((uint32_t)(sb - ss), 31u);
+  uint32_t al = (ms >> d) | nonzero_bit(__builtin_amdgcn_ubfe(ms, 0, d));
+  const uint32_t sd = pg ^ c.sg;
+  uint32_t x = ((al ^ sd) + mb) - sd;
+  uint32_t lz = min(ffbh_raw(x), 1u << 21);
+  int sn = sb + 1 - (int)lz;
+  uint32_t z = x << (lz & 31u);
+  c.m = round_to_scale((z >> 1) | (z & 1u), sn);
+  c.sc = sn;
+  c.sg = gb;
+  return (uint32_t)(sn - dec::FAST_MIN) <= (uint32_t)(dec::FAST_MAX - dec::FAST_MIN);
+}
+
 // Largest accumulator / product scale for which BK more steps cannot pass FAST_MAX:
 // each step at most adds a product and rounds up by a factor <= 1.25 (a result in the
 // fast range keeps at least one fraction bit), so after BK steps

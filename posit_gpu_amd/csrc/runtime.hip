@@ -36,6 +36,53 @@ struct ScratchSlot {
 };
 static ScratchSlot g_scratch[8];
 
+// Host <-> device copies of a rows x cols column-major block (leading dimension ld on
+// the host, rows on the device) for the synchronous host-pointer entry points.  The
+// columns pass through one pinned staging buffer in a single DMA: a pitched copy of
+// pageable memory moves one column at a time.  Each copy has finished when the call
+// returns, so the one buffer serves every call in turn.
+static std::mutex g_stage_mu;
+static void* g_stage = nullptr;
+static size_t g_stage_bytes = 0;
+
+static hipError_t stage_reserve(size_t bytes) {
+  if (bytes <= g_stage_bytes) return hipSuccess;
+  if (g_stage) (void)hipHostFree(g_stage);
+  g_stage = nullptr;
+  g_stage_bytes = 0;
+  const size_t want = bytes + bytes / 2;
+  hipError_t e = hipHostMalloc(&g_stage, want, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    g_stage = nullptr;
+    return e;
+  }
+  g_stage_bytes = want;
+  return hipSuccess;
+}
+
+hipError_t copy_h2d_block(uint32_t* d, const uint32_t* h, int rows, int cols, int ld, hipStream_t s) {
+  if (rows <= 0 || cols <= 0) return hipSuccess;
+  const size_t colb = (size_t)rows * sizeof(uint32_t), bytes = colb * cols;
+  std::lock_guard<std::mutex> lk(g_stage_mu);
+  hipError_t e = stage_reserve(bytes);
+  if (e != hipSuccess) return e;
+  for (int c = 0; c < cols; ++c) memcpy((char*)g_stage + c * colb, h + (size_t)c * ld, colb);
+  if ((e = hipMemcpyAsync(d, g_stage, bytes, hipMemcpyHostToDevice, s)) != hipSuccess) return e;
+  return hipStreamSynchronize(s);
+}
+
+hipError_t copy_d2h_block(uint32_t* h, const uint32_t* d, int rows, int cols, int ld, hipStream_t s) {
+  if (rows <= 0 || cols <= 0) return hipSuccess;
+  const size_t colb = (size_t)rows * sizeof(uint32_t), bytes = colb * cols;
+  std::lock_guard<std::mutex> lk(g_stage_mu);
+  hipError_t e = stage_reserve(bytes);
+  if (e != hipSuccess) return e;
+  if ((e = hipMemcpyAsync(g_stage, d, bytes, hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
+  if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+  for (int c = 0; c < cols; ++c) memcpy(h + (size_t)c * ld, (const char*)g_stage + c * colb, colb);
+  return hipSuccess;
+}
+
 void* scratch(int slot, size_t bytes, pga_status* st) {
   *st = PGA_SUCCESS;
   ScratchSlot& s = g_scratch[slot];
@@ -355,19 +402,16 @@ pga_status pga_rgemm(char transa, char transb, int m, int n, int k, uint32_t alp
     if (st) return st;
     dB = (uint32_t*)scratch(1, (size_t)brows * bcols * w, &st);
     if (st) return st;
-    PGA_HIP(hipMemcpy2DAsync(dA, arows * w, a, (size_t)lda * w, arows * w, acols,
-                             hipMemcpyHostToDevice, s));
-    PGA_HIP(hipMemcpy2DAsync(dB, brows * w, b, (size_t)ldb * w, brows * w, bcols,
-                             hipMemcpyHostToDevice, s));
+    PGA_HIP(copy_h2d_block(dA, a, arows, acols, lda, s));
+    PGA_HIP(copy_h2d_block(dB, b, brows, bcols, ldb, s));
   }
   uint32_t* dC = (uint32_t*)scratch(2, (size_t)m * n * w, &st);
   if (st) return st;
   if (read_c)
-    PGA_HIP(hipMemcpy2DAsync(dC, m * w, c, (size_t)ldc * w, m * w, n, hipMemcpyHostToDevice, s));
+    PGA_HIP(copy_h2d_block(dC, c, m, n, ldc, s));
   PGA_HIP(rgemm_launch(ta, tb, m, n, k, alpha, dA, arows > 0 ? arows : 1, dB,
                        brows > 0 ? brows : 1, beta, dC, m, s, g_rt.gemm_impl));
-  PGA_HIP(hipMemcpy2DAsync(c, (size_t)ldc * w, dC, m * w, m * w, n, hipMemcpyDeviceToHost, s));
-  PGA_HIP(hipStreamSynchronize(s));
+  PGA_HIP(copy_d2h_block(c, dC, m, n, ldc, s));
   return PGA_SUCCESS;
 }
 

@@ -67,6 +67,26 @@ def test_trsm_left_lower_tall(pga, oracle, diag, special):
         _same(got, want)
 
 
+@pytest.mark.parametrize("diag", ["N", "U"])
+def test_trsm_left_lower_trans_chain(pga, oracle, diag):
+    """Left, lower, transposed: one chain of additions per right-hand side, longer than
+    the kernel's LDS chunk (4096 terms), with zero products, a NaR entry and sums beyond
+    the decoded fast range (a huge right-hand-side entry)."""
+    m, n = 4200, 4
+    A = _tri(oracle, 74, m, False)
+    B = posit_matrix(oracle, 75, m, n)
+    A[4150:4190, 17] = 0  # zero products inside row 17's chain
+    A[m - 5, 100] = NAR   # rows 100 and below-index rows of every column become NaR
+    B[m - 3, 2] = oracle.from_double(1e35)  # column 2: sums beyond the fast range
+    B[9, 3] = 0
+    for alpha in (ONE, 0x3A000000):
+        want = B.copy(order="F")
+        assert oracle.rtrsm("L", "L", "T", diag, m, n, alpha, A, want) == 0
+        got = B.copy(order="F")
+        pga.Rtrsm("L", "L", "T", diag, m, n, alpha, A, m, got, m)
+        _same(got, want)
+
+
 @pytest.mark.parametrize("uplo", ["U", "L"])
 @pytest.mark.parametrize("diag", ["N", "U"])
 def test_trsm_left_wide(pga, oracle, uplo, diag):
