@@ -325,7 +325,7 @@ def bench_lu(pga, info, args):
         from datagen import posit_matrix
 
         cores = min(16, os.cpu_count() or 1)
-        n_cpu = 768
+        n_cpu = 1536
         Ac = posit_matrix(O, 4243, n_cpu, n_cpu)
         t0 = time.perf_counter()
         O.rgetrf(Ac, nb=64, nthreads=cores)
@@ -385,7 +385,7 @@ def bench_cholesky(pga, info, args):
         # the oracle's blocked lower Cholesky (port of Rpotrf_opencl / Rpotrf2) at n_cpu
         O = _oracle()
         cores = min(16, os.cpu_count() or 1)
-        n_cpu = 1024
+        n_cpu = 2048
         Mc = uniform_doubles(4301, n_cpu * n_cpu).reshape(n_cpu, n_cpu)
         Sc = Mc @ Mc.T / n_cpu + np.eye(n_cpu)
         Ac = np.asfortranarray(O.from_double_v(Sc.ravel(order="F")).reshape(n_cpu, n_cpu, order="F"))

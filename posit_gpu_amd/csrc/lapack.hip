@@ -487,30 +487,113 @@ __global__ __launch_bounds__(WG) void k_scale_block(int m, int n, uint32_t alpha
     B[e % m + (size_t)(e / m) * ldb] = mul(alpha, B[e % m + (size_t)(e / m) * ldb]);
 }
 
-// Left, lower, not transposed (Rtrsm.cpp:139-155) in blocks of TRSM_BLOCK rows: each
-// block is solved by the sweep kernel, then its rows update every row below it through
-// the GEMM with zero multipliers skipped.  Each element still receives its terms in
-// ascending k, and a zero b_k still skips its term, so the result is the same bits; the
-// bulk of the work runs on the GEMM's decoded path.
+// The not-transposed left sweep (Rtrsm.cpp:120-155) on a diagonal block of ib <= 64 rows,
+// one wave per right-hand side: lane r holds b_r, the block of A sits in LDS, and each
+// step's b_k reaches every lane by a shuffle, so the steps need no barrier.  Lower: k
+// ascending, rows below k; upper: k descending, rows above k.  A zero b_k skips its step;
+// with nounit b_k is divided by a_kk before its terms.  alpha is applied by the caller.
+constexpr int TRSM_DIAG_MAX = 64;
+
+template <bool UPPER>
+__global__ __launch_bounds__(64) void k_trsm_diag(int ib, int n, int nounit, const uint32_t* __restrict__ A,
+                                                  int lda, uint32_t* __restrict__ B, int ldb, const int* stop) {
+  PGA_KERNEL_VGPR_FLOOR();
+  __shared__ uint32_t sa[TRSM_DIAG_MAX][TRSM_DIAG_MAX + 1];  // sa[r][c] = a(r, c)
+  const int j = blockIdx.x;
+  if (j >= n || stopped(stop)) return;
+  const int r = threadIdx.x;
+  for (int c = 0; c < ib; ++c)
+    if (r < ib) sa[r][c] = A[r + (size_t)c * lda];
+  __syncthreads();
+  uint32_t* bj = B + (size_t)j * ldb;
+  uint32_t b = r < ib ? bj[r] : 0;
+  for (int s = 0; s < ib; ++s) {
+    const int k = UPPER ? ib - 1 - s : s;
+    uint32_t bk = __shfl(b, k);
+    if (bk == 0) continue;  // the same value in every lane
+    if (nounit) bk = div(bk, sa[k][k]);
+    if (UPPER ? r < k : (r > k && r < ib)) b = sub(b, mul(bk, sa[r][k]));
+    if (r == k) b = bk;
+  }
+  if (r < ib) bj[r] = b;
+}
+
+// Rows [0, rows) of B after an upper diagonal block of kb rows is solved:
+// b_i -= b_k * a_ik for the block's k in descending order, a zero b_k skipped
+// (Rtrsm.cpp:127-133), one lane per (row, right-hand side).
+__global__ __launch_bounds__(WG) void k_trsm_upper_update(int rows, int n, int kb, const uint32_t* __restrict__ A,
+                                                          int lda, const uint32_t* __restrict__ Bk,
+                                                          uint32_t* __restrict__ B, int ldb, const int* stop) {
+  PGA_KERNEL_VGPR_FLOOR();
+  const long e = blockIdx.x * (long)WG + threadIdx.x;
+  if (e >= (long)rows * n || stopped(stop)) return;
+  const int i = (int)(e % rows), j = (int)(e / rows);
+  uint32_t b = B[i + (size_t)j * ldb];
+  for (int kk = kb - 1; kk >= 0; --kk) {
+    const uint32_t bk = Bk[kk + (size_t)j * ldb];
+    if (bk != 0) b = sub(b, mul(bk, A[i + (size_t)kk * lda]));
+  }
+  B[i + (size_t)j * ldb] = b;
+}
+
+// Left, not transposed (Rtrsm.cpp:120-155) in blocks of rows: each diagonal block is
+// solved, then its rows update every row still to come -- below it through the GEMM with
+// zero multipliers skipped (lower), above it in descending k (upper).  Each element
+// still receives its terms in the reference's k order, and a zero b_k still skips its
+// term, so the result is the same bits.  Narrow right-hand sides solve their diagonal
+// blocks with k_trsm_diag (64 rows); wide ones with the column-sweep kernel (32 rows).
 constexpr int TRSM_BLOCK = 32;
+
+static bool trsm_narrow(int n) { return (long)n < 8L * runtime().cus; }
+
+static hipError_t trsm_diag(const Ctx& c, bool upper, bool nounit, int ib, int n, const uint32_t* A, int lda,
+                            uint32_t* B, int ldb, const int* stop) {
+  if (!trsm_narrow(n)) return trsm(c, true, upper, false, nounit, ib, n, P_ONE, A, lda, B, ldb, stop);
+  if (upper)
+    hipLaunchKernelGGL(k_trsm_diag<true>, dim3(n), dim3(64), 0, c.s, ib, n, (int)nounit, A, lda, B, ldb, stop);
+  else
+    hipLaunchKernelGGL(k_trsm_diag<false>, dim3(n), dim3(64), 0, c.s, ib, n, (int)nounit, A, lda, B, ldb, stop);
+  return hipGetLastError();
+}
+
+static hipError_t trsm_scale(const Ctx& c, int m, int n, uint32_t alpha, uint32_t* B, int ldb, const int* stop) {
+  if (alpha == P_ONE) return hipSuccess;
+  const long total = (long)m * n;
+  hipLaunchKernelGGL(k_scale_block, dim3((unsigned)std::min<long>((total + WG - 1) / WG, 65536)), dim3(WG), 0, c.s,
+                     m, n, alpha, B, ldb, stop);
+  return hipGetLastError();
+}
 
 static hipError_t trsm_lln_blocked(const Ctx& c, bool nounit, int m, int n, uint32_t alpha, const uint32_t* A,
                                    int lda, uint32_t* B, int ldb, const int* stop) {
-  if (alpha != P_ONE) {
-    const long total = (long)m * n;
-    hipLaunchKernelGGL(k_scale_block, dim3((unsigned)std::min<long>((total + WG - 1) / WG, 65536)), dim3(WG), 0,
-                       c.s, m, n, alpha, B, ldb, stop);
-    PGA_TRY_E(hipGetLastError());
-  }
+  PGA_TRY_E(trsm_scale(c, m, n, alpha, B, ldb, stop));
   GemmOpts o;
   o.skip_zero_b = true;
   o.stop = stop;
-  for (int i0 = 0; i0 < m; i0 += TRSM_BLOCK) {
-    const int ib = std::min(TRSM_BLOCK, m - i0);
-    PGA_TRY_E(trsm(c, true, false, false, nounit, ib, n, P_ONE, A + i0 + (size_t)i0 * lda, lda, B + i0, ldb, stop));
+  const int tb = trsm_narrow(n) ? TRSM_DIAG_MAX : TRSM_BLOCK;
+  for (int i0 = 0; i0 < m; i0 += tb) {
+    const int ib = std::min(tb, m - i0);
+    PGA_TRY_E(trsm_diag(c, false, nounit, ib, n, A + i0 + (size_t)i0 * lda, lda, B + i0, ldb, stop));
     if (i0 + ib < m)
       PGA_TRY_E(rgemm_launch(false, false, m - i0 - ib, n, ib, P_MONE, A + i0 + ib + (size_t)i0 * lda, lda, B + i0,
                              ldb, P_ONE, B + i0 + ib, ldb, c.s, c.impl, o));
+  }
+  return hipSuccess;
+}
+
+// Upper form, narrow right-hand sides only (the blocks go from the bottom up).
+static hipError_t trsm_lun_blocked(const Ctx& c, bool nounit, int m, int n, uint32_t alpha, const uint32_t* A,
+                                   int lda, uint32_t* B, int ldb, const int* stop) {
+  PGA_TRY_E(trsm_scale(c, m, n, alpha, B, ldb, stop));
+  for (int i1 = m; i1 > 0; i1 -= TRSM_DIAG_MAX) {
+    const int i0 = std::max(0, i1 - TRSM_DIAG_MAX), ib = i1 - i0;
+    PGA_TRY_E(trsm_diag(c, true, nounit, ib, n, A + i0 + (size_t)i0 * lda, lda, B + i0, ldb, stop));
+    if (i0 > 0) {
+      const long total = (long)i0 * n;
+      hipLaunchKernelGGL(k_trsm_upper_update, dim3((unsigned)((total + WG - 1) / WG)), dim3(WG), 0, c.s, i0, n, ib,
+                         A + (size_t)i0 * lda, lda, B + i0, B, ldb, stop);
+      PGA_TRY_E(hipGetLastError());
+    }
   }
   return hipSuccess;
 }
@@ -521,6 +604,8 @@ static hipError_t trsm(const Ctx& c, bool left, bool upper, bool trans, bool nou
   if (m == 0 || n == 0) return hipSuccess;
   if (left && !upper && !trans && alpha != 0 && m > 2 * TRSM_BLOCK)
     return trsm_lln_blocked(c, nounit, m, n, alpha, A, lda, B, ldb, stop);
+  if (left && upper && !trans && alpha != 0 && m > TRSM_DIAG_MAX && trsm_narrow(n))
+    return trsm_lun_blocked(c, nounit, m, n, alpha, A, lda, B, ldb, stop);
   if (alpha == 0) {
     hipLaunchKernelGGL(k_zero, dim3(blocks_for(m)), dim3(WG), 0, c.s, m, n, B, ldb, stop);
     return hipGetLastError();

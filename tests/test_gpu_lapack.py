@@ -68,6 +68,28 @@ def test_trsm_left_lower_tall(pga, oracle, diag, special):
 
 
 @pytest.mark.parametrize("diag", ["N", "U"])
+@pytest.mark.parametrize("special", [False, True])
+def test_trsm_left_upper_tall(pga, oracle, diag, special):
+    """Left upper solves with many rows (64-row diagonal blocks from the bottom up, then a
+    descending-k update of the rows above each block)."""
+    m, n = 200, 7
+    A = _tri(oracle, 76, m, True)
+    B = posit_matrix(oracle, 77, m, n)
+    B[150, :] = 0  # zero b_k skip their step
+    B[70, ::2] = 0
+    if special:  # NaR in U under skipped and unskipped multipliers
+        A[40, 150] = NAR
+        A[10, 190] = NAR
+        A[3, 64] = NAR
+    for alpha in (ONE, 0x3A000000):
+        want = B.copy(order="F")
+        assert oracle.rtrsm("L", "U", "N", diag, m, n, alpha, A, want) == 0
+        got = B.copy(order="F")
+        pga.Rtrsm("L", "U", "N", diag, m, n, alpha, A, m, got, m)
+        _same(got, want)
+
+
+@pytest.mark.parametrize("diag", ["N", "U"])
 def test_trsm_left_lower_trans_chain(pga, oracle, diag):
     """Left, lower, transposed: one chain of additions per right-hand side, longer than
     the kernel's LDS chunk (4096 terms), with zero products, a NaR entry and sums beyond
