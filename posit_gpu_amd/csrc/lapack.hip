@@ -407,13 +407,26 @@ __global__ __launch_bounds__(WG) void k_getrf2_row(uint32_t* a, int* ipiv, int* 
 
 constexpr int LEAF_TRSM_N1 = 16, LEAF_TRSM_N2 = 32;
 
-// One Rgetrf2 leaf (and a small node's Rtrsm), lu_leaf.hpp
-__global__ __launch_bounds__(WG) void k_getrf2_leaf(int m_root, int w_root, uint32_t* root, int lda, int j,
-                                                    int* ipiv, int* info, int row_off, int col_off, int ts,
-                                                    int tm, int te) {
+// One Rgetrf2 leaf (and a small node's Rtrsm), lu_leaf.hpp.  The leaves are latency
+// bound: 1024 lanes give each lane a quarter of the column that 256 would.
+constexpr int LEAF_NT = 1024;
+
+__global__ __launch_bounds__(LEAF_NT) void k_getrf2_leaf(int m_root, int w_root, uint32_t* root, int lda, int j,
+                                                         int* ipiv, int* info, int row_off, int col_off, int ts,
+                                                         int tm, int te) {
   PGA_KERNEL_VGPR_FLOOR();
-  leaf::getrf2_leaf<LEAF_TRSM_N1, LEAF_TRSM_N2>(m_root, w_root, root, lda, j, ipiv, info, row_off, col_off, ts, tm,
-                                                te);
+  leaf::getrf2_leaf<LEAF_TRSM_N1, LEAF_TRSM_N2, LEAF_NT>(m_root, w_root, root, lda, j, ipiv, info, row_off, col_off,
+                                                         ts, tm, te);
+}
+
+// A whole two-column node of the recursion in one launch (leaf, 1 x 1 Rgemm, leaf, and a
+// pending small node's Rtrsm), lu_leaf.hpp
+__global__ __launch_bounds__(LEAF_NT) void k_getrf2_pair(int m_root, int w_root, uint32_t* root, int lda, int j,
+                                                         int* ipiv, int* info, int row_off, int col_off, int ts,
+                                                         int tm, int te) {
+  PGA_KERNEL_VGPR_FLOOR();
+  leaf::getrf2_pair<LEAF_TRSM_N1, LEAF_TRSM_N2, LEAF_NT>(m_root, w_root, root, lda, j, ipiv, info, row_off, col_off,
+                                                         ts, tm, te);
 }
 
 // ---------------------------------------------------------------------------------
@@ -689,7 +702,12 @@ static hipError_t getrf2_node(const Ctx& c, int m, int n, uint32_t* a, int lda, 
     return hipSuccess;
   }
   if (cols == 1) {
-    hipLaunchKernelGGL(k_getrf2_leaf, dim3(1), dim3(WG), 0, c.s, m, n, a, lda, s, ipiv, info, row_off, col_off,
+    hipLaunchKernelGGL(k_getrf2_leaf, dim3(1), dim3(LEAF_NT), 0, c.s, m, n, a, lda, s, ipiv, info, row_off, col_off,
+                       ps, pm, pe);
+    return hipGetLastError();
+  }
+  if (cols == 2) {  // rows >= 2 here: the node splits 1 + 1
+    hipLaunchKernelGGL(k_getrf2_pair, dim3(1), dim3(LEAF_NT), 0, c.s, m, n, a, lda, s, ipiv, info, row_off, col_off,
                        ps, pm, pe);
     return hipGetLastError();
   }
