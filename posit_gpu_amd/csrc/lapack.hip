@@ -36,6 +36,7 @@ __global__ __launch_bounds__(WG) void k_trsm_left(int m, int n, uint32_t alpha, 
                                                   uint32_t* __restrict__ B, int ldb, const int* stop) {
   static_assert(UPPER || !TRANS, "the lower transposed form is k_trsm_lltn");
   PGA_KERNEL_VGPR_FLOOR();
+  PGA_URGENT();
   const int j = blockIdx.x;
   if (j >= n || stopped(stop)) return;
   uint32_t* b = B + (size_t)j * ldb;
@@ -177,6 +178,7 @@ __global__ __launch_bounds__(WG) void k_trsm_left_cols(int m, int n, uint32_t al
                                                        uint32_t* __restrict__ B, int ldb, int cpw,
                                                        const int* stop) {
   PGA_KERNEL_VGPR_FLOOR();
+  PGA_URGENT();
   const int c0 = blockIdx.x * cpw;
   if (c0 >= n || stopped(stop)) return;
   const int C = min(cpw, n - c0);
@@ -216,6 +218,7 @@ __global__ __launch_bounds__(WG) void k_trsm_right(int m, int n, uint32_t alpha,
                                                    uint32_t* __restrict__ B, int ldb,
                                                    const int* stop) {
   PGA_KERNEL_VGPR_FLOOR();
+  PGA_URGENT();
   const int i = blockIdx.x * WG + threadIdx.x;
   if (i >= m || stopped(stop)) return;
   auto a = [&](int r, int c) { return A[r + (size_t)c * lda]; };
@@ -281,6 +284,7 @@ __global__ __launch_bounds__(WG) void k_trsm_right_sweep(int m, int n, uint32_t 
                                                          uint32_t* __restrict__ B, int ldb, int rows,
                                                          const int* stop) {
   PGA_KERNEL_VGPR_FLOOR();
+  PGA_URGENT();
   const int r0 = blockIdx.x * rows;
   if (r0 >= m || stopped(stop)) return;
   const int R = min(rows, m - r0);
@@ -366,6 +370,7 @@ __global__ __launch_bounds__(WG) void k_iamax(int n, const uint32_t* x, int incx
 __global__ __launch_bounds__(WG) void k_laswp(int n, uint32_t* a, int lda, int k1, int k2,
                                               const int* ipiv, int incx, int ipoff, const int* stop) {
   PGA_KERNEL_VGPR_FLOOR();
+  PGA_URGENT();
   const int col = blockIdx.x * WG + threadIdx.x;
   if (col >= n || stopped(stop)) return;
   int ix0, i1, i2, inc;
@@ -400,6 +405,7 @@ __global__ __launch_bounds__(WG) void k_laswp(int n, uint32_t* a, int lda, int k
 __global__ __launch_bounds__(WG) void k_getrf2_row(uint32_t* a, int* ipiv, int* info, int row_off,
                                                    int col_off) {
   PGA_KERNEL_VGPR_FLOOR();
+  PGA_URGENT();
   if (threadIdx.x != 0) return;
   ipiv[0] = row_off + 1;
   if (a[0] == 0 && *info == 0) *info = col_off + 1;
@@ -415,6 +421,7 @@ __global__ __launch_bounds__(LEAF_NT) void k_getrf2_leaf(int m_root, int w_root,
                                                          int* ipiv, int* info, int row_off, int col_off, int ts,
                                                          int tm, int te) {
   PGA_KERNEL_VGPR_FLOOR();
+  PGA_URGENT();
   leaf::getrf2_leaf<LEAF_TRSM_N1, LEAF_TRSM_N2, LEAF_NT>(m_root, w_root, root, lda, j, ipiv, info, row_off, col_off,
                                                          ts, tm, te);
 }
@@ -425,6 +432,7 @@ __global__ __launch_bounds__(LEAF_NT) void k_getrf2_pair(int m_root, int w_root,
                                                          int* ipiv, int* info, int row_off, int col_off, int ts,
                                                          int tm, int te) {
   PGA_KERNEL_VGPR_FLOOR();
+  PGA_URGENT();
   leaf::getrf2_pair<LEAF_TRSM_N1, LEAF_TRSM_N2, LEAF_NT>(m_root, w_root, root, lda, j, ipiv, info, row_off, col_off,
                                                          ts, tm, te);
 }
@@ -434,6 +442,7 @@ __global__ __launch_bounds__(LEAF_NT) void k_getrf2_pair(int m_root, int w_root,
 // ---------------------------------------------------------------------------------
 __global__ void k_potrf2_leaf(uint32_t* a, int* info, int col_off) {
   PGA_KERNEL_VGPR_FLOOR();
+  PGA_URGENT();
   if (threadIdx.x != 0 || *info) return;
   uint32_t v = a[0];
   if (le(v, 0)) {
@@ -454,6 +463,7 @@ constexpr int POTRF2_ONE_WG_MAX = 64;
 template <bool UPPER>
 __global__ __launch_bounds__(WG, 2) void k_potrf2_wg(int n, uint32_t* a, int lda, int* info, int col_off) {
   PGA_KERNEL_VGPR_FLOOR();
+  PGA_URGENT();
   __shared__ wg::Shared sh;
   __shared__ uint32_t blk[POTRF2_ONE_WG_MAX * POTRF2_ONE_WG_MAX];
   if (*info) return;  // every lane reads the same flag
@@ -511,6 +521,7 @@ template <bool UPPER>
 __global__ __launch_bounds__(64) void k_trsm_diag(int ib, int n, int nounit, const uint32_t* __restrict__ A,
                                                   int lda, uint32_t* __restrict__ B, int ldb, const int* stop) {
   PGA_KERNEL_VGPR_FLOOR();
+  PGA_URGENT();
   __shared__ uint32_t sa[TRSM_DIAG_MAX][TRSM_DIAG_MAX + 1];  // sa[r][c] = a(r, c)
   const int j = blockIdx.x;
   if (j >= n || stopped(stop)) return;
@@ -538,6 +549,7 @@ __global__ __launch_bounds__(WG) void k_trsm_upper_update(int rows, int n, int k
                                                           int lda, const uint32_t* __restrict__ Bk,
                                                           uint32_t* __restrict__ B, int ldb, const int* stop) {
   PGA_KERNEL_VGPR_FLOOR();
+  PGA_URGENT();
   const long e = blockIdx.x * (long)WG + threadIdx.x;
   if (e >= (long)rows * n || stopped(stop)) return;
   const int i = (int)(e % rows), j = (int)(e / rows);
@@ -721,8 +733,10 @@ static hipError_t getrf2_node(const Ctx& c, int m, int n, uint32_t* a, int lda, 
     PGA_TRY(getrf2_node(c, m, n, a, lda, ipiv, info, row_off, col_off, s, mid, 0, 0, 0));
     PGA_TRY(trsm(c, true, false, false, false, n1, n2, P_ONE, a11, lda, a12, lda, nullptr));
   }
+  GemmOpts o;
+  o.urgent = true;
   PGA_TRY(rgemm_launch(false, false, rows - n1, n2, n1, P_MONE, a11 + n1, lda, a12, lda, P_ONE, a12 + n1, lda,
-                       c.s, c.impl));
+                       c.s, c.impl, o));
   return getrf2_node(c, m, n, a, lda, ipiv, info, row_off, col_off, mid, e, ps, pm, pe);
 }
 
@@ -764,15 +778,17 @@ static void aux_give(const Aux& x) {
 // rows below.  Columns are independent here, so any split of the right-hand columns
 // gives the same bits.
 static hipError_t getrf_update(const Ctx& c, int m, uint32_t* a, int lda, const int* ipiv, int j, int jb, int c0,
-                               int w) {
+                               int w, bool urgent) {
   if (w <= 0) return hipSuccess;
   uint32_t* ajj = a + j + (size_t)j * lda;
   uint32_t* a12 = a + j + (size_t)c0 * lda;
   PGA_TRY(laswp(c, w, a + (size_t)c0 * lda, lda, j + 1, j + jb, ipiv, 1, 0, nullptr));
   PGA_TRY(trsm(c, true, false, false, false, jb, w, P_ONE, ajj, lda, a12, lda, nullptr));
+  GemmOpts o;
+  o.urgent = urgent;
   if (j + jb < m)
     PGA_TRY(rgemm_launch(false, false, m - j - jb, w, jb, P_MONE, ajj + jb, lda, a12, lda, P_ONE, a12 + jb, lda,
-                         c.s, c.impl));
+                         c.s, c.impl, o));
   return hipSuccess;
 }
 
@@ -799,11 +815,11 @@ static hipError_t getrf_lookahead(const Ctx& c, int m, int n, uint32_t* a, int l
     if ((e = hipEventRecord(x.join, c.s)) != hipSuccess) break;
     if ((e = hipStreamWaitEvent(x.s, x.join, 0)) != hipSuccess) break;
     if (j > 0 && (e = laswp(cx, j, a, lda, j + 1, j + jb, ipiv, 1, 0, nullptr)) != hipSuccess) break;
-    if ((e = getrf_update(cx, m, a, lda, ipiv, j, jb, c1, n - c1)) != hipSuccess) break;
+    if ((e = getrf_update(cx, m, a, lda, ipiv, j, jb, c1, n - c1, false)) != hipSuccess) break;
     if (recorded && (e = hipStreamWaitEvent(c.s, x.done, 0)) != hipSuccess) break;
     if ((e = hipEventRecord(x.done, x.s)) != hipSuccess) break;
     recorded = true;
-    if ((e = getrf_update(c, m, a, lda, ipiv, j, jb, c0, w1)) != hipSuccess) break;
+    if ((e = getrf_update(c, m, a, lda, ipiv, j, jb, c0, w1, true)) != hipSuccess) break;
     if (c0 < mn) e = getrf2(c, m - c0, std::min(nb, mn - c0), a + c0 + (size_t)c0 * lda, lda, ipiv + c0, info, c0, c0);
   }
   if (recorded) {
@@ -978,8 +994,10 @@ static hipError_t potrf_overlap(const Ctx& c, bool upper, int n, uint32_t* a, in
 // the lower trapezoid, tiles numbered column by column.
 template <int T>
 __global__ __launch_bounds__(WG) void k_potrf_trail(int m, int ncols, int kb, const uint32_t* __restrict__ L,
-                                                    int lda, uint32_t* __restrict__ C, int nb, const int* stop) {
+                                                    int lda, uint32_t* __restrict__ C, int nb, int urgent,
+                                                    const int* stop) {
   PGA_KERNEL_VGPR_FLOOR();
+  if (urgent) PGA_URGENT();
   __shared__ uint32_t lds[gemm::TileLds<T, T, 16>::words];
   if (stopped(stop)) return;
   const int tm = (m + T - 1) / T, tc = (ncols + T - 1) / T, t = blockIdx.x;
@@ -1029,16 +1047,16 @@ static int trail_tile(long m, long ncols, int nb) {
 }
 
 static hipError_t potrf_trail(hipStream_t s, int m, int ncols, int kb, const uint32_t* L, int lda, uint32_t* C,
-                              int nb, const int* stop) {
+                              int nb, bool urgent, const int* stop) {
   if (m <= 0 || ncols <= 0 || kb <= 0) return hipSuccess;
   const int T = trail_tile(m, ncols, nb);
   const long tm = (m + T - 1) / T, tc = (ncols + T - 1) / T;
   const unsigned grid = (unsigned)(tc * tm - tc * (tc - 1) / 2);
   if (T == 64)
-    hipLaunchKernelGGL(k_potrf_trail<64>, dim3(grid), dim3(WG), 0, s, m, ncols, kb, L, lda, C, nb, stop);
+    hipLaunchKernelGGL(k_potrf_trail<64>, dim3(grid), dim3(WG), 0, s, m, ncols, kb, L, lda, C, nb, (int)urgent, stop);
   else if (T == 32)
-    hipLaunchKernelGGL(k_potrf_trail<32>, dim3(grid), dim3(WG), 0, s, m, ncols, kb, L, lda, C, nb, stop);
-  else hipLaunchKernelGGL(k_potrf_trail<16>, dim3(grid), dim3(WG), 0, s, m, ncols, kb, L, lda, C, nb, stop);
+    hipLaunchKernelGGL(k_potrf_trail<32>, dim3(grid), dim3(WG), 0, s, m, ncols, kb, L, lda, C, nb, (int)urgent, stop);
+  else hipLaunchKernelGGL(k_potrf_trail<16>, dim3(grid), dim3(WG), 0, s, m, ncols, kb, L, lda, C, nb, (int)urgent, stop);
   return hipGetLastError();
 }
 
@@ -1072,7 +1090,7 @@ static hipError_t potrf_right_lower(const Ctx& c, int n, uint32_t* a, int lda, i
     if (c1 < n) {
       if ((e = hipEventRecord(x.join, c.s)) != hipSuccess) break;
       if ((e = hipStreamWaitEvent(x.s, x.join, 0)) != hipSuccess) break;
-      if ((e = potrf_trail(x.s, n - c1, n - c1, nb, A(c1, j), lda, A(c1, c1), nb, info)) != hipSuccess) break;
+      if ((e = potrf_trail(x.s, n - c1, n - c1, nb, A(c1, j), lda, A(c1, c1), nb, false, info)) != hipSuccess) break;
     }
     // block column j+1 has had every earlier block's terms once the previous aux launch ends
     if (recorded && (e = hipStreamWaitEvent(c.s, x.done, 0)) != hipSuccess) break;
@@ -1080,7 +1098,7 @@ static hipError_t potrf_right_lower(const Ctx& c, int n, uint32_t* a, int lda, i
       if ((e = hipEventRecord(x.done, x.s)) != hipSuccess) break;
       recorded = true;
     }
-    if ((e = potrf_trail(c.s, n - c0, nb1, nb, A(c0, j), lda, A(c0, c0), nb, info)) != hipSuccess) break;
+    if ((e = potrf_trail(c.s, n - c0, nb1, nb, A(c0, j), lda, A(c0, c0), nb, true, info)) != hipSuccess) break;
     if ((e = potrf2(c, false, nb1, A(c0, c0), lda, info, c0)) != hipSuccess) break;
     if (c1 < n) e = trsm(c, false, false, true, true, n - c1, nb1, P_ONE, A(c0, c0), lda, A(c1, c0), lda, info);
   }

@@ -16,6 +16,7 @@ struct GemmOpts {
   int tri = 0;               // 0 full C, 1 store i >= j only, 2 store i <= j only
   const int* stop = nullptr; // device flag: when nonzero the launch does nothing
   int tile = 0;              // 0 automatic, else 64 / 32 / 16
+  bool urgent = false;       // a step on a factorization's critical path (PGA_URGENT)
 };
 
 // Device-pointer GEMM launch (rgemm.hip); arguments already validated.
@@ -106,3 +107,9 @@ hipError_t copy_d2h_block(uint32_t* h, const uint32_t* d, int rows, int cols, in
 // 32 or more VGPRs allocated computed correct values in every run.  Occupancy is not
 // affected (32 VGPRs still allow the maximum number of waves per SIMD).
 #define PGA_KERNEL_VGPR_FLOOR() asm volatile("" ::: "v31")
+
+// Raises the issue priority of the calling wave (s_setprio 3): latency-bound steps on a
+// factorization's critical path (panel leaves, diagonal blocks, their small updates)
+// then win the SIMD's instruction arbitration against the large trailing-update GEMMs
+// that a second stream runs beside them, which fill the cycles these waves leave idle.
+#define PGA_URGENT() __builtin_amdgcn_s_setprio(3)

@@ -18,8 +18,9 @@ template <int BM, int BN, int BK, int TM, int TN, bool TA, bool TB, bool SKIP0>
 __global__ __launch_bounds__((BM / TM) * (BN / TN)) void rgemm_tiled(
     int m, int n, int k, uint32_t alpha, const uint32_t* __restrict__ A, int lda,
     const uint32_t* __restrict__ B, int ldb, uint32_t beta, uint32_t* __restrict__ C, int ldc,
-    int tiles_m, int tri, const int* __restrict__ stop) {
+    int tiles_m, int tri, int urgent, const int* __restrict__ stop) {
   PGA_KERNEL_VGPR_FLOOR();
+  if (urgent) PGA_URGENT();
   // tiles in column-major order of the tile grid: consecutive workgroups share B slices
   const int bid = blockIdx.x;
   const int i0 = (bid % tiles_m) * BM;
@@ -122,14 +123,14 @@ __global__ __launch_bounds__(256) void scale_c(int m, int n, uint32_t beta, uint
 template <int BM, int TM, bool TA, bool TB, bool SKIP0>
 static hipError_t launch_tiled_cfg(int m, int n, int k, uint32_t alpha, const uint32_t* A, int lda,
                                    const uint32_t* B, int ldb, uint32_t beta, uint32_t* C, int ldc,
-                                   hipStream_t s, int tri, const int* stop) {
+                                   hipStream_t s, int tri, int urgent, const int* stop) {
   constexpr int BN = BM, TN = TM, BK = 16;
   constexpr int NT = (BM / TM) * (BN / TN);
   int tm = (m + BM - 1) / BM, tn = (n + BN - 1) / BN;
   long long blocks = (long long)tm * tn;
   if (blocks > 0x7FFFFFFF) return hipErrorInvalidValue;
   hipLaunchKernelGGL((rgemm_tiled<BM, BN, BK, TM, TN, TA, TB, SKIP0>), dim3((unsigned)blocks), dim3(NT),
-                     0, s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tm, tri, stop);
+                     0, s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tm, tri, urgent, stop);
   return hipGetLastError();
 }
 
@@ -145,12 +146,12 @@ static int pick_tile(int m, int n, int cus) {
 template <bool TA, bool TB, bool SKIP0>
 static hipError_t launch_tiled(int m, int n, int k, uint32_t alpha, const uint32_t* A, int lda,
                                const uint32_t* B, int ldb, uint32_t beta, uint32_t* C, int ldc,
-                               hipStream_t s, int tri, const int* stop, int tile) {
+                               hipStream_t s, int tri, int urgent, const int* stop, int tile) {
   if (tile == 64)
-    return launch_tiled_cfg<64, 4, TA, TB, SKIP0>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, tri, stop);
+    return launch_tiled_cfg<64, 4, TA, TB, SKIP0>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, tri, urgent, stop);
   if (tile == 32)
-    return launch_tiled_cfg<32, 2, TA, TB, SKIP0>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, tri, stop);
-  return launch_tiled_cfg<16, 1, TA, TB, SKIP0>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, tri, stop);
+    return launch_tiled_cfg<32, 2, TA, TB, SKIP0>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, tri, urgent, stop);
+  return launch_tiled_cfg<16, 1, TA, TB, SKIP0>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, tri, urgent, stop);
 }
 
 template <bool TA, bool TB, bool SKIP0>
@@ -167,17 +168,17 @@ static hipError_t launch_exact(int m, int n, int k, uint32_t alpha, const uint32
 template <bool SKIP0>
 static hipError_t dispatch(bool ta, bool tb, int m, int n, int k, uint32_t alpha, const uint32_t* A,
                            int lda, const uint32_t* B, int ldb, uint32_t beta, uint32_t* C, int ldc,
-                           hipStream_t s, GemmImpl impl, int tri, const int* stop, int tile) {
+                           hipStream_t s, GemmImpl impl, int tri, int urgent, const int* stop, int tile) {
   if (impl == GemmImpl::Exact) {
     if (!ta && !tb) return launch_exact<false, false, SKIP0>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, tri, stop);
     if (!ta && tb) return launch_exact<false, true, SKIP0>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, tri, stop);
     if (ta && !tb) return launch_exact<true, false, SKIP0>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, tri, stop);
     return launch_exact<true, true, SKIP0>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, tri, stop);
   }
-  if (!ta && !tb) return launch_tiled<false, false, SKIP0>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, tri, stop, tile);
-  if (!ta && tb) return launch_tiled<false, true, SKIP0>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, tri, stop, tile);
-  if (ta && !tb) return launch_tiled<true, false, SKIP0>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, tri, stop, tile);
-  return launch_tiled<true, true, SKIP0>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, tri, stop, tile);
+  if (!ta && !tb) return launch_tiled<false, false, SKIP0>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, tri, urgent, stop, tile);
+  if (!ta && tb) return launch_tiled<false, true, SKIP0>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, tri, urgent, stop, tile);
+  if (ta && !tb) return launch_tiled<true, false, SKIP0>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, tri, urgent, stop, tile);
+  return launch_tiled<true, true, SKIP0>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, tri, urgent, stop, tile);
 }
 
 template <int BM, int TM, bool TA, bool TB>
@@ -228,8 +229,8 @@ hipError_t rgemm_launch(bool ta, bool tb, int m, int n, int k, uint32_t alpha, c
   }
   int tile = o.tile ? o.tile : pick_tile(m, n, runtime().cus);
   if (o.skip_zero_b)
-    return dispatch<true>(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, impl, o.tri, o.stop, tile);
-  return dispatch<false>(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, impl, o.tri, o.stop, tile);
+    return dispatch<true>(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, impl, o.tri, o.urgent, o.stop, tile);
+  return dispatch<false>(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, impl, o.tri, o.urgent, o.stop, tile);
 }
 
 }  // namespace pga
