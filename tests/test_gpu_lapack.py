@@ -361,7 +361,7 @@ def test_cholesky_1024_device(pga, oracle):
 @pytest.mark.parametrize("uplo,n,nb,bad", [("L", 1100, 64, 0), ("L", 1100, 64, 700), ("U", 1100, 64, 0),
                                            ("U", 1100, 64, 700), ("L", 1300, 128, 0), ("L", 1300, 128, 5),
                                            ("L", 1300, 128, 1290), ("L", 1111, 48, 1063), ("L", 1040, 80, 0),
-                                           ("L", 1040, 80, 959)])
+                                           ("L", 1040, 80, 959), ("L", 1030, 50, 0), ("L", 1030, 50, 777)])
 def test_cholesky_panel_overlap(pga, oracle, uplo, n, nb, bad):
     """Orders from 1024 run on two streams: the upper factor with each block's panel GEMM
     beside the block's Rsyrk / Rpotrf2, the lower one right-looking with one block of
