@@ -414,8 +414,11 @@ __global__ __launch_bounds__(WG) void k_getrf2_row(uint32_t* a, int* ipiv, int* 
 constexpr int LEAF_TRSM_N1 = 16, LEAF_TRSM_N2 = 32;
 
 // One Rgetrf2 leaf (and a small node's Rtrsm), lu_leaf.hpp.  The leaves are latency
-// bound: 1024 lanes give each lane a quarter of the column that 256 would.
-constexpr int LEAF_NT = 1024;
+// bound.  512 lanes (two waves per SIMD) measured fastest beside the concurrent trailing
+// GEMM: each lane has half the column that 256 would give it, and the workgroup still fits
+// in the registers that the GEMM's waves leave free on a CU (1024 lanes often do not, and
+// then wait for a CU to drain).
+constexpr int LEAF_NT = 512;
 
 __global__ __launch_bounds__(LEAF_NT) void k_getrf2_leaf(int m_root, int w_root, uint32_t* root, int lda, int j,
                                                          int* ipiv, int* info, int row_off, int col_off, int ts,
