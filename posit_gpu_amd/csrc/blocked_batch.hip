@@ -4,10 +4,11 @@
 // Every problem follows the blocking of the per-call driver it mirrors
 // (Rgetrf_opencl.cpp:93-146, Rpotrf_opencl.cpp:86-160) and all problems advance together,
 // one column block per step.  A step is at most five launches for the whole batch:
-//   1. the Cholesky diagonal-block SYRKs (one persistent GEMM launch over all their tiles),
-//   2. the panels: one workgroup per problem runs Rgetrf2 on its column block (rows
-//      interchanged within the block as each pivot is chosen) or Rpotrf2 on its diagonal
-//      block (wg_factor.hpp, the code of the one-workgroup task kernel),
+//   1. the Cholesky diagonal-block SYRKs (one GEMM launch over all their tiles),
+//   2. the panels: Rpotrf2 on each Cholesky diagonal block by one workgroup (wg_factor.hpp,
+//      the code of the one-workgroup task kernel); the LU column blocks walked through the
+//      Rgetrf2 recursion node by node for all problems at once (two-column nodes in one
+//      launch, lu_leaf.hpp; larger nodes' GEMMs as one launch over every problem's tiles),
 //   3. the LU row-block triangular solves (U12 := L11^-1 A12), a few columns per
 //      workgroup, each column first taking the block's row interchanges (Rlaswp),
 //   4. the trailing GEMMs of every problem (LU A22 -= A21 U12; Cholesky panel updates),
@@ -18,11 +19,13 @@
 // receives the reference routines' operations in their order (the GEMM tiles are the
 // plain kernel's), so every factor, pivot, info code and solution equals the per-call
 // result bit for bit.  A failing Cholesky block sets info, which stops every later step
-// of that problem exactly where the reference stops.
+// of that problem exactly where the reference stops.  The plan deals its tasks into
+// groups that run the sequence side by side on their own streams (blocked_plan_launch).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "lu_leaf.hpp"
@@ -59,7 +62,7 @@ __device__ __forceinline__ bool is_lu(int kind) {
 }
 
 // info := 0 for every task (the per-call drivers start that way; a solve reports 0), and
-// the persistent-launch counters of the whole plan := 0
+// the tile counters of the group's launches := 0
 __global__ void k_bb_init(const FactorTask* __restrict__ tasks, int ntasks, int* __restrict__ counters,
                           int ncounters) {
   PGA_KERNEL_VGPR_FLOOR();
@@ -121,22 +124,25 @@ __global__ __launch_bounds__(NT) void k_bb_trsm(const FactorTask* __restrict__ t
   else wg::trsm_rltn(it.cnt, jb, d, lda, d + jb + it.off, lda);
 }
 
-// LU panels of RECUR_W columns, all problems at once: leaf jr of every problem's panel
-// (iamax, interchange within the panel, scaling), then, when jr ends the left half of a
-// recursion node [ts, te) split at tm, that node's Rtrsm in LDS (lu_leaf.hpp; the
-// per-call driver runs the same leaf one problem at a time).
-__global__ __launch_bounds__(NT) void k_bb_leaf(const FactorTask* __restrict__ tasks, const Item* __restrict__ items,
-                                                int jr, int ts, int tm, int te) {
+// LU panels of RECUR_W columns, all problems at once, walked node by node in the
+// reference's depth-first order (the per-call driver runs the same steps one problem at a
+// time).
+// Two-column nodes [jr, jr+2) of every problem's panel: leaf jr, the node's 1 x 1 Rgemm,
+// leaf jr+1, then the Rtrsm of the node whose left half ends at jr+1 (lu_leaf.hpp).
+constexpr int PAIR_NT = 512;
+
+__global__ __launch_bounds__(PAIR_NT) void k_bb_pair(const FactorTask* __restrict__ tasks,
+                                                     const Item* __restrict__ items, int jr, int ts, int tm, int te) {
   PGA_KERNEL_VGPR_FLOOR();
   const Item it = items[blockIdx.x];
   const FactorTask& t = tasks[it.task];
   uint32_t* root = t.a + it.j + (size_t)it.j * t.lda;
-  leaf::getrf2_leaf<RECUR_W / 2, RECUR_W / 2>(t.n - it.j, it.jb, root, t.lda, jr, t.ipiv + it.j, t.info, it.j, it.j,
-                                              ts, tm, te);
+  leaf::getrf2_pair<RECUR_W / 2, RECUR_W / 2, PAIR_NT>(t.n - it.j, it.jb, root, t.lda, jr, t.ipiv + it.j, t.info,
+                                                       it.j, it.j, ts, tm, te);
 }
 
 // The node's Rgemm (A22 -= A21 A12, Rgetrf2.cpp:155-159) of every problem's panel in one
-// persistent launch of T x T tiles.
+// launch of T x T tiles (workgroups take tiles from a counter).
 template <int T>
 __global__ __launch_bounds__(NT) void k_bb_node_gemm(const FactorTask* __restrict__ tasks,
                                                      const Item* __restrict__ items,
@@ -191,7 +197,7 @@ __global__ __launch_bounds__(NT) void k_bb_laswp_left(const FactorTask* __restri
   }
 }
 
-// Persistent GEMM over the tiles of many problems: T x T tiles (64: 4 x 4 elements per
+// GEMM over the tiles of many problems, workgroups taking tiles from a counter: T x T tiles (64: 4 x 4 elements per
 // lane; 32: 2 x 2; 16: 1 x 1), the size chosen per launch so that the tiles fill the
 // device (a step of a few problems has few large tiles; K is never split).
 template <int T, bool TA, bool TB, bool SKIP0>
@@ -233,8 +239,11 @@ __global__ __launch_bounds__(NT) void k_bb_gemm(const Gemm* __restrict__ probs, 
 
 // Solves, one workgroup per right-hand side, with the column staged in LDS:
 // Rgetrs 'N' (Rlaswp, L, U), Rpotrs upper (U^T, U) or lower (L, L^T).
-__global__ __launch_bounds__(NT) void k_bb_solve(const FactorTask* __restrict__ tasks,
-                                                 const Item* __restrict__ items) {
+// 1024 lanes: each step of a sweep updates up to n rows of the column in LDS at once.
+constexpr int SOLVE_NT = 1024;
+
+__global__ __launch_bounds__(SOLVE_NT) void k_bb_solve(const FactorTask* __restrict__ tasks,
+                                                       const Item* __restrict__ items) {
   PGA_KERNEL_VGPR_FLOOR();
   extern __shared__ uint32_t col[];
   const Item it = items[blockIdx.x];
@@ -243,7 +252,7 @@ __global__ __launch_bounds__(NT) void k_bb_solve(const FactorTask* __restrict__ 
   if (factor && *t.info) return;  // factor-and-solve tasks solve only after success
   const int n = t.n;
   uint32_t* b = t.b + (size_t)it.off * t.ldb;
-  for (int i = threadIdx.x; i < n; i += NT) col[i] = b[i];
+  for (int i = threadIdx.x; i < n; i += SOLVE_NT) col[i] = b[i];
   __syncthreads();
   if (is_lu(t.kind)) {
     if (threadIdx.x == 0) {  // Rlaswp(1, n, ipiv, 1): the interchanges in order
@@ -257,16 +266,16 @@ __global__ __launch_bounds__(NT) void k_bb_solve(const FactorTask* __restrict__ 
       }
     }
     __syncthreads();
-    wg::trsm_llnu(n, 1, t.a, t.lda, col, n);
-    wg::trsm_lunn(n, 1, t.a, t.lda, col, n);
+    wg::trsm_llnu<SOLVE_NT>(n, 1, t.a, t.lda, col, n);
+    wg::trsm_lunn<SOLVE_NT>(n, 1, t.a, t.lda, col, n);
   } else if (t.uplo == 'U') {
-    wg::trsm_lutn(n, 1, t.a, t.lda, col, n);
-    wg::trsm_lunn(n, 1, t.a, t.lda, col, n);
+    wg::trsm_lutn<SOLVE_NT>(n, 1, t.a, t.lda, col, n);
+    wg::trsm_lunn<SOLVE_NT>(n, 1, t.a, t.lda, col, n);
   } else {
-    wg::trsm_llnn(n, 1, t.a, t.lda, col, n);
-    wg::trsm_lltn(n, 1, t.a, t.lda, col, n);
+    wg::trsm_llnn<SOLVE_NT>(n, 1, t.a, t.lda, col, n);
+    wg::trsm_lltn<SOLVE_NT>(n, 1, t.a, t.lda, col, n);
   }
-  for (int i = threadIdx.x; i < n; i += NT) b[i] = col[i];
+  for (int i = threadIdx.x; i < n; i += SOLVE_NT) b[i] = col[i];
 }
 
 }  // namespace bb
@@ -274,7 +283,8 @@ __global__ __launch_bounds__(NT) void k_bb_solve(const FactorTask* __restrict__ 
 // ===================================================================================
 // Host side: the plan image (descriptors of every step) and the launch sequence
 // ===================================================================================
-struct BlockedPlan {
+// One group of tasks: its own lockstep sequence of steps on one stream.
+struct BlockedGroup {
   struct Node {  // a recursion node of the RECUR_W-wide LU panels: its Rgemm launch
     int jr, s, mid, e, tile, total, counter;
     size_t tiles;
@@ -289,7 +299,6 @@ struct BlockedPlan {
   std::vector<Step> steps;
   size_t tasks = 0, solve = 0, counters = 0, spare_info = 0, swaps = 0;
   int ntasks = 0, n_solve = 0, ncounters = 0, max_solve_n = 0, n_swaps = 0, node_counter_base = 0;
-  int gemm_grid[3] = {1, 1, 1};  // persistent workgroups for tiles of 64, 32, 16
   unsigned char* dev = nullptr;  // device copy of the image
 };
 
@@ -331,19 +340,13 @@ size_t put_gemms(std::vector<unsigned char>& img, std::vector<bb::Gemm>& g, size
   return off;
 }
 
-int grid_index(int tile) { return tile == 64 ? 0 : (tile == 32 ? 1 : 2); }
-
-template <class K>
-int occupancy(K kernel) {
-  int v = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, kernel, bb::NT, 0) != hipSuccess || v < 1) v = 1;
-  return v;
-}
-
-// one persistent GEMM launch with the step's tile size
-hipError_t launch_gemm(const BlockedPlan* p, int tile, int total, int nprob, const unsigned char* probs,
+// one GEMM launch over the step's tiles, at the step's tile size
+hipError_t launch_gemm(const BlockedGroup* p, int tile, int total, int nprob, const unsigned char* probs,
                        const unsigned char* starts, int* counter, hipStream_t s) {
-  const unsigned grid = (unsigned)std::min(total, p->gemm_grid[grid_index(tile)]);
+  // one workgroup per tile: they retire as tiles finish, so another group's launches on
+  // another stream get workgroup slots meanwhile
+  const unsigned grid = (unsigned)total;
+  (void)p;
   const bb::Gemm* g = reinterpret_cast<const bb::Gemm*>(probs);
   const int* st = reinterpret_cast<const int*>(starts);
   if (tile == 64) hipLaunchKernelGGL(bb::k_bb_gemm<64>, dim3(grid), dim3(bb::NT), 0, s, g, st, nprob, total, counter);
@@ -367,9 +370,9 @@ bool node_after_host(int jr, int w, int& s, int& mid, int& e) {
   return false;
 }
 
-hipError_t launch_node_gemm(const BlockedPlan::Node& nd, const FactorTask* tasks, const bb::Item* items, int nitems,
-                            const int* starts, int* counter, unsigned grid, hipStream_t s) {
-  const unsigned g = std::min<unsigned>(grid, (unsigned)nd.total);
+hipError_t launch_node_gemm(const BlockedGroup::Node& nd, const FactorTask* tasks, const bb::Item* items, int nitems,
+                            const int* starts, int* counter, hipStream_t s) {
+  const unsigned g = (unsigned)nd.total;  // one workgroup per tile
 #define PGA_NODE(T)                                                                                          \
   hipLaunchKernelGGL(bb::k_bb_node_gemm<T>, dim3(g), dim3(bb::NT), 0, s, tasks, items, starts, nitems, nd.total, \
                      counter, nd.s, nd.mid, nd.e)
@@ -387,8 +390,8 @@ bool lu_kind(int k) { return k == PGA_TASK_GETRF || k == PGA_TASK_GESV || k == P
 
 }  // namespace
 
-BlockedPlan* blocked_plan_build(const FactorTask* tasks, int count) {
-  auto* p = new BlockedPlan();
+BlockedGroup* group_build(const FactorTask* tasks, int count) {
+  auto* p = new BlockedGroup();
   p->ntasks = count;
   std::vector<FactorTask> tv(tasks, tasks + count);
   p->tasks = put(p->image, tv);
@@ -448,7 +451,7 @@ BlockedPlan* blocked_plan_build(const FactorTask* tasks, int count) {
         }
       }
     }
-    BlockedPlan::Step st;
+    BlockedGroup::Step st;
     st.syrk = put_gemms(p->image, syrk, st.syrk_tiles, st.syrk_total, st.syrk_tile);
     st.n_syrk = (int)syrk.size();
     st.panel = put(p->image, panel);
@@ -465,7 +468,8 @@ BlockedPlan* blocked_plan_build(const FactorTask* tasks, int count) {
       for (int jr = 0; jr < bb::RECUR_W; ++jr) {
         int ns, nmid, ne;
         if (!node_after_host(jr, bb::RECUR_W, ns, nmid, ne)) continue;
-        BlockedPlan::Node nd{jr, ns, nmid, ne, 16, 0, 0, 0};
+        if (ne - ns == 2) continue;  // two-column nodes run inside k_bb_pair
+        BlockedGroup::Node nd{jr, ns, nmid, ne, 16, 0, 0, 0};
         long t64 = 0, t32 = 0;
         for (const bb::Item& it : recur) {
           const int m = tasks[it.task].n - it.j - nmid;
@@ -511,20 +515,18 @@ BlockedPlan* blocked_plan_build(const FactorTask* tasks, int count) {
   p->counters = put(p->image, std::vector<int>(std::max(1, p->ncounters), 0));
   // info of tasks given none: the steps read it as their stop flag
   p->spare_info = put(p->image, std::vector<int>(std::max(1, count), 0));
-  static const int per_cu[3] = {occupancy(bb::k_bb_gemm<64>), occupancy(bb::k_bb_gemm<32>), occupancy(bb::k_bb_gemm<16>)};
-  for (int i = 0; i < 3; ++i) p->gemm_grid[i] = per_cu[i] * runtime().cus;
   return p;
 }
 
-size_t blocked_plan_bytes(const BlockedPlan* p) { return align16(p->image.size()); }
+size_t group_bytes(const BlockedGroup* p) { return align16(p->image.size()); }
 
-hipError_t blocked_plan_upload(BlockedPlan* p, void* dev, hipStream_t s) {
+hipError_t group_upload(BlockedGroup* p, void* dev, hipStream_t s) {
   p->dev = static_cast<unsigned char*>(dev);
   FactorTask* t = reinterpret_cast<FactorTask*>(p->image.data() + p->tasks);
   for (int i = 0; i < p->ntasks; ++i)
     if (!t[i].info) t[i].info = reinterpret_cast<int*>(p->dev + p->spare_info) + i;
   // the Cholesky GEMMs stop with their task
-  for (const BlockedPlan::Step& st : p->steps) {
+  for (const BlockedGroup::Step& st : p->steps) {
     for (auto [off, cnt] : {std::pair<size_t, int>(st.syrk, st.n_syrk), std::pair<size_t, int>(st.gemm, st.n_gemm)}) {
       bb::Gemm* g = reinterpret_cast<bb::Gemm*>(p->image.data() + off);
       for (int k = 0; k < cnt; ++k) g[k].stop = g[k].task >= 0 ? t[g[k].task].info : nullptr;
@@ -533,7 +535,7 @@ hipError_t blocked_plan_upload(BlockedPlan* p, void* dev, hipStream_t s) {
   return hipMemcpyAsync(dev, p->image.data(), p->image.size(), hipMemcpyHostToDevice, s);
 }
 
-hipError_t blocked_plan_launch(const BlockedPlan* p, hipStream_t s) {
+hipError_t group_launch(const BlockedGroup* p, hipStream_t s) {
   using namespace bb;
   unsigned char* d = p->dev;
   const FactorTask* tasks = reinterpret_cast<const FactorTask*>(d + p->tasks);
@@ -543,7 +545,7 @@ hipError_t blocked_plan_launch(const BlockedPlan* p, hipStream_t s) {
     hipLaunchKernelGGL(k_bb_init, dim3((init_n + 255) / 256), dim3(256), 0, s, tasks, p->ntasks, counters,
                        p->ncounters);
   for (size_t i = 0; i < p->steps.size(); ++i) {
-    const BlockedPlan::Step& st = p->steps[i];
+    const BlockedGroup::Step& st = p->steps[i];
     if (st.syrk_total > 0)
       (void)launch_gemm(p, st.syrk_tile, st.syrk_total, st.n_syrk, d + st.syrk, d + st.syrk_tiles, counters + 2 * i, s);
     if (st.n_panel > 0)
@@ -552,15 +554,15 @@ hipError_t blocked_plan_launch(const BlockedPlan* p, hipStream_t s) {
     if (st.n_recur > 0) {
       const Item* ri = reinterpret_cast<const Item*>(d + st.recur);
       size_t nn = 0;
-      for (int jr = 0; jr < RECUR_W; ++jr) {
-        const BlockedPlan::Node* nd = (nn < st.nodes.size() && st.nodes[nn].jr == jr) ? &st.nodes[nn++] : nullptr;
-        // the node's Rtrsm rides on the leaf that ends its left half
-        hipLaunchKernelGGL(k_bb_leaf, dim3(st.n_recur), dim3(NT), 0, s, tasks, ri, jr, nd ? nd->s : 0,
+      for (int jr = 0; jr < RECUR_W; jr += 2) {
+        const BlockedGroup::Node* nd =
+            (nn < st.nodes.size() && st.nodes[nn].jr == jr + 1) ? &st.nodes[nn++] : nullptr;
+        // the pair's second leaf ends the left half of node nd: its Rtrsm rides along
+        hipLaunchKernelGGL(k_bb_pair, dim3(st.n_recur), dim3(PAIR_NT), 0, s, tasks, ri, jr, nd ? nd->s : 0,
                            nd ? nd->mid : 0, nd ? nd->e : 0);
         if (nd && nd->total > 0)
           (void)launch_node_gemm(*nd, tasks, ri, st.n_recur, reinterpret_cast<const int*>(d + nd->tiles),
-                                 counters + p->node_counter_base + nd->counter,
-                                 (unsigned)p->gemm_grid[grid_index(nd->tile)], s);
+                                 counters + p->node_counter_base + nd->counter, s);
       }
     }
     if (st.n_trsm_lu > 0)
@@ -577,11 +579,101 @@ hipError_t blocked_plan_launch(const BlockedPlan* p, hipStream_t s) {
     hipLaunchKernelGGL(k_bb_laswp_left, dim3(p->n_swaps), dim3(NT), 0, s, tasks,
                        reinterpret_cast<const Item*>(d + p->swaps));
   if (p->n_solve > 0)
-    hipLaunchKernelGGL(k_bb_solve, dim3(p->n_solve), dim3(NT), (size_t)p->max_solve_n * 4, s, tasks,
+    hipLaunchKernelGGL(k_bb_solve, dim3(p->n_solve), dim3(SOLVE_NT), (size_t)p->max_solve_n * 4, s, tasks,
                        reinterpret_cast<const Item*>(d + p->solve));
   return hipGetLastError();
 }
 
-void blocked_plan_destroy(BlockedPlan* p) { delete p; }
+// ---------------------------------------------------------------------------------
+// The plan: the tasks dealt into groups that run side by side on their own streams.
+// A group's panel steps are latency bound and leave most of the device idle; another
+// group's GEMMs fill it meanwhile.  Tasks are independent, so the bits do not change.
+// ---------------------------------------------------------------------------------
+struct BlockedPlan {
+  std::vector<BlockedGroup*> groups;
+  std::vector<size_t> offsets;  // of each group's image in the device buffer
+  size_t bytes = 0;
+};
+
+#define BB_TRY(x)                    \
+  do {                               \
+    hipError_t _e = (x);             \
+    if (_e != hipSuccess) return _e; \
+  } while (0)
+
+namespace {
+
+// a stream and an event for each extra group of a launch (kept for reuse)
+struct Lane {
+  hipStream_t s = nullptr;
+  hipEvent_t done = nullptr;
+};
+std::mutex g_lane_mu;
+std::vector<Lane> g_lanes;
+hipEvent_t g_fork = nullptr;
+
+
+}  // namespace
+
+BlockedPlan* blocked_plan_build(const FactorTask* tasks, int count) {
+  auto* p = new BlockedPlan();
+  // LU tasks in one group, Cholesky tasks in the other: their per-step chains (the
+  // Rgetrf2 panel walk; the Rpotrf2 blocks) then run side by side instead of one after the
+  // other on one stream.  (Measured on the 100-task mix: 0.59 s against 0.75 s for two
+  // groups balanced by work regardless of kind; a third group for LU gained nothing.)
+  std::vector<FactorTask> part[2];
+  for (int i = 0; i < count; ++i) part[lu_kind(tasks[i].kind) ? 0 : 1].push_back(tasks[i]);
+  for (const std::vector<FactorTask>& v : part) {
+    if (v.empty()) continue;
+    p->groups.push_back(group_build(v.data(), (int)v.size()));
+    p->offsets.push_back(p->bytes);
+    p->bytes += (group_bytes(p->groups.back()) + 255) & ~(size_t)255;
+  }
+  if (p->groups.empty()) {  // no tasks: one empty group keeps launch and upload uniform
+    p->groups.push_back(group_build(nullptr, 0));
+    p->offsets.push_back(0);
+    p->bytes = (group_bytes(p->groups.back()) + 255) & ~(size_t)255;
+  }
+  return p;
+}
+
+size_t blocked_plan_bytes(const BlockedPlan* p) { return p->bytes; }
+
+hipError_t blocked_plan_upload(BlockedPlan* p, void* dev, hipStream_t s) {
+  for (size_t g = 0; g < p->groups.size(); ++g) {
+    hipError_t e = group_upload(p->groups[g], static_cast<unsigned char*>(dev) + p->offsets[g], s);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+hipError_t blocked_plan_launch(const BlockedPlan* p, hipStream_t s) {
+  const int G = (int)p->groups.size();
+  if (G == 1) return group_launch(p->groups[0], s);
+  std::lock_guard<std::mutex> lk(g_lane_mu);  // the lanes and the fork event serve one launch at a time
+  while ((int)g_lanes.size() < G - 1) {
+    Lane l;
+    BB_TRY(hipStreamCreateWithFlags(&l.s, hipStreamNonBlocking));
+    BB_TRY(hipEventCreateWithFlags(&l.done, hipEventDisableTiming));
+    g_lanes.push_back(l);
+  }
+  if (!g_fork) BB_TRY(hipEventCreateWithFlags(&g_fork, hipEventDisableTiming));
+  BB_TRY(hipEventRecord(g_fork, s));
+  for (int g = 1; g < G; ++g) {
+    const Lane& l = g_lanes[g - 1];
+    BB_TRY(hipStreamWaitEvent(l.s, g_fork, 0));
+    BB_TRY(group_launch(p->groups[g], l.s));
+    BB_TRY(hipEventRecord(l.done, l.s));
+  }
+  BB_TRY(group_launch(p->groups[0], s));
+  for (int g = 1; g < G; ++g) BB_TRY(hipStreamWaitEvent(s, g_lanes[g - 1].done, 0));
+  return hipSuccess;
+}
+
+void blocked_plan_destroy(BlockedPlan* p) {
+  if (!p) return;
+  for (BlockedGroup* g : p->groups) delete g;
+  delete p;
+}
 
 }  // namespace pga

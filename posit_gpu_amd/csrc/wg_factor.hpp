@@ -122,10 +122,11 @@ __device__ __forceinline__ void gemm(int m, int n, int k, const uint32_t* A, int
 // form sweeps over columns.
 // ---------------------------------------------------------------------------------
 // B := L^-1 B, L unit lower m x m (Rtrsm.cpp:139-155); b_k == 0 skips step k
+template <int T = NT>
 static __device__ __noinline__ void trsm_llnu(int m, int n, const uint32_t* L, int ldl, uint32_t* B, int ldb) {
   for (int k = 0; k + 1 < m; ++k) {
     const int rows = m - 1 - k;
-    for (int e = threadIdx.x; e < rows * n; e += NT) {
+    for (int e = threadIdx.x; e < rows * n; e += T) {
       const int i = k + 1 + e % rows, j = e / rows;
       const uint32_t bk = B[k + (size_t)j * ldb];
       if (bk != 0) B[i + (size_t)j * ldb] = sub(B[i + (size_t)j * ldb], mul(bk, L[i + (size_t)k * ldl]));
@@ -135,16 +136,17 @@ static __device__ __noinline__ void trsm_llnu(int m, int n, const uint32_t* L, i
 }
 
 // B := L^-1 B, L non-unit lower (Rtrsm.cpp:139-155 with the division of :147-149)
+template <int T = NT>
 static __device__ __noinline__ void trsm_llnn(int m, int n, const uint32_t* L, int ldl, uint32_t* B, int ldb) {
   for (int k = 0; k < m; ++k) {
     const uint32_t lkk = L[k + (size_t)k * ldl];
-    for (int j = threadIdx.x; j < n; j += NT) {
+    for (int j = threadIdx.x; j < n; j += T) {
       const uint32_t bk = B[k + (size_t)j * ldb];
       if (bk != 0) B[k + (size_t)j * ldb] = div(bk, lkk);
     }
     __syncthreads();
     const int rows = m - 1 - k;
-    for (int e = threadIdx.x; e < rows * n; e += NT) {
+    for (int e = threadIdx.x; e < rows * n; e += T) {
       const int i = k + 1 + e % rows, j = e / rows;
       const uint32_t bk = B[k + (size_t)j * ldb];  // nonzero stays nonzero after the division
       if (bk != 0) B[i + (size_t)j * ldb] = sub(B[i + (size_t)j * ldb], mul(bk, L[i + (size_t)k * ldl]));
@@ -154,15 +156,16 @@ static __device__ __noinline__ void trsm_llnn(int m, int n, const uint32_t* L, i
 }
 
 // B := U^-1 B, U non-unit upper (Rtrsm.cpp:120-137)
+template <int T = NT>
 static __device__ __noinline__ void trsm_lunn(int m, int n, const uint32_t* U, int ldu, uint32_t* B, int ldb) {
   for (int k = m - 1; k >= 0; --k) {
     const uint32_t ukk = U[k + (size_t)k * ldu];
-    for (int j = threadIdx.x; j < n; j += NT) {
+    for (int j = threadIdx.x; j < n; j += T) {
       const uint32_t bk = B[k + (size_t)j * ldb];
       if (bk != 0) B[k + (size_t)j * ldb] = div(bk, ukk);
     }
     __syncthreads();
-    for (int e = threadIdx.x; e < k * n; e += NT) {
+    for (int e = threadIdx.x; e < k * n; e += T) {
       const int i = e % k, j = e / k;
       const uint32_t bk = B[k + (size_t)j * ldb];
       if (bk != 0) B[i + (size_t)j * ldb] = sub(B[i + (size_t)j * ldb], mul(bk, U[i + (size_t)k * ldu]));
@@ -173,13 +176,14 @@ static __device__ __noinline__ void trsm_lunn(int m, int n, const uint32_t* U, i
 
 // B := U^-T B, U non-unit upper (Rtrsm.cpp:161-173): row i is b_i minus a_ki*b_k for
 // k = 0..i-1 in ascending order, then divided by u_ii; no zero skips in this form
+template <int T = NT>
 static __device__ __noinline__ void trsm_lutn(int m, int n, const uint32_t* U, int ldu, uint32_t* B, int ldb) {
   for (int k = 0; k < m; ++k) {
     const uint32_t ukk = U[k + (size_t)k * ldu];
-    for (int j = threadIdx.x; j < n; j += NT) B[k + (size_t)j * ldb] = div(B[k + (size_t)j * ldb], ukk);
+    for (int j = threadIdx.x; j < n; j += T) B[k + (size_t)j * ldb] = div(B[k + (size_t)j * ldb], ukk);
     __syncthreads();
     const int rows = m - 1 - k;
-    for (int e = threadIdx.x; e < rows * n; e += NT) {
+    for (int e = threadIdx.x; e < rows * n; e += T) {
       const int i = k + 1 + e % rows, j = e / rows;
       B[i + (size_t)j * ldb] = sub(B[i + (size_t)j * ldb], mul(U[k + (size_t)i * ldu], B[k + (size_t)j * ldb]));
     }
@@ -189,8 +193,9 @@ static __device__ __noinline__ void trsm_lutn(int m, int n, const uint32_t* U, i
 
 // B := L^-T B, L non-unit lower (Rtrsm.cpp:174-186): row i needs b_{i+1} for its first
 // term, so each column is one chain; one lane per column
+template <int T = NT>
 static __device__ __noinline__ void trsm_lltn(int m, int n, const uint32_t* L, int ldl, uint32_t* B, int ldb) {
-  for (int j = threadIdx.x; j < n; j += NT) {
+  for (int j = threadIdx.x; j < n; j += T) {
     uint32_t* b = B + (size_t)j * ldb;
     for (int i = m - 1; i >= 0; --i) {
       uint32_t t = b[i];
@@ -202,13 +207,14 @@ static __device__ __noinline__ void trsm_lltn(int m, int n, const uint32_t* L, i
 }
 
 // B := B L^-T, B m x n, L non-unit lower n x n (Rtrsm.cpp:263-285)
+template <int T = NT>
 static __device__ __noinline__ void trsm_rltn(int m, int n, const uint32_t* L, int ldl, uint32_t* B, int ldb) {
   for (int k = 0; k < n; ++k) {
     const uint32_t r = div(P_ONE, L[k + (size_t)k * ldl]);
-    for (int i = threadIdx.x; i < m; i += NT) B[i + (size_t)k * ldb] = mul(r, B[i + (size_t)k * ldb]);
+    for (int i = threadIdx.x; i < m; i += T) B[i + (size_t)k * ldb] = mul(r, B[i + (size_t)k * ldb]);
     __syncthreads();
     const int cols = n - 1 - k;
-    for (int e = threadIdx.x; e < m * cols; e += NT) {
+    for (int e = threadIdx.x; e < m * cols; e += T) {
       const int i = e % m, j = k + 1 + e / m;
       const uint32_t t = L[j + (size_t)k * ldl];
       if (t != 0) B[i + (size_t)j * ldb] = sub(B[i + (size_t)j * ldb], mul(t, B[i + (size_t)k * ldb]));
