@@ -75,6 +75,7 @@ __global__ void k_bb_init(const FactorTask* __restrict__ tasks, int ntasks, int*
 __global__ __launch_bounds__(NT, 2) void k_bb_panel(const FactorTask* __restrict__ tasks,
                                                     const Item* __restrict__ items) {
   PGA_KERNEL_VGPR_FLOOR();
+  PGA_URGENT();
   __shared__ wg::Shared sh;
   const Item it = items[blockIdx.x];
   const FactorTask& t = tasks[it.task];
@@ -98,6 +99,7 @@ __global__ __launch_bounds__(NT, 2) void k_bb_panel(const FactorTask* __restrict
 __global__ __launch_bounds__(NT) void k_bb_trsm(const FactorTask* __restrict__ tasks,
                                                 const Item* __restrict__ items) {
   PGA_KERNEL_VGPR_FLOOR();
+  PGA_URGENT();
   const Item it = items[blockIdx.x];
   const FactorTask& t = tasks[it.task];
   const int lda = t.lda, j = it.j, jb = it.jb;
@@ -134,6 +136,7 @@ constexpr int PAIR_NT = 512;
 __global__ __launch_bounds__(PAIR_NT) void k_bb_pair(const FactorTask* __restrict__ tasks,
                                                      const Item* __restrict__ items, int jr, int ts, int tm, int te) {
   PGA_KERNEL_VGPR_FLOOR();
+  PGA_URGENT();
   const Item it = items[blockIdx.x];
   const FactorTask& t = tasks[it.task];
   uint32_t* root = t.a + it.j + (size_t)it.j * t.lda;
@@ -182,6 +185,7 @@ __global__ __launch_bounds__(NT) void k_bb_node_gemm(const FactorTask* __restric
 __global__ __launch_bounds__(NT) void k_bb_laswp_left(const FactorTask* __restrict__ tasks,
                                                       const Item* __restrict__ items) {
   PGA_KERNEL_VGPR_FLOOR();
+  PGA_URGENT();
   const Item it = items[blockIdx.x];
   const FactorTask& t = tasks[it.task];
   if (threadIdx.x >= it.cnt) return;
@@ -245,6 +249,7 @@ constexpr int SOLVE_NT = 1024;
 __global__ __launch_bounds__(SOLVE_NT) void k_bb_solve(const FactorTask* __restrict__ tasks,
                                                        const Item* __restrict__ items) {
   PGA_KERNEL_VGPR_FLOOR();
+  PGA_URGENT();
   extern __shared__ uint32_t col[];
   const Item it = items[blockIdx.x];
   const FactorTask& t = tasks[it.task];
