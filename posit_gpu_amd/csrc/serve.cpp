@@ -147,6 +147,7 @@ struct pga_server {
   std::unordered_map<int64_t, std::shared_ptr<Factor>> factors;  // resident or in production
   std::deque<Request*> pending;   // arrival order
   std::deque<Request*> small_ready, large_ready;
+  double t_last_submit = -1.0;    // seconds from t0
   int inflight = 0;
   pga_server_stats stats{};
 
@@ -427,6 +428,16 @@ void worker_main(pga_server* S) {
   for (;;) {
     S->cv_work.wait(lk, [&] { return S->stop || !S->large_ready.empty(); });
     if (S->stop) break;
+    if (S->cfg.mode != PGA_SERVER_PLAIN) {
+      // a burst of submissions still arriving (the last one under 1 ms ago): let it gather
+      // for at most 4 ms, so that it forms a few large blocked plans instead of many small
+      // ones; requests that arrive alone are taken after at most 1 ms
+      const double t_wait0 = now_s(S->t0);
+      while (!S->stop && now_s(S->t0) - S->t_last_submit < 1e-3 && now_s(S->t0) - t_wait0 < 4e-3)
+        S->cv_work.wait_for(lk, std::chrono::microseconds(500));
+      if (S->stop) break;
+      if (S->large_ready.empty()) continue;  // another worker took them meanwhile
+    }
     Request* first = S->large_ready.front();
     if (S->cfg.mode == PGA_SERVER_PLAIN || first->r.n > PGA_BLOCKED_TASK_MAX_N) {
       S->large_ready.pop_front();
@@ -627,6 +638,7 @@ pga_status pga_server_submit(pga_server* S, const pga_request* r, int64_t* id) {
   q->res.t_submit = now_s(S->t0);
   *id = q->id;
   S->pending.push_back(q.get());
+  S->t_last_submit = q->res.t_submit;
   S->reqs[q->id] = std::move(q);
   S->inflight++;
   S->stats.submitted++;
