@@ -378,3 +378,60 @@ def test_cholesky_panel_overlap(pga, oracle, uplo, n, nb, bad):
     pga.synchronize()
     assert int(di.to_host()[0]) == winfo == (bad + 1 if bad else 0)
     _same(dA.to_host(), want)
+
+
+def test_full_size_invariants(pga):
+    """The n = 8192 configurations through properties that do not need the oracle: the LU
+    with nb = 256 equals the LU with nb = 64 bit for bit (pivots included), the lower
+    Cholesky with nb = 64 equals the one with nb = 128, and both solves have small FP64
+    backward errors."""
+    n = 8192
+    ONE_ = 0x40000000
+    dA0 = pga.DeviceArray((n, n))
+    pga.fill_uniform(dA0, 4242)
+    ones = pga.DeviceArray.from_host(np.full(n, ONE_, np.uint32))
+    db = pga.DeviceArray(n, np.uint32, "C")
+    pga.gemv_device("N", n, n, ONE_, dA0, n, ones, 1, 0, db, 1)
+    dp = pga.DeviceArray(n, np.int32, "C")
+    di = pga.DeviceArray(1, np.int32, "C")
+    dA = pga.DeviceArray((n, n))
+    dA.copy_from(dA0)
+    pga.getrf_device(n, n, dA, n, dp, di, 256)
+    pga.synchronize()
+    F256, P256 = dA.to_host(), dp.to_host()
+    assert int(di.to_host()[0]) == 0
+    dx = pga.DeviceArray((n, 1))
+    dx.copy_from(db)
+    pga.getrs_device("N", n, 1, dA, n, dp, dx, n)
+    assert pga.backward_error_device(n, dA0, n, dx, db) < 1e-4
+    dA.copy_from(dA0)
+    pga.getrf_device(n, n, dA, n, dp, di, 64)
+    pga.synchronize()
+    _same(dp.to_host(), P256)
+    _same(dA.to_host(), F256)
+    del F256
+    # SPD input on the device: S = M M^T / n + I is built in FP64 on the host
+    M = uniform_doubles(4300, n * n).reshape(n, n)
+    S = M @ M.T
+    S /= n
+    S[np.diag_indices(n)] += 1.0
+    del M
+    dS = pga.DeviceArray.from_host(np.asfortranarray(S))
+    del S
+    dC0 = pga.DeviceArray((n, n))
+    pga._check(pga._lib.pga_p32_from_f64_device(dS.ptr, dC0.ptr, n * n, None), "from_f64")
+    dS.free()
+    pga.gemv_device("N", n, n, ONE_, dC0, n, ones, 1, 0, db, 1)
+    dC = pga.DeviceArray((n, n))
+    dC.copy_from(dC0)
+    pga.potrf_device("L", n, dC, n, di, 64)
+    pga.synchronize()
+    assert int(di.to_host()[0]) == 0
+    L64 = np.tril(dC.to_host())
+    dx.copy_from(db)
+    pga.potrs_device("L", n, 1, dC, n, dx, n)
+    assert pga.backward_error_device(n, dC0, n, dx, db) < 1e-6
+    dC.copy_from(dC0)
+    pga.potrf_device("L", n, dC, n, di, 128)
+    pga.synchronize()
+    _same(np.tril(dC.to_host()), L64)
