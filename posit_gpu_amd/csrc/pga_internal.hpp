@@ -71,6 +71,7 @@ struct Runtime {
   hipStream_t stream = nullptr;
   GemmImpl gemm_impl = GemmImpl::Tiled;
   int cus = 256;
+  int gemm_tile = 0;  // PGA_GEMM_TILE: 0 per shape, else the tile edge of the plain GEMM
 };
 Runtime& runtime();
 

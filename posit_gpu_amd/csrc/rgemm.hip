@@ -227,7 +227,7 @@ hipError_t rgemm_launch(bool ta, bool tb, int m, int n, int k, uint32_t alpha, c
     }
     return hipGetLastError();
   }
-  int tile = o.tile ? o.tile : pick_tile(m, n, runtime().cus);
+  int tile = o.tile ? o.tile : (runtime().gemm_tile ? runtime().gemm_tile : pick_tile(m, n, runtime().cus));
   if (o.skip_zero_b)
     return dispatch<true>(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, impl, o.tri, o.urgent, o.stop, tile);
   return dispatch<false>(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, impl, o.tri, o.urgent, o.stop, tile);

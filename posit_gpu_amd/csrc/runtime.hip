@@ -216,6 +216,14 @@ pga_status pga_init(int device) {
   }
   const char* impl = getenv("PGA_GEMM_IMPL");
   if (impl && strcmp(impl, "exact") == 0) g_rt.gemm_impl = GemmImpl::Exact;
+  // the tile edge of the plain GEMM (the role of OPENCL_GEMM_BLOCKSIZE in the reference):
+  // 16, 32 or 64; anything else keeps the per-shape choice.  K is never split, so every
+  // tile gives the same bits.
+  const char* tile = getenv("PGA_GEMM_TILE");
+  if (tile) {
+    const int t = atoi(tile);
+    g_rt.gemm_tile = (t == 16 || t == 32 || t == 64) ? t : 0;
+  }
   g_rt.initialized = true;
   return PGA_SUCCESS;
 }

@@ -186,3 +186,30 @@ def test_trailing_shape_16384_sampled_columns(pga, oracle):
     Cs = np.asfortranarray(C0[:, cols])
     oracle.rgemm("N", "N", M, cols.size, K, MONE, A, Bs, ONE, Cs, nthreads=THREADS)
     _assert_same(np.asfortranarray(got[:, cols]), Cs)
+
+
+def test_tile_override_same_bits(pga, tmp_path):
+    """PGA_GEMM_TILE fixes the plain GEMM's tile edge (the role of OPENCL_GEMM_BLOCKSIZE);
+    a process started with each value computes the same bits as the per-shape choice."""
+    import os
+    import subprocess
+    import sys
+    m, n, k = 300, 270, 130
+    script = (
+        "import sys, numpy as np; sys.path.insert(0, {root!r}); import posit_gpu_amd as pga; pga.init(0)\n"
+        "A = pga.DeviceArray(({m}, {k})); B = pga.DeviceArray(({k}, {n})); C = pga.DeviceArray(({m}, {n}))\n"
+        "pga.fill_uniform(A, 91); pga.fill_uniform(B, 92); pga.fill_uniform(C, 93)\n"
+        "pga.rgemm_device('N', 'N', {m}, {n}, {k}, 0xC0000000, A, {m}, B, {k}, 0x40000000, C, {m})\n"
+        "pga.synchronize(); np.save(sys.argv[1], C.to_host())\n"
+    ).format(root=os.path.dirname(os.path.dirname(os.path.abspath(__file__))), m=m, n=n, k=k)
+    outs = {}
+    for tile in ("", "16", "32", "64"):
+        env = dict(os.environ)
+        env.pop("PGA_GEMM_TILE", None)
+        if tile:
+            env["PGA_GEMM_TILE"] = tile
+        out = tmp_path / f"c{tile or 'auto'}.npy"
+        subprocess.run([sys.executable, "-c", script, str(out)], env=env, check=True, timeout=240)
+        outs[tile] = np.load(out)
+    for tile in ("16", "32", "64"):
+        assert np.array_equal(outs[tile], outs[""]), tile
