@@ -463,6 +463,77 @@ __global__ void k_potrf2_leaf(uint32_t* a, int* info, int col_off) {
 // The block is staged in LDS: each recursion step then waits on LDS, not on L2.
 constexpr int POTRF2_ONE_WG_MAX = 64;
 
+// Lower Rpotrf2 of an n x n block (n <= 64) with the lower triangle held in registers,
+// T lanes, packed column-major elements dealt round-robin.  The recursion
+// (Rpotrf2.cpp:102-145) gives element (i, j), i > j, the terms -L(j,l)*L(i,l) for
+// l = 0 .. j-1 in ascending order -- from the Rsyrk 'L','N' of every node that has both
+// columns in its right half (l over that node's left half), then from the Rtrsm
+// right/lower/transposed of the node that splits them (l from the node's start to j) --
+// each skipped when the multiplier L(j,l) is zero (Rsyrk.cpp:169, Rtrsm.cpp:272), and
+// then the scale by 1/L(j,j) (Rtrsm.cpp:266).  The diagonal gets the same terms from the
+// Rsyrk calls alone, then the test and the sqrt of the leaf (Rpotrf2.cpp:90-97).  An
+// Rsyrk term c + (-m)*x and an Rtrsm term c - m*x round alike (posit rounding is
+// symmetric in sign), so the right-looking order below -- column j final, then its terms
+// applied to every later column -- computes the same bits.  The state the recursion
+// leaves after a failing leaf is not this one: on a failure the caller redoes the block
+// from its unchanged input with the recursion itself.  Returns 0 or the failing column
+// (1-based); the block in memory is written only on success.
+template <int T>
+static __device__ int potrf2_lower_regs(int n, uint32_t* a, int lda) {
+  constexpr int SLOTS = (POTRF2_ONE_WG_MAX * (POTRF2_ONE_WG_MAX + 1) / 2 + T - 1) / T;
+  __shared__ uint32_t colv[POTRF2_ONE_WG_MAX];
+  __shared__ uint32_t dg;
+  uint32_t c[SLOTS];
+  int ri[SLOTS], ci[SLOTS];
+  {
+    int j = 0, off = 0;  // off: packed index of column j's diagonal element
+#pragma unroll
+    for (int q = 0; q < SLOTS; ++q) {
+      const int e = threadIdx.x + q * T;
+      while (j < n && off + (n - j) <= e) {
+        off += n - j;
+        ++j;
+      }
+      ri[q] = j + (e - off);
+      ci[q] = j < n ? j : -1;  // -1: no element (matches no step)
+      c[q] = j < n ? a[ri[q] + (size_t)j * lda] : 0u;
+    }
+  }
+  if (threadIdx.x == 0) dg = a[0];
+  __syncthreads();
+  for (int j = 0; j < n; ++j) {
+    const uint32_t v = dg;  // every lane reads the same value
+    if (le(v, 0)) return j + 1;
+    const uint32_t d = pga::sqrt(v);
+    const uint32_t r = j + 1 < n ? div(P_ONE, d) : 0u;
+#pragma unroll
+    for (int q = 0; q < SLOTS; ++q) {
+      if (ci[q] == j) {
+        if (ri[q] == j) {
+          c[q] = d;
+        } else {
+          c[q] = mul(r, c[q]);
+          colv[ri[q]] = c[q];
+        }
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < SLOTS; ++q) {
+      if (ci[q] > j) {
+        const uint32_t m = colv[ci[q]];
+        if (m != 0) c[q] = sub(c[q], mul(m, colv[ri[q]]));
+        if (ri[q] == j + 1 && ci[q] == j + 1) dg = c[q];
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int q = 0; q < SLOTS; ++q)
+    if (ci[q] >= 0) a[ri[q] + (size_t)ci[q] * lda] = c[q];
+  return 0;
+}
+
 template <bool UPPER>
 __global__ __launch_bounds__(WG, 2) void k_potrf2_wg(int n, uint32_t* a, int lda, int* info, int col_off) {
   PGA_KERNEL_VGPR_FLOOR();
@@ -470,6 +541,7 @@ __global__ __launch_bounds__(WG, 2) void k_potrf2_wg(int n, uint32_t* a, int lda
   __shared__ wg::Shared sh;
   __shared__ uint32_t blk[POTRF2_ONE_WG_MAX * POTRF2_ONE_WG_MAX];
   if (*info) return;  // every lane reads the same flag
+  if (!UPPER && potrf2_lower_regs<WG>(n, a, lda) == 0) return;
   for (int e = threadIdx.x; e < n * n; e += WG) blk[e] = a[e % n + (size_t)(e / n) * lda];
   __syncthreads();
   const int f = wg::potrf2<UPPER>(n, blk, n, sh);

@@ -319,6 +319,53 @@ def test_cholesky_residual_solve_and_upper(oracle):
         assert np.abs(oracle.to_double_v(B[:, 0]) - 1).max() < 1e-3
 
 
+def _unblocked_right_looking_lower(oracle, A):
+    """Lower Cholesky one column at a time, right-looking: column j final (test, sqrt,
+    scale by 1/l_jj), then its terms -l_kj*l_ij applied to every later column k, each
+    skipped when the multiplier l_kj is zero.  The order of the device's register-resident
+    diagonal block; None when a diagonal entry fails the test."""
+    n = A.shape[0]
+    A = A.copy(order="F")
+    for j in range(n):
+        v = int(A[j, j])
+        if v == 0 or (v >> 31):
+            return None
+        d = oracle.sqrt(v)
+        A[j, j] = d
+        if j + 1 == n:
+            break
+        r = oracle.div(ONE, d)
+        A[j + 1:, j] = oracle.mul_v(np.full(n - j - 1, r, dtype=np.uint32), A[j + 1:, j])
+        for k in range(j + 1, n):
+            m = int(A[k, j])
+            if m == 0:
+                continue
+            A[k:, k] = oracle.sub_v(A[k:, k], oracle.mul_v(np.full(n - k, m, dtype=np.uint32), A[k:, j]))
+    return A
+
+
+@pytest.mark.parametrize("n,kind", [(64, "dense"), (37, "dense"), (64, "sparse"), (50, "sparse"), (1, "dense")])
+def test_cholesky_lower_unblocked_order(oracle, n, kind):
+    """Rpotrf2 'L' (recursive) gives each element the same terms in the same order as the
+    column-at-a-time right-looking form, zero-multiplier skips included."""
+    if kind == "dense":
+        A = spd_matrix(oracle, 55, n)
+    else:  # diagonally dominant with a sparse pattern: exact zeros in the factor
+        rng = np.random.default_rng(56 + n)
+        S = rng.uniform(-1, 1, (n, n)) * (rng.uniform(0, 1, (n, n)) < 0.15)
+        S = np.tril(S, -1)
+        S = S + S.T + np.eye(n) * (n / 4 + 2)
+        A = np.asfortranarray(oracle.from_double_v(S.ravel(order="F")).reshape(n, n, order="F"))
+    want = A.copy(order="F")
+    rc, info = oracle.rpotrf2("L", want)
+    assert rc == 0 and info == 0
+    got = _unblocked_right_looking_lower(oracle, A)
+    assert got is not None
+    assert np.array_equal(got, want)
+    if kind == "sparse":
+        assert (got[np.tril_indices(n, -1)] == 0).sum() > n  # the skips were exercised
+
+
 def test_cholesky_not_pd_info(oracle):
     n = 70
     A = spd_matrix(oracle, 53, n)
