@@ -217,27 +217,34 @@ PGA_HD uint32_t div(uint32_t a, uint32_t b) {
   uint32_t ma, mb;
   unpack27(a, na, sa, ma);
   unpack27(b, nb, sb, mb);
+  // q = floor(ma * 2^36 / mb): the quotient from the FP64 division (num has at most 28
+  // significant bits, so both operands convert exactly, and the rounded quotient is
+  // within one of the true one), corrected against the exact remainder.  A 64-bit
+  // integer division costs several times as many instructions on the GPU.
   uint64_t num = (uint64_t)ma << 36;
-  uint64_t q = num / mb;
-  uint64_t r = num - q * mb;
+  uint64_t q = (uint64_t)((double)num / (double)mb);
+  uint64_t p = q * mb;  // q <= 2^37 + 1 and mb < 2^28: no overflow
+  while (p > num) {
+    --q;
+    p -= mb;
+  }
+  while (num - p >= mb) {
+    ++q;
+    p += mb;
+  }
+  uint64_t r = num - p;
   int hb = (q >> 36) ? 36 : 35;
   return encode_rne(na != nb, sa - sb - 36 + hb, q, hb, r != 0);
 }
 
-// floor(sqrt(x)) for x < 2^62, plus exactness
+// floor(sqrt(x)) for x < 2^62 with at most 53 significant bits, plus exactness: the
+// FP64 square root (x converts exactly; the result is within one of the true root),
+// corrected with exact integer squares.
 PGA_HD uint64_t isqrt64(uint64_t x, bool& exact) {
-  uint64_t res = 0, bit = 1ull << 62;
-  while (bit > x) bit >>= 2;
-  while (bit) {
-    if (x >= res + bit) {
-      x -= res + bit;
-      res = (res >> 1) + bit;
-    } else {
-      res >>= 1;
-    }
-    bit >>= 2;
-  }
-  exact = (x == 0);
+  uint64_t res = (uint64_t)__builtin_sqrt((double)x);
+  while (res * res > x) --res;
+  while ((res + 1) * (res + 1) <= x) ++res;
+  exact = (res * res == x);
   return res;
 }
 

@@ -152,3 +152,19 @@ def test_commutativity_and_negation(oracle):
     s = oracle.add_v(a, b)
     assert np.array_equal(oracle.add_v(na, nb), (-s.astype(np.int64) & 0xFFFFFFFF).astype(np.uint32))
     assert np.array_equal(oracle.mul_v(na, b), (-oracle.mul_v(a, b).astype(np.int64) & 0xFFFFFFFF).astype(np.uint32))
+
+
+def test_core_div_sqrt_host_build(tmp_path):
+    """The kernels' pattern-level div and sqrt (posit32_core.hpp: FP64-seeded quotient and
+    root, corrected with exact integer arithmetic), built for the host, agree with the
+    oracle's restatement on a stride of all patterns (sqrt) and on random and edge pairs
+    (div)."""
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = str(tmp_path / "core_host_check")
+    oracle_dir = os.path.join(root, "oracle")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-I", os.path.join(root, "posit_gpu_amd", "csrc"),
+                    os.path.join(root, "tests", "cpp", "core_host_check.cpp"), "-L", oracle_dir, "-lorc",
+                    "-Wl,-rpath," + oracle_dir, "-o", exe], check=True, timeout=120)
+    out = subprocess.run([exe, "4099", "2000000"], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout[-2000:]
