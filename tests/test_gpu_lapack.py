@@ -278,6 +278,33 @@ def test_potrf_not_pd(pga, oracle, uplo):
         _same(got, want)  # the factorization stops at the same place
 
 
+@pytest.mark.parametrize("kind", ["huge", "tiny", "sparse", "nar"])
+def test_potrf_lower_diagonal_block_ranges(pga, oracle, kind):
+    """The lower diagonal blocks run decoded while every product and sum stays in the
+    fast scale range and fall back to the pattern-level form otherwise: scaled far up or
+    down, with exact zeros (skipped multipliers), or with a NaR (a failing block)."""
+    n = 150
+    if kind == "sparse":
+        rng = np.random.default_rng(57)
+        S = rng.uniform(-1, 1, (n, n)) * (rng.uniform(0, 1, (n, n)) < 0.1)
+        S = np.tril(S, -1)
+        S = S + S.T + np.eye(n) * (n / 5 + 2)
+    else:
+        M = uniform_doubles(58, n * n).reshape(n, n)
+        S = M @ M.T / n + np.eye(n)
+        S *= {"huge": 2.0 ** 112, "tiny": 2.0 ** -116, "nar": 1.0}[kind]
+    A = np.asfortranarray(oracle.from_double_v(S.ravel(order="F")).reshape(n, n, order="F"))
+    if kind == "nar":
+        A[100, 70] = 0x80000000
+    for nb in (0, 16):
+        want = A.copy(order="F")
+        _, winfo = oracle.rpotrf("L", want, nb=nb, nthreads=THREADS)
+        got = A.copy(order="F")
+        info = pga.Rpotrf_opencl("L", n, got, n, nb)
+        assert info == winfo and (info > 0) == (kind == "nar")
+        _same(got, want)
+
+
 def test_potrf2(pga, oracle):
     n = 77
     A = _spd(oracle, 53, n)
