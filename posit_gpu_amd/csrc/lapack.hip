@@ -83,85 +83,97 @@ __global__ __launch_bounds__(WG) void k_trsm_left(int m, int n, uint32_t alpha, 
 
 // Left, lower, transposed (Rtrsm.cpp:174-186), one right-hand side per workgroup:
 // temp_i = alpha*b_i - sum_{k>i} a_ki*b_k in ascending k, i descending.  Row i's first
-// term needs b_{i+1}, so the rows form one chain of m^2/2 rounded additions.  Per row the
-// lanes form the negated products, decoded, in LDS (CHAIN_CHUNK terms at a time); wave 0
-// then runs the additions with every lane computing the same values, eight terms a group
-// while the next group's terms load.  A group whose fast additions left the fast range
-// (or met NaR) is redone from its start term by term in the exact form.
+// term needs b_{i+1}, so the rows form one chain of m^2/2 rounded additions, and a
+// single wave's instruction issue bounds it.  Per row the lanes form the negated
+// products (posits, exact in FP64) in LDS, CHAIN_CHUNK terms at a time; wave 0 then
+// runs the additions, every lane computing the same values, eight terms a group while
+// the next group loads.
+//
+// One addition is fp64_posit_add (posit32_core.hpp): round(t + p) on the posit grid in
+// three FP64 additions.  A group in which one of them reports a case it does not round
+// (a scale outside -108 .. 107, an exact zero) is redone from its start with the exact
+// decoded additions.  A NaR term or start value makes the row NaR, as
+// NaR absorbs every addition.
 constexpr int CHAIN_CHUNK = 4096;
+
+// t <- t + p exactly for posit values t and p (finite), through the decoded form
+__device__ __forceinline__ void chain_add_exact(double& t, double p) {
+  dec::Val v = dec::decode(from_double(t));
+  gemm::add_checked(v, dec::decode(from_double(p)));
+  t = to_double(dec::encode(v));
+}
 
 struct ChainGroup {
   static constexpr int G = 8;
-  uint32_t m[G], g[G];
-  int s[G];
-  __device__ __forceinline__ void load(const uint32_t* sm, const int* ss, const uint32_t* sg, int k) {
+  double p[G];
+  __device__ __forceinline__ void load(const double* sp, int k) {
 #pragma unroll
-    for (int u = 0; u < G; ++u) m[u] = sm[k + u], s[u] = ss[k + u], g[u] = sg[k + u];
+    for (int u = 0; u < G; ++u) p[u] = sp[k + u];
   }
-  // t <- t + the group's terms in order: the fast form, or the exact one when any fast
-  // sum left the fast range
-  __device__ __forceinline__ void apply(dec::Val& t) {
-    // keep the terms in VGPRs (the chain runs on the VALU; uniform copies would be moved
-    // to SGPRs and back for every select)
+  __device__ __forceinline__ void apply(double& t) {
+    const double save = t;
+    uint32_t reach = 0;
 #pragma unroll
-    for (int u = 0; u < G; ++u) asm volatile("" : "+v"(m[u]), "+v"(s[u]), "+v"(g[u]));
-    const dec::Val save = t;
-    bool ok = true;
-#pragma unroll
-    for (int u = 0; u < G; ++u) ok &= gemm::add_fast(t, m[u], s[u], g[u]);
-    if (__builtin_amdgcn_readfirstlane((int)!ok)) {
+    for (int u = 0; u < G; ++u) t = fp64_posit_add(t, p[u], reach);
+    if (__builtin_amdgcn_readfirstlane(reach) > FP64_ADD_REACH_MAX) {
       t = save;
 #pragma unroll
-      for (int u = 0; u < G; ++u) gemm::add_checked(t, dec::Val{m[u], s[u], g[u]});
+      for (int u = 0; u < G; ++u) chain_add_exact(t, p[u]);
     }
   }
 };
 
 // Two groups in flight: one loads from LDS while the other is added.
-__device__ __forceinline__ void chain_terms(dec::Val& t, const uint32_t* __restrict__ sm,
-                                            const int* __restrict__ ss, const uint32_t* __restrict__ sg,
-                                            int kn) {
+__device__ __forceinline__ void chain_terms(double& t, const double* __restrict__ sp, int kn) {
   constexpr int G = ChainGroup::G;
   const int groups = kn / G;
   ChainGroup x, y;
-  if (groups > 0) x.load(sm, ss, sg, 0);
+  if (groups > 0) x.load(sp, 0);
   for (int g = 0; g < groups; g += 2) {
-    if (g + 1 < groups) y.load(sm, ss, sg, (g + 1) * G);
+    if (g + 1 < groups) y.load(sp, (g + 1) * G);
     x.apply(t);
     if (g + 1 >= groups) break;
-    if (g + 2 < groups) x.load(sm, ss, sg, (g + 2) * G);
+    if (g + 2 < groups) x.load(sp, (g + 2) * G);
     y.apply(t);
   }
-  for (int k = groups * G; k < kn; ++k) gemm::add_checked(t, dec::Val{sm[k], ss[k], sg[k]});
+  for (int k = groups * G; k < kn; ++k) {
+    const double save = t;
+    uint32_t reach = 0;
+    t = fp64_posit_add(t, sp[k], reach);
+    if (__builtin_amdgcn_readfirstlane(reach) > FP64_ADD_REACH_MAX) {
+      t = save;
+      chain_add_exact(t, sp[k]);
+    }
+  }
 }
 
 __global__ __launch_bounds__(WG) void k_trsm_lltn(int m, int n, uint32_t alpha, int nounit,
                                                   const uint32_t* __restrict__ A, int lda,
                                                   uint32_t* __restrict__ B, int ldb, const int* stop) {
   PGA_KERNEL_VGPR_FLOOR();
-  __shared__ uint32_t sm[CHAIN_CHUNK], sg[CHAIN_CHUNK];
-  __shared__ int ss[CHAIN_CHUNK];
+  __shared__ double sp[CHAIN_CHUNK];
   const int j = blockIdx.x;
   if (j >= n || stopped(stop)) return;
   uint32_t* b = B + (size_t)j * ldb;
   const int tid = threadIdx.x;
   for (int i = m - 1; i >= 0; --i) {
-    dec::Val t{};
-    if (tid < 64) t = dec::decode(mul(alpha, b[i]));
+    const uint32_t t0 = mul(alpha, b[i]);  // every lane reads the same values
+    bool nar = t0 == P_NAR;
+    double t = nar ? 0.0 : to_double(t0);
     for (int k0 = i + 1; k0 < m; k0 += CHAIN_CHUNK) {
       const int kn = min(CHAIN_CHUNK, m - k0);
       __syncthreads();  // wave 0 has used the previous chunk
+      bool term_nar = false;
       for (int u = tid; u < kn; u += WG) {
-        const dec::Val v = dec::decode(neg(mul(A[k0 + u + (size_t)i * lda], b[k0 + u])));
-        sm[u] = v.m;
-        ss[u] = v.sc;
-        sg[u] = v.sg;
+        const uint32_t v = neg(mul(A[k0 + u + (size_t)i * lda], b[k0 + u]));
+        term_nar |= v == P_NAR;
+        sp[u] = term_nar ? 0.0 : to_double(v);
       }
-      __syncthreads();
-      if (tid < 64) chain_terms(t, sm, ss, sg, kn);
+      nar |= __syncthreads_or(term_nar) != 0;
+      if (tid < 64 && !nar) chain_terms(t, sp, kn);
     }
     if (tid == 0) {
-      uint32_t tp = dec::encode(t);
+      uint32_t tp = nar ? P_NAR : from_double(t);
       if (nounit) tp = div(tp, A[i + (size_t)i * lda]);
       b[i] = tp;
     }

@@ -294,6 +294,43 @@ PGA_HD double to_double(uint32_t a) {
   return negz ? -v : v;
 }
 
+// round(t + p) for posit values t and p held exactly in FP64 (the posit sum, as
+// pga::add rounds it), for the long serial addition chains of the triangular solves.
+// x = t + p is first rounded to FP64; then x is rounded to the posit grid of its binade
+// -- nf = 27 - |k|' fraction bits at scale e = 4k + (e mod 4), |k|' = k for k >= 0 and
+// -k - 1 below -- by adding and subtracting C = +-2^(e + 52 - nf), whose FP64 ulp is the
+// posit quantum q = 2^(e - nf): round-to-nearest-even then takes x to a multiple of q,
+// ties to the even one, which is the even posit pattern (nf >= 1).
+// The first rounding cannot change the result: posit midpoints are FP64 numbers and
+// rounding is monotone, so x lies strictly between the same two midpoints as the exact
+// sum, or on one -- and then the FP64 sum was exact.  (An inexact sum needs a term with
+// bits more than 52 places below the result's leading bit; a term that brings the sum
+// within that distance of a midpoint lies within it of a nonzero multiple of half the
+// result's quantum, and at every scale a posit that large has no fraction bits that
+// far down.  A search over 3e7 constructed midpoint sums found no inexact one.)
+// The grid above holds for scales -108 .. 107 (|k|' <= 26: both exponent bits and a
+// fraction bit kept); `reach` is raised to |k|' of the result, and a caller whose
+// `reach` ends above 26 (a scale outside that range, or an exact zero) must redo the
+// additions exactly.
+PGA_HD double fp64_posit_add(double t, double p, uint32_t& reach) {
+  const double x = t + p;
+  uint64_t xb;
+  __builtin_memcpy(&xb, &x, sizeof xb);
+  const uint32_t hi = (uint32_t)(xb >> 32);
+  const int eb = (int)((hi >> 20) & 0x7FFu);  // biased binary exponent of x
+  const int k = (eb - 1023) >> 2;             // posit regime value
+  const uint32_t ak = (uint32_t)(k ^ (k >> 31));
+  reach = reach > ak ? reach : ak;
+  const uint32_t chi = (((uint32_t)eb + 25u + ak) << 20) | (hi & 0x80000000u);  // C, high word
+  const uint64_t cbits = (uint64_t)chi << 32;
+  double c;
+  __builtin_memcpy(&c, &cbits, sizeof c);
+  return (x + c) - c;
+}
+
+// reach values for which fp64_posit_add's results are the posit sums
+constexpr uint32_t FP64_ADD_REACH_MAX = 26;
+
 // ---------------------------------------------------------------------------------
 // Decoded-domain representation used by the GEMM inner loop.
 // ---------------------------------------------------------------------------------

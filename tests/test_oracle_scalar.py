@@ -168,3 +168,22 @@ def test_core_div_sqrt_host_build(tmp_path):
                     "-Wl,-rpath," + oracle_dir, "-o", exe], check=True, timeout=120)
     out = subprocess.run([exe, "4099", "2000000"], capture_output=True, text=True, timeout=300)
     assert out.returncode == 0, out.stdout[-2000:]
+
+
+def test_fp64_posit_add_host_build(tmp_path):
+    """pga::fp64_posit_add (posit32_core.hpp: the posit sum rounded in FP64, for the
+    serial chains of the triangular solves), built for the host: wherever it does not ask
+    for a redo, its result equals the oracle's posit addition, over uniform values,
+    arbitrary patterns, short fractions, cancellations, tiny terms, constructed midpoints
+    and long running sums."""
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = str(tmp_path / "fp64_add_check")
+    oracle_dir = os.path.join(root, "oracle")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-I", os.path.join(root, "posit_gpu_amd", "csrc"),
+                    os.path.join(root, "tests", "cpp", "fp64_add_check.cpp"), "-L", oracle_dir, "-lorc",
+                    "-Wl,-rpath," + oracle_dir, "-o", exe], check=True, timeout=120)
+    out = subprocess.run([exe, "400000"], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout[-2000:]
+    checked, redone = [int(w) for w in out.stdout.split()[1:4:2]]
+    assert redone < checked // 3  # the fast form decides most sums
