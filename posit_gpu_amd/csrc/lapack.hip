@@ -111,6 +111,9 @@ struct ChainGroup {
     for (int u = 0; u < G; ++u) p[u] = sp[k + u];
   }
   __device__ __forceinline__ void apply(double& t) {
+    // keep the terms in VGPRs: uniform copies would be moved to SGPRs one word at a time
+#pragma unroll
+    for (int u = 0; u < G; ++u) asm volatile("" : "+v"(p[u]));
     const double save = t;
     uint32_t reach = 0;
 #pragma unroll

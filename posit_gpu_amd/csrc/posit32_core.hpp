@@ -317,11 +317,19 @@ PGA_HD double fp64_posit_add(double t, double p, uint32_t& reach) {
   uint64_t xb;
   __builtin_memcpy(&xb, &x, sizeof xb);
   const uint32_t hi = (uint32_t)(xb >> 32);
-  const int eb = (int)((hi >> 20) & 0x7FFu);  // biased binary exponent of x
-  const int k = (eb - 1023) >> 2;             // posit regime value
-  const uint32_t ak = (uint32_t)(k ^ (k >> 31));
+  const int e = (int)((hi >> 20) & 0x7FFu) - 1023;  // binary exponent of x
+  // |k|' for the regime value k = e >> 2 (the sign of k is that of e: one step shorter)
+  const uint32_t ak = (uint32_t)((e >> 2) ^ (e >> 31));
   reach = reach > ak ? reach : ak;
-  const uint32_t chi = (((uint32_t)eb + 25u + ak) << 20) | (hi & 0x80000000u);  // C, high word
+  // C's high word: x's sign and exponent field raised by 25 + |k|' (no carry out of the
+  // field for any finite x), formed so that only the last step waits for |k|'
+  const uint32_t base = (hi & 0xFFF00000u) + (25u << 20);
+#if defined(__HIP_DEVICE_COMPILE__)
+  uint32_t chi;
+  asm("v_lshl_add_u32 %0, %1, 20, %2" : "=v"(chi) : "v"(ak), "v"(base));
+#else
+  const uint32_t chi = (ak << 20) + base;
+#endif
   const uint64_t cbits = (uint64_t)chi << 32;
   double c;
   __builtin_memcpy(&c, &cbits, sizeof c);
