@@ -203,7 +203,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "posit32 (integer VALU)",
+        "dtype": "posit32 (VALU: integer and exact FP64 operations)",
         "data": "synthetic: seeded uniform[-1,1) FP64 -> posit32 RNE, generated on the device",
         "config": {
             "workload": "blocked-LU trailing-update GEMMs at n=16384, nb=256: C -= A*B with "
@@ -279,7 +279,7 @@ def bench_dropin(pga, info, args):
     out = {
         "metric": "posit32 GEMM Gop/s (2MNK/t)", "value": round(ops / t_dev / 1e9, 2), "unit": "Gop/s",
         "n_gpus": 1, "steps": reps, "ms_per_step": round(t_dev * 1e3, 4), "higher_is_better": True,
-        "dtype": "posit32 (integer VALU)",
+        "dtype": "posit32 (VALU: integer and exact FP64 operations)",
         "data": "synthetic: seeded uniform[-1,1) FP64 -> posit32, generated on the device",
         "config": {"workload": "Rgemm NN M=N=K=512, alpha=1, beta=0 (plain drop-in case)",
                    "dropin_ms_per_call": round(t_e2e * 1e3, 4),
@@ -334,7 +334,7 @@ def bench_lu(pga, info, args):
                "kind": "port", "sample": f"oracle Rgetrf n={n_cpu} nb=64, {t_cpu:.2f} s"}
     out = {
         "metric": "posit32 LU Gop/s ((2/3)n^3/t)", "value": round(ops / t_f / 1e9, 2), "unit": "Gop/s",
-        "n_gpus": 1, "higher_is_better": True, "dtype": "posit32 (integer VALU)",
+        "n_gpus": 1, "higher_is_better": True, "dtype": "posit32 (VALU: integer and exact FP64 operations)",
         "data": "synthetic: seeded uniform[-1,1) FP64 -> posit32, generated on the device",
         "config": {"workload": f"Rgetrf_opencl n={n} nb={nb} + Rgetrs (b = A*ones)", "factor_s": round(t_f, 4),
                    "solve_s": round(t_s, 4), "info": info_v, "backward_error": berr,
@@ -396,7 +396,7 @@ def bench_cholesky(pga, info, args):
                "kind": "port", "sample": f"oracle Rpotrf lower n={n_cpu} nb=64, {t_cpu:.2f} s"}
     out = {
         "metric": "posit32 Cholesky Gop/s ((1/3)n^3/t)", "value": round(ops / t_f / 1e9, 2), "unit": "Gop/s",
-        "n_gpus": 1, "higher_is_better": True, "dtype": "posit32 (integer VALU)",
+        "n_gpus": 1, "higher_is_better": True, "dtype": "posit32 (VALU: integer and exact FP64 operations)",
         "data": "synthetic: M*M^T/n + I with M seeded uniform[-1,1), FP64 -> posit32",
         "config": {"workload": f"Rpotrf_opencl lower n={n} nb=64 + Rpotrs", "factor_s": round(t_f, 4),
                    "solve_s": round(t_s, 4), "info": info_v, "backward_error": berr,
@@ -439,7 +439,7 @@ def bench_batched(pga, info, args):
     t_plain = min(_timed(pga, per_call) for _ in range(max(1, reps // 2)))
     out = {
         "metric": "batched posit32 GEMM Gop/s (sum 2MNK/t)", "value": round(total_ops / t_graph / 1e9, 2),
-        "unit": "Gop/s", "n_gpus": 1, "higher_is_better": True, "dtype": "posit32 (integer VALU)",
+        "unit": "Gop/s", "n_gpus": 1, "higher_is_better": True, "dtype": "posit32 (VALU: integer and exact FP64 operations)",
         "data": "synthetic: seeded uniform[-1,1) posit32, shapes from a seeded generator",
         "config": {"workload": "4096 GEMMs, M,N,K uniform on {16,...,512}, alpha=beta=1", "plan": plan.info(),
                    "graph_s": round(t_graph, 5), "per_call_s": round(t_plain, 5),
@@ -519,7 +519,7 @@ def bench_batched_factor(pga, info, args):
         mismatches += 0 if same else 1
     out = {
         "metric": "batched posit32 factorizations/s", "value": round(len(items) / t_graph, 1),
-        "unit": "factorizations/s", "n_gpus": 1, "higher_is_better": True, "dtype": "posit32 (integer VALU)",
+        "unit": "factorizations/s", "n_gpus": 1, "higher_is_better": True, "dtype": "posit32 (VALU: integer and exact FP64 operations)",
         "data": "synthetic: LU inputs uniform[-1,1), Cholesky inputs M*M^T/n + I, posit32",
         "config": {"workload": f"{count} Rgetrf_opencl (nb=64) + {count} lower Rpotrf_opencl (nb=64), n uniform "
                                "in [32, 512], one persistent launch in a hipGraph",
@@ -591,7 +591,7 @@ def bench_blocked(pga, info, args):
                      inf == int(it["info"].to_host()[0])) else 1 for it, (a, b, inf) in zip(items, got))
     out = {
         "metric": "blocked batched posit32 solves/s", "value": round(count / t_plan, 2), "unit": "solves/s",
-        "n_gpus": 1, "higher_is_better": True, "dtype": "posit32 (integer VALU)",
+        "n_gpus": 1, "higher_is_better": True, "dtype": "posit32 (VALU: integer and exact FP64 operations)",
         "data": "synthetic: leading blocks of a uniform[-1,1) matrix and of an SPD M*M^T/n + I matrix, posit32",
         "config": {"workload": f"{count} factor-and-solve tasks, alternating Rgetrf_opencl+Rgetrs and upper "
                                "Rpotrf_opencl+Rpotrs (nb 64), n log-uniform in [513, 4096], 1-16 RHS",
@@ -718,7 +718,7 @@ def bench_serve(pga, info, args):
     sizes = [r["n"] for r in reqs]
     out = {
         "metric": "served posit32 solves/s", "value": cap_s_st["solves_per_s"], "unit": "solves/s", "n_gpus": 1,
-        "higher_is_better": True, "dtype": "posit32 (integer VALU)",
+        "higher_is_better": True, "dtype": "posit32 (VALU: integer and exact FP64 operations)",
         "data": "synthetic: general uniform[-1,1) and SPD M*M^T/n + I blocks, RHS uniform[-1,1), posit32",
         "config": {"workload": f"{len(reqs)} solve requests: 50% LU / 50% Cholesky (upper), n log-uniform "
                                "[64, 4096], 1-16 RHS, 20% re-solves on resident factors",

@@ -15,7 +15,7 @@ namespace gemm {
 // Tiled kernel
 // ---------------------------------------------------------------------------------
 template <int BM, int BN, int BK, int TM, int TN, bool TA, bool TB, bool SKIP0>
-__global__ __launch_bounds__((BM / TM) * (BN / TN)) void rgemm_tiled(
+__global__ __launch_bounds__((BM / TM) * (BN / TN), 2) void rgemm_tiled(
     int m, int n, int k, uint32_t alpha, const uint32_t* __restrict__ A, int lda,
     const uint32_t* __restrict__ B, int ldb, uint32_t beta, uint32_t* __restrict__ C, int ldc,
     int tiles_m, int tri, int urgent, const int* __restrict__ stop) {
@@ -37,7 +37,7 @@ __global__ __launch_bounds__((BM / TM) * (BN / TN)) void rgemm_tiled(
 // caller wants them scheduled, e.g. most expensive first).  Each tile is computed by the
 // same gemm_tile as the plain kernel, so every element gets the same bits as a plain call.
 template <int BM, int BN, int BK, int TM, int TN, bool TA, bool TB>
-__global__ __launch_bounds__((BM / TM) * (BN / TN)) void rgemm_batched_persistent(
+__global__ __launch_bounds__((BM / TM) * (BN / TN), 2) void rgemm_batched_persistent(
     const BatchProblem* __restrict__ probs, const int* __restrict__ tile_start, int nprob, int total_tiles,
     int* __restrict__ counter) {
   PGA_KERNEL_VGPR_FLOOR();

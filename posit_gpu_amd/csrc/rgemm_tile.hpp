@@ -9,6 +9,71 @@
 namespace pga {
 namespace gemm {
 
+// ---------------------------------------------------------------------------------
+// The multiply-add in FP64.  Posit values with regimes up to 27 bits long have at most
+// 28 significant bits, so every operand and accumulator is exact in FP64.  A value x is
+// rounded to the posit grid of its binade by y = (x + C) - C with C = +-2^(e + 52 - nf)
+// (nf = 27 - |k|' fraction bits at scale e, |k|' = k for k >= 0 and -k - 1 below), whose
+// FP64 ulp is the posit quantum: round-to-nearest-even rounds to a multiple of it, ties
+// to the even one, which is the even posit pattern (see fp64_posit_add).  C's high word
+// depends only on x's sign and exponent field, so it comes from a table in LDS indexed by
+// the top 12 bits of x: scales -108 .. 107 get their C; zero gets 1.0 (zero stays zero);
+// every other exponent gets an infinite C, which turns the result into NaN -- the slice
+// is then redone exactly.
+//   product: ph = t*a rounded to FP64 (it may need 56 bits), then rounded to the grid.
+//     FP64's rounding cannot change the posit rounding unless ph lands exactly on a
+//     posit midpoint (midpoints are FP64 numbers and rounding is monotone); that case,
+//     |ph - y| = q/2, is flagged and the slice redone exactly.
+//   sum: c + y rounded to FP64 and then to the grid (fp64_posit_add: a sum that lands on
+//     a midpoint was exact, so ties round as the posit addition does).
+// ---------------------------------------------------------------------------------
+constexpr int ROUND_TABLE_N = 4096;
+static __shared__ uint32_t round_table[ROUND_TABLE_N];
+
+// Fills the table (every lane of the block takes part; a barrier must follow before use).
+// Refilling while other waves read it is harmless: the entries never change.
+PGA_D void round_table_fill(int tid, int nt) {
+  for (int idx = tid; idx < ROUND_TABLE_N; idx += nt) {
+    const uint32_t sign = (uint32_t)(idx >> 11) << 31;
+    const int eb = idx & 0x7FF;
+    const int e = eb - 1023;
+    uint32_t chi;
+    if (eb == 0) {
+      chi = 0x3FF00000u;  // C = 1: a zero stays zero
+    } else if (e >= dec::FAST_MIN && e <= dec::FAST_MAX) {
+      const int ak = (e >> 2) ^ (e >> 31);
+      chi = sign | ((uint32_t)(eb + 25 + ak) << 20);
+    } else {
+      chi = sign | 0x7FF00000u;  // infinite C: NaN result, the slice is redone
+    }
+    round_table[idx] = chi;
+  }
+}
+
+PGA_D uint32_t hi_word(double x) { return (uint32_t)(__double_as_longlong(x) >> 32); }
+
+// y = x rounded to the posit grid (NaN outside the table's scales); chi is C's high word
+PGA_D double grid_round(double x, uint32_t& chi) {
+  chi = round_table[hi_word(x) >> 20];
+  const double c = __hiloint2double((int)chi, 0);
+  return (x + c) - c;
+}
+
+// c <- round(c + round(t * a)); tie is set when the product's FP64 rounding may hide the
+// posit rounding decision
+PGA_D double fp64_mac(double c, double t, double a, bool& tie) {
+  const double ph = t * a;
+  uint32_t chi;
+  const double y = grid_round(ph, chi);
+  const double half_q = __hiloint2double((int)(chi - (53u << 20)), 0);
+  tie |= __builtin_fabs(ph - y) == __builtin_fabs(half_q);
+  uint32_t chi2;
+  return grid_round(c + y, chi2);
+}
+
+// binary exponent of a finite nonzero double
+PGA_D int fp64_scale(double x) { return (int)((hi_word(x) >> 20) & 0x7FFu) - 1023; }
+
 // LDS needed by gemm_tile<BM, BN, BK, ...>, in 32-bit words: three decoded planes
 // (mantissa, scale, sign) of the BK x BM A-slice and of the BK x BN B-side slice, plus
 // the per-wave slice bounds.
@@ -42,8 +107,10 @@ __device__ __forceinline__ void gemm_tile_impl(int m, int n, int k, uint32_t alp
   const int ty = tid / NX;
   const int wave = tid / 64;
 
-  // ---- accumulators ----
-  Val acc[TM][TN];
+  round_table_fill(tid, NT);  // visible after the first slice's barrier
+
+  // ---- accumulators: posit values in FP64 (NaR as NaN) ----
+  double acc[TM][TN];
 #pragma unroll
   for (int r = 0; r < TM; ++r) {
 #pragma unroll
@@ -54,7 +121,7 @@ __device__ __forceinline__ void gemm_tile_impl(int m, int n, int k, uint32_t alp
         uint32_t cij = C[i + (size_t)j * ldc];
         init = (beta == P_ONE) ? cij : pga::mul(beta, cij);
       }
-      acc[r][c] = dec::decode(init);
+      acc[r][c] = pga::to_double(init);
     }
   }
 
@@ -78,12 +145,12 @@ __device__ __forceinline__ void gemm_tile_impl(int m, int n, int k, uint32_t alp
       int i = i0 + ii, l = kt + ll;
       uint32_t v = 0;
       if (i < m && l < k) v = TA ? A[l + (size_t)i * lda] : A[i + (size_t)l * lda];
-      Val d = dec::decode(v);
-      sAm[ll][ii] = d.m;
-      sAs[ll][ii] = d.sc;
-      sAg[ll][ii] = d.sg;
-      amax = max(amax, d.sc);
-      if (v != 0) amin = min(amin, d.sc);
+      const double dv = pga::to_double(v);  // exact; NaR -> NaN
+      sAm[ll][ii] = (uint32_t)__double_as_longlong(dv);
+      sAs[ll][ii] = (int)(__double_as_longlong(dv) >> 32);
+      const int sc = (v == 0) ? dec::ZERO_SC : (v == P_NAR ? dec::NAR_SC : fp64_scale(dv));
+      amax = max(amax, sc);
+      if (v != 0) amin = min(amin, sc);
     }
 #pragma unroll
     for (int q = 0; q < (BN * BK) / NT; ++q) {
@@ -102,12 +169,12 @@ __device__ __forceinline__ void gemm_tile_impl(int m, int n, int k, uint32_t alp
       // alpha*b; with SKIP0 a zero b stays zero (it marks a skipped step even if alpha is NaR)
       if (!SUM_FIRST && !(SKIP0 && v == 0))
         v = alpha_one ? v : (alpha_mone ? pga::neg(v) : pga::mul(alpha, v));
-      Val d = dec::decode(v);
-      sTm[ll][jj] = d.m;
-      sTs[ll][jj] = d.sc;
-      sTg[ll][jj] = d.sg;
-      tmax = max(tmax, d.sc);
-      if (v != 0) tmin = min(tmin, d.sc);
+      const double dv = pga::to_double(v);
+      sTm[ll][jj] = (uint32_t)__double_as_longlong(dv);
+      sTs[ll][jj] = (int)(__double_as_longlong(dv) >> 32);
+      const int sc = (v == 0) ? dec::ZERO_SC : (v == P_NAR ? dec::NAR_SC : fp64_scale(dv));
+      tmax = max(tmax, sc);
+      if (v != 0) tmin = min(tmin, sc);
     }
     // wave reductions of the slice bounds
 #pragma unroll
@@ -135,70 +202,51 @@ __device__ __forceinline__ void gemm_tile_impl(int m, int n, int k, uint32_t alp
     bool fast = (amax + tmax + 1 <= U);
     if (amin != INT32_MAX && tmin != INT32_MAX) fast = fast && (amin + tmin >= dec::FAST_MIN);
     // accumulators of this lane (zeros have scales far below every bound)
-    int cmax = INT32_MIN, cmin = INT32_MAX;
+    int cmax = INT32_MIN;
 #pragma unroll
     for (int r = 0; r < TM; ++r)
 #pragma unroll
       for (int c = 0; c < TN; ++c) {
-        cmax = max(cmax, acc[r][c].sc + 1);
-        if (acc[r][c].m != 0) cmin = min(cmin, acc[r][c].sc);
+        // NaN (NaR) has exponent field 2047: above every bound
+        const int sc = (acc[r][c] == 0.0) ? dec::ZERO_SC : fp64_scale(acc[r][c]);
+        cmax = max(cmax, sc + 1);
       }
     fast = fast && (cmax <= U);
     const bool wave_fast = __all(fast);
-    // Every operand of the slice is a multiple of 2^g, g = (smallest nonzero scale) - 27
-    // (a posit has at most 27 fraction bits), and sums and roundings keep that, so a
-    // nonzero sum is at least 2^g: with g >= FAST_MIN no step can leave the fast range
-    // downwards and the per-step check is dropped.
-    const int pmin = (amin != INT32_MAX && tmin != INT32_MAX) ? amin + tmin : INT32_MAX;
-    const bool wave_nocheck = __all(min(pmin, cmin) >= dec::FAST_MIN + 27);
 
     const int lmax = min(BK, k - kt);
     bool redo = !wave_fast;
     if (wave_fast) {
-      Val saved[TM][TN];
+      double saved[TM][TN];
 #pragma unroll
       for (int r = 0; r < TM; ++r)
 #pragma unroll
         for (int c = 0; c < TN; ++c) saved[r][c] = acc[r][c];
-      bool low = false;
-      auto step = [&](int l, auto check) {
-        uint32_t am[TM], ag[TM], tm[TN], tg[TN];
-        int as_[TM], ts[TN];
+      bool tie = false;
+      auto step = [&](int l) {
+        double ad[TM], td[TN];
 #pragma unroll
-        for (int r = 0; r < TM; ++r) {
-          am[r] = sAm[l][tx + r * NX];
-          as_[r] = sAs[l][tx + r * NX];
-          ag[r] = sAg[l][tx + r * NX];
-        }
+        for (int r = 0; r < TM; ++r) ad[r] = __hiloint2double(sAs[l][tx + r * NX], (int)sAm[l][tx + r * NX]);
 #pragma unroll
-        for (int c = 0; c < TN; ++c) {
-          tm[c] = sTm[l][ty + c * NY];
-          ts[c] = sTs[l][ty + c * NY];
-          tg[c] = sTg[l][ty + c * NY];
-        }
+        for (int c = 0; c < TN; ++c) td[c] = __hiloint2double(sTs[l][ty + c * NY], (int)sTm[l][ty + c * NY]);
 #pragma unroll
         for (int r = 0; r < TM; ++r)
 #pragma unroll
-          for (int c = 0; c < TN; ++c)
-            low |= mac<decltype(check)::value>(acc[r][c], tm[c], ts[c], tg[c], am[r], as_[r], ag[r]);
+          for (int c = 0; c < TN; ++c) acc[r][c] = fp64_mac(acc[r][c], td[c], ad[r], tie);
       };
       // two steps per iteration: the loop then carries no register rotation
-      if (wave_nocheck) {
-        int l = 0;
-        for (; l + 1 < lmax; l += 2) {
-          step(l, std::false_type());
-          step(l + 1, std::false_type());
-        }
-        if (l < lmax) step(l, std::false_type());
-      } else {
-        int l = 0;
-        for (; l + 1 < lmax; l += 2) {
-          step(l, std::true_type());
-          step(l + 1, std::true_type());
-        }
-        if (l < lmax) step(l, std::true_type());
+      int l = 0;
+      for (; l + 1 < lmax; l += 2) {
+        step(l);
+        step(l + 1);
       }
-      if (__any(low)) {
+      if (l < lmax) step(l);
+      bool bad = tie;
+#pragma unroll
+      for (int r = 0; r < TM; ++r)
+#pragma unroll
+        for (int c = 0; c < TN; ++c) bad |= acc[r][c] != acc[r][c];  // NaN: a scale left the table
+      if (__any(bad)) {
         redo = true;
 #pragma unroll
         for (int r = 0; r < TM; ++r)
@@ -212,24 +260,17 @@ __device__ __forceinline__ void gemm_tile_impl(int m, int n, int k, uint32_t alp
       for (int r = 0; r < TM; ++r) {
 #pragma unroll
         for (int c = 0; c < TN; ++c) {
-          uint32_t pc = dec::encode(acc[r][c]);
+          uint32_t pc = pga::from_double(acc[r][c]);  // exact; NaN -> NaR
           for (int l = 0; l < lmax; ++l) {
-            Val a{sAm[l][tx + r * NX], sAs[l][tx + r * NX], sAg[l][tx + r * NX]};
-            Val t{sTm[l][ty + c * NY], sTs[l][ty + c * NY], sTg[l][ty + c * NY]};
-            uint32_t tp = dec::encode(t);
+            const uint32_t ap = pga::from_double(__hiloint2double(sAs[l][tx + r * NX], (int)sAm[l][tx + r * NX]));
+            const uint32_t tp = pga::from_double(__hiloint2double(sTs[l][ty + c * NY], (int)sTm[l][ty + c * NY]));
             if (SKIP0 && tp == 0) continue;  // Rsyrk.cpp:149,169 skip zero multipliers
-            pc = pga::add(pc, pga::mul(tp, dec::encode(a)));
+            pc = pga::add(pc, pga::mul(tp, ap));
           }
-          acc[r][c] = dec::decode(pc);
+          acc[r][c] = pga::to_double(pc);
         }
       }
     }
-    // keep zero accumulators at the canonical zero scale (sums of zeros drift downwards)
-#pragma unroll
-    for (int r = 0; r < TM; ++r)
-#pragma unroll
-      for (int c = 0; c < TN; ++c)
-        if (acc[r][c].sc < dec::ZERO_LIMIT) acc[r][c].sc = dec::ZERO_SC;
     __syncthreads();
   }
 
@@ -241,7 +282,7 @@ __device__ __forceinline__ void gemm_tile_impl(int m, int n, int k, uint32_t alp
       int i = i0 + tx + r * NX, j = j0 + ty + c * NY;
       bool keep = (tri == 0) || (tri == 1 ? i >= j : i <= j);
       if (i < m && j < n && keep) {
-        uint32_t v = dec::encode(acc[r][c]);
+        uint32_t v = pga::from_double(acc[r][c]);
         if (SUM_FIRST) {
           uint32_t at = pga::mul(alpha, v);
           v = (beta == 0) ? at : pga::add(at, pga::mul(beta, C[i + (size_t)j * ldc]));
