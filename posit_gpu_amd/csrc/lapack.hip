@@ -1186,7 +1186,7 @@ static hipError_t potrf_overlap(const Ctx& c, bool upper, int n, uint32_t* a, in
 // T divides nb, so each T x T tile is wholly one or the other.  One workgroup per tile of
 // the lower trapezoid, tiles numbered column by column.
 template <int T>
-__global__ __launch_bounds__(WG) void k_potrf_trail(int m, int ncols, int kb, const uint32_t* __restrict__ L,
+__global__ __launch_bounds__(WG, 2) void k_potrf_trail(int m, int ncols, int kb, const uint32_t* __restrict__ L,
                                                     int lda, uint32_t* __restrict__ C, int nb, int urgent,
                                                     const int* stop) {
   PGA_KERNEL_VGPR_FLOOR();
