@@ -294,16 +294,8 @@ __global__ __launch_bounds__(WG) void k_trsm_right(int m, int n, uint32_t alpha,
 // applies alpha to column k.  Every element sees its terms in the reference's order; the
 // (row, column) pairs of a step are independent.  A workgroup owns `rows` rows.
 template <bool UPPER>
-__global__ __launch_bounds__(WG) void k_trsm_right_sweep(int m, int n, uint32_t alpha, int nounit,
-                                                         const uint32_t* __restrict__ A, int lda,
-                                                         uint32_t* __restrict__ B, int ldb, int rows,
-                                                         const int* stop) {
-  PGA_KERNEL_VGPR_FLOOR();
-  PGA_URGENT();
-  const int r0 = blockIdx.x * rows;
-  if (r0 >= m || stopped(stop)) return;
-  const int R = min(rows, m - r0);
-  uint32_t* b = B + r0;
+__device__ void right_sweep_rows(int R, int n, uint32_t alpha, int nounit, const uint32_t* __restrict__ A, int lda,
+                                 uint32_t* __restrict__ b, int ldb) {
   const int tid = threadIdx.x;
   for (int s = 0; s < n; ++s) {
     const int k = UPPER ? n - 1 - s : s;
@@ -324,6 +316,74 @@ __global__ __launch_bounds__(WG) void k_trsm_right_sweep(int m, int n, uint32_t 
     }
     __syncthreads();
   }
+}
+
+template <bool UPPER>
+__global__ __launch_bounds__(WG) void k_trsm_right_sweep(int m, int n, uint32_t alpha, int nounit,
+                                                         const uint32_t* __restrict__ A, int lda,
+                                                         uint32_t* __restrict__ B, int ldb, int rows,
+                                                         const int* stop) {
+  PGA_KERNEL_VGPR_FLOOR();
+  PGA_URGENT();
+  const int r0 = blockIdx.x * rows;
+  if (r0 >= m || stopped(stop)) return;
+  right_sweep_rows<UPPER>(min(rows, m - r0), n, alpha, nounit, A, lda, B + r0, ldb);
+}
+
+// The lower Cholesky panel solve, B := B L^-T with L non-unit lower n x n, alpha = 1
+// (Rtrsm.cpp:263-285), on RSW_ROWS rows of B per workgroup with B's rows and L held in
+// LDS as FP64 values: step k scales column k by 1/l_kk (the reciprocals formed once, in
+// parallel), then subtracts l_jk * b_k from every later column j (skipped when l_jk is
+// zero), each product and sum rounded to the posit grid as in the GEMM (gemm::fp64_mac).
+// A product on a posit midpoint, a scale outside the table or a NaR leaves the block
+// untouched in memory and the rows are solved again with the pattern-level sweep.
+constexpr int RSW_ROWS = 16, RSW_NMAX = 64;
+
+__global__ __launch_bounds__(WG) void k_trsm_rltn_fp64(int m, int n, const uint32_t* __restrict__ A, int lda,
+                                                       uint32_t* __restrict__ B, int ldb, const int* stop) {
+  PGA_KERNEL_VGPR_FLOOR();
+  PGA_URGENT();
+  __shared__ double sL[RSW_NMAX][RSW_NMAX];  // sL[k][j] = l_jk
+  __shared__ double sB[RSW_NMAX][RSW_ROWS];  // sB[column][row]
+  __shared__ double sR[RSW_NMAX];            // 1 / l_kk
+  const int r0 = blockIdx.x * RSW_ROWS;
+  if (r0 >= m || stopped(stop)) return;
+  const int R = min(RSW_ROWS, m - r0);
+  const int tid = threadIdx.x;
+  uint32_t* b = B + r0;
+  gemm::round_table_fill(tid, WG);
+  for (int e = tid; e < n * n; e += WG) {
+    const int k = e / n, j = e % n;
+    sL[k][j] = j >= k ? to_double(A[j + (size_t)k * lda]) : 0.0;
+  }
+  for (int e = tid; e < R * n; e += WG) sB[e / R][e % R] = to_double(b[e % R + (size_t)(e / R) * ldb]);
+  if (tid < n) sR[tid] = to_double(div(P_ONE, A[tid + (size_t)tid * lda]));
+  __syncthreads();
+  bool tie = false;
+  for (int k = 0; k < n; ++k) {
+    if (tid < R) {
+      uint32_t chi;
+      const double ph = sR[k] * sB[k][tid];
+      const double y = gemm::grid_round(ph, chi);
+      tie |= __builtin_fabs(ph - y) == __builtin_fabs(__hiloint2double((int)(chi - (53u << 20)), 0));
+      sB[k][tid] = y;
+    }
+    __syncthreads();
+    const int cnt = n - 1 - k;
+    for (int e = tid; e < R * cnt; e += WG) {
+      const int i = e % R, j = k + 1 + e / R;
+      const double t = sL[k][j];
+      if (t != 0.0) sB[j][i] = gemm::fp64_mac(sB[j][i], -t, sB[k][i], tie);
+    }
+    __syncthreads();
+  }
+  bool bad = tie;
+  for (int e = tid; e < R * n; e += WG) bad |= sB[e / R][e % R] != sB[e / R][e % R];  // NaN
+  if (__syncthreads_or(bad)) {
+    right_sweep_rows<false>(R, n, P_ONE, 1, A, lda, b, ldb);
+    return;
+  }
+  for (int e = tid; e < R * n; e += WG) b[e % R + (size_t)(e / R) * ldb] = from_double(sB[e / R][e % R]);
 }
 
 // B = 0 (alpha == 0, Rtrsm.cpp:101-108)
@@ -854,6 +914,11 @@ static hipError_t trsm(const Ctx& c, bool left, bool upper, bool trans, bool nou
     // enough row blocks for every CU twice where m allows, at least 4 rows each
     const int rows = (int)std::min<long>(256, std::max<long>(4, (m + 2L * runtime().cus - 1) / (2L * runtime().cus)));
     const unsigned grid = (unsigned)((m + rows - 1) / rows);
+    if (!upper && nounit && alpha == P_ONE && n <= RSW_NMAX) {
+      hipLaunchKernelGGL(k_trsm_rltn_fp64, dim3((unsigned)((m + RSW_ROWS - 1) / RSW_ROWS)), dim3(WG), 0, c.s, m, n,
+                         A, lda, B, ldb, stop);
+      return hipGetLastError();
+    }
     if (upper)
       hipLaunchKernelGGL((k_trsm_right_sweep<true>), dim3(grid), dim3(WG), 0, c.s, m, n, alpha, (int)nounit, A,
                          lda, B, ldb, rows, stop);
