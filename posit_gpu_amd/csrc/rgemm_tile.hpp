@@ -82,9 +82,246 @@ struct TileLds {
   static constexpr int words = 3 * BK * BM + 3 * BK * BN + 16 * 4;
 };
 
-// One BM x BN tile of C at (i0, j0): the body shared by the plain and the batched kernel.
+// The tile body with the integer decoded multiply-add (gemm::mac): used for one element
+// per lane, where the FP64 form would wait on two LDS table reads per step with nothing
+// else to issue.
 template <int BM, int BN, int BK, int TM, int TN, bool TA, bool TB, bool SKIP0>
-__device__ __forceinline__ void gemm_tile_impl(int m, int n, int k, uint32_t alpha,
+__device__ __forceinline__ void gemm_tile_impl_int(int m, int n, int k, uint32_t alpha,
+                                               const uint32_t* __restrict__ A, int lda,
+                                               const uint32_t* __restrict__ B, int ldb, uint32_t beta,
+                                               uint32_t* __restrict__ C, int ldc, int i0, int j0, int tri,
+                                               uint32_t (*__restrict__ sAm)[BM], int (*__restrict__ sAs)[BM],
+                                               uint32_t (*__restrict__ sAg)[BM], uint32_t (*__restrict__ sTm)[BN],
+                                               int (*__restrict__ sTs)[BN], uint32_t (*__restrict__ sTg)[BN],
+                                               int (*__restrict__ sBound)[4]) {
+  constexpr int NX = BM / TM;
+  constexpr int NY = BN / TN;
+  constexpr int NT = NX * NY;
+  constexpr int NW = NT / 64;
+  constexpr bool SUM_FIRST = TA;  // TN and TT sum a*b from zero, then scale
+  constexpr int U = dec::FAST_MAX - growth_bits(BK);
+  static_assert((BM * BK) % NT == 0 && (BN * BK) % NT == 0, "tile loads");
+  static_assert(NT % 64 == 0, "whole waves");
+  static_assert(NW <= 16, "slice-bound slots in TileLds");
+
+  const int tid = threadIdx.x;
+  const int tx = tid % NX;
+  const int ty = tid / NX;
+  const int wave = tid / 64;
+
+  // ---- accumulators ----
+  Val acc[TM][TN];
+#pragma unroll
+  for (int r = 0; r < TM; ++r) {
+#pragma unroll
+    for (int c = 0; c < TN; ++c) {
+      int i = i0 + tx + r * NX, j = j0 + ty + c * NY;
+      uint32_t init = 0;
+      if (!SUM_FIRST && beta != 0 && i < m && j < n) {
+        uint32_t cij = C[i + (size_t)j * ldc];
+        init = (beta == P_ONE) ? cij : pga::mul(beta, cij);
+      }
+      acc[r][c] = dec::decode(init);
+    }
+  }
+
+  const bool alpha_one = (alpha == P_ONE);
+  const bool alpha_mone = (alpha == P_MONE);
+
+  for (int kt = 0; kt < k; kt += BK) {
+    // ---- stage the A slice (BM x BK) and the B-side slice (BK x BN), decoded ----
+    int amax = INT32_MIN, amin = INT32_MAX, tmax = INT32_MIN, tmin = INT32_MAX;
+#pragma unroll
+    for (int q = 0; q < (BM * BK) / NT; ++q) {
+      int e = tid + q * NT;
+      int ii, ll;
+      if (!TA) {  // A is m x k: walk rows fastest (coalesced)
+        ii = e % BM;
+        ll = e / BM;
+      } else {  // A is k x m: walk l fastest
+        ll = e % BK;
+        ii = e / BK;
+      }
+      int i = i0 + ii, l = kt + ll;
+      uint32_t v = 0;
+      if (i < m && l < k) v = TA ? A[l + (size_t)i * lda] : A[i + (size_t)l * lda];
+      Val d = dec::decode(v);
+      sAm[ll][ii] = d.m;
+      sAs[ll][ii] = d.sc;
+      sAg[ll][ii] = d.sg;
+      amax = max(amax, d.sc);
+      if (v != 0) amin = min(amin, d.sc);
+    }
+#pragma unroll
+    for (int q = 0; q < (BN * BK) / NT; ++q) {
+      int e = tid + q * NT;
+      int jj, ll;
+      if (!TB) {  // B is k x n: walk l fastest
+        ll = e % BK;
+        jj = e / BK;
+      } else {  // B is n x k: walk columns fastest
+        jj = e % BN;
+        ll = e / BN;
+      }
+      int j = j0 + jj, l = kt + ll;
+      uint32_t v = 0;
+      if (j < n && l < k) v = TB ? B[j + (size_t)l * ldb] : B[l + (size_t)j * ldb];
+      // alpha*b; with SKIP0 a zero b stays zero (it marks a skipped step even if alpha is NaR)
+      if (!SUM_FIRST && !(SKIP0 && v == 0))
+        v = alpha_one ? v : (alpha_mone ? pga::neg(v) : pga::mul(alpha, v));
+      Val d = dec::decode(v);
+      sTm[ll][jj] = d.m;
+      sTs[ll][jj] = d.sc;
+      sTg[ll][jj] = d.sg;
+      tmax = max(tmax, d.sc);
+      if (v != 0) tmin = min(tmin, d.sc);
+    }
+    // wave reductions of the slice bounds
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      amax = max(amax, __shfl_xor(amax, off));
+      amin = min(amin, __shfl_xor(amin, off));
+      tmax = max(tmax, __shfl_xor(tmax, off));
+      tmin = min(tmin, __shfl_xor(tmin, off));
+    }
+    if ((tid & 63) == 0) {
+      sBound[wave][0] = amax;
+      sBound[wave][1] = amin;
+      sBound[wave][2] = tmax;
+      sBound[wave][3] = tmin;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+      amax = max(amax, sBound[w][0]);
+      amin = min(amin, sBound[w][1]);
+      tmax = max(tmax, sBound[w][2]);
+      tmin = min(tmin, sBound[w][3]);
+    }
+    // products of nonzero operands lie in [amin + tmin, amax + tmax + 1]
+    bool fast = (amax + tmax + 1 <= U);
+    if (amin != INT32_MAX && tmin != INT32_MAX) fast = fast && (amin + tmin >= dec::FAST_MIN);
+    // accumulators of this lane (zeros have scales far below every bound)
+    int cmax = INT32_MIN, cmin = INT32_MAX;
+#pragma unroll
+    for (int r = 0; r < TM; ++r)
+#pragma unroll
+      for (int c = 0; c < TN; ++c) {
+        cmax = max(cmax, acc[r][c].sc + 1);
+        if (acc[r][c].m != 0) cmin = min(cmin, acc[r][c].sc);
+      }
+    fast = fast && (cmax <= U);
+    const bool wave_fast = __all(fast);
+    // Every operand of the slice is a multiple of 2^g, g = (smallest nonzero scale) - 27
+    // (a posit has at most 27 fraction bits), and sums and roundings keep that, so a
+    // nonzero sum is at least 2^g: with g >= FAST_MIN no step can leave the fast range
+    // downwards and the per-step check is dropped.
+    const int pmin = (amin != INT32_MAX && tmin != INT32_MAX) ? amin + tmin : INT32_MAX;
+    const bool wave_nocheck = __all(min(pmin, cmin) >= dec::FAST_MIN + 27);
+
+    const int lmax = min(BK, k - kt);
+    bool redo = !wave_fast;
+    if (wave_fast) {
+      Val saved[TM][TN];
+#pragma unroll
+      for (int r = 0; r < TM; ++r)
+#pragma unroll
+        for (int c = 0; c < TN; ++c) saved[r][c] = acc[r][c];
+      bool low = false;
+      auto step = [&](int l, auto check) {
+        uint32_t am[TM], ag[TM], tm[TN], tg[TN];
+        int as_[TM], ts[TN];
+#pragma unroll
+        for (int r = 0; r < TM; ++r) {
+          am[r] = sAm[l][tx + r * NX];
+          as_[r] = sAs[l][tx + r * NX];
+          ag[r] = sAg[l][tx + r * NX];
+        }
+#pragma unroll
+        for (int c = 0; c < TN; ++c) {
+          tm[c] = sTm[l][ty + c * NY];
+          ts[c] = sTs[l][ty + c * NY];
+          tg[c] = sTg[l][ty + c * NY];
+        }
+#pragma unroll
+        for (int r = 0; r < TM; ++r)
+#pragma unroll
+          for (int c = 0; c < TN; ++c)
+            low |= mac<decltype(check)::value>(acc[r][c], tm[c], ts[c], tg[c], am[r], as_[r], ag[r]);
+      };
+      // two steps per iteration: the loop then carries no register rotation
+      if (wave_nocheck) {
+        int l = 0;
+        for (; l + 1 < lmax; l += 2) {
+          step(l, std::false_type());
+          step(l + 1, std::false_type());
+        }
+        if (l < lmax) step(l, std::false_type());
+      } else {
+        int l = 0;
+        for (; l + 1 < lmax; l += 2) {
+          step(l, std::true_type());
+          step(l + 1, std::true_type());
+        }
+        if (l < lmax) step(l, std::true_type());
+      }
+      if (__any(low)) {
+        redo = true;
+#pragma unroll
+        for (int r = 0; r < TM; ++r)
+#pragma unroll
+          for (int c = 0; c < TN; ++c) acc[r][c] = saved[r][c];
+      }
+    }
+    if (redo) {
+      // exact path for this slice: pattern-level SoftPosit operations
+#pragma unroll
+      for (int r = 0; r < TM; ++r) {
+#pragma unroll
+        for (int c = 0; c < TN; ++c) {
+          uint32_t pc = dec::encode(acc[r][c]);
+          for (int l = 0; l < lmax; ++l) {
+            Val a{sAm[l][tx + r * NX], sAs[l][tx + r * NX], sAg[l][tx + r * NX]};
+            Val t{sTm[l][ty + c * NY], sTs[l][ty + c * NY], sTg[l][ty + c * NY]};
+            uint32_t tp = dec::encode(t);
+            if (SKIP0 && tp == 0) continue;  // Rsyrk.cpp:149,169 skip zero multipliers
+            pc = pga::add(pc, pga::mul(tp, dec::encode(a)));
+          }
+          acc[r][c] = dec::decode(pc);
+        }
+      }
+    }
+    // keep zero accumulators at the canonical zero scale (sums of zeros drift downwards)
+#pragma unroll
+    for (int r = 0; r < TM; ++r)
+#pragma unroll
+      for (int c = 0; c < TN; ++c)
+        if (acc[r][c].sc < dec::ZERO_LIMIT) acc[r][c].sc = dec::ZERO_SC;
+    __syncthreads();
+  }
+
+  // ---- epilogue and store ----
+#pragma unroll
+  for (int r = 0; r < TM; ++r) {
+#pragma unroll
+    for (int c = 0; c < TN; ++c) {
+      int i = i0 + tx + r * NX, j = j0 + ty + c * NY;
+      bool keep = (tri == 0) || (tri == 1 ? i >= j : i <= j);
+      if (i < m && j < n && keep) {
+        uint32_t v = dec::encode(acc[r][c]);
+        if (SUM_FIRST) {
+          uint32_t at = pga::mul(alpha, v);
+          v = (beta == 0) ? at : pga::add(at, pga::mul(beta, C[i + (size_t)j * ldc]));
+        }
+        C[i + (size_t)j * ldc] = v;
+      }
+    }
+  }
+}
+
+// The tile body with the FP64 multiply-add (fp64_mac), for several elements per lane.
+template <int BM, int BN, int BK, int TM, int TN, bool TA, bool TB, bool SKIP0>
+__device__ __forceinline__ void gemm_tile_impl_fp64(int m, int n, int k, uint32_t alpha,
                                                const uint32_t* __restrict__ A, int lda,
                                                const uint32_t* __restrict__ B, int ldb, uint32_t beta,
                                                uint32_t* __restrict__ C, int ldc, int i0, int j0, int tri,
@@ -291,6 +528,25 @@ __device__ __forceinline__ void gemm_tile_impl(int m, int n, int k, uint32_t alp
       }
     }
   }
+}
+
+// One BM x BN tile of C at (i0, j0): the body shared by the plain and the batched kernel.
+// Both forms give every element the reference's bits.
+template <int BM, int BN, int BK, int TM, int TN, bool TA, bool TB, bool SKIP0>
+__device__ __forceinline__ void gemm_tile_impl(int m, int n, int k, uint32_t alpha,
+                                               const uint32_t* __restrict__ A, int lda,
+                                               const uint32_t* __restrict__ B, int ldb, uint32_t beta,
+                                               uint32_t* __restrict__ C, int ldc, int i0, int j0, int tri,
+                                               uint32_t (*__restrict__ sAm)[BM], int (*__restrict__ sAs)[BM],
+                                               uint32_t (*__restrict__ sAg)[BM], uint32_t (*__restrict__ sTm)[BN],
+                                               int (*__restrict__ sTs)[BN], uint32_t (*__restrict__ sTg)[BN],
+                                               int (*__restrict__ sBound)[4]) {
+  if constexpr (TM * TN >= 4)
+    gemm_tile_impl_fp64<BM, BN, BK, TM, TN, TA, TB, SKIP0>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, i0, j0, tri,
+                                                           sAm, sAs, sAg, sTm, sTs, sTg, sBound);
+  else
+    gemm_tile_impl_int<BM, BN, BK, TM, TN, TA, TB, SKIP0>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, i0, j0, tri,
+                                                          sAm, sAs, sAg, sTm, sTs, sTg, sBound);
 }
 
 // The tile with its own LDS planes (separate objects), for kernels that run one tile shape.
