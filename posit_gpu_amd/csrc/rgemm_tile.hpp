@@ -83,8 +83,8 @@ struct TileLds {
 };
 
 // The tile body with the integer decoded multiply-add (gemm::mac): used for one element
-// per lane, where the FP64 form would wait on two LDS table reads per step with nothing
-// else to issue.
+// per lane and short K, where the FP64 form would wait on two LDS table reads per step
+// with nothing else to issue and fill its table for little work.
 template <int BM, int BN, int BK, int TM, int TN, bool TA, bool TB, bool SKIP0>
 __device__ __forceinline__ void gemm_tile_impl_int(int m, int n, int k, uint32_t alpha,
                                                const uint32_t* __restrict__ A, int lda,
@@ -541,7 +541,10 @@ __device__ __forceinline__ void gemm_tile_impl(int m, int n, int k, uint32_t alp
                                                uint32_t (*__restrict__ sAg)[BM], uint32_t (*__restrict__ sTm)[BN],
                                                int (*__restrict__ sTs)[BN], uint32_t (*__restrict__ sTg)[BN],
                                                int (*__restrict__ sBound)[4]) {
-  if constexpr (TM * TN >= 4)
+  // one element per lane: the FP64 form pays for its table and its LDS round trips only
+  // over long K loops (a 512^3 GEMM: 1.45x the integer form; the LU panel's node updates,
+  // K <= 128: 0.35x)
+  if (TM * TN >= 4 || k >= 256)
     gemm_tile_impl_fp64<BM, BN, BK, TM, TN, TA, TB, SKIP0>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, i0, j0, tri,
                                                            sAm, sAs, sAg, sTm, sTs, sTg, sBound);
   else
