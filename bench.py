@@ -173,7 +173,7 @@ def main():
     macs = float(n0) * n0 * K
     peak = info["int32_valu_peak"]
     roof = {"bound": "valu", "achieved": None, "peak": round(peak / 1e12, 3), "unit": "Tlane-op/s",
-            "frac": None, "traffic": None, "kernel": "rgemm_tiled NN 64x64x16",
+            "frac": None, "traffic": None, "kernel": "rgemm_tiled NN 64x32x16",
             "avg_ms": round(per_shape_ms[0], 4), "gops": round(2 * macs / t_dom / 1e9, 2)}
     if os.path.exists(COUNTS):
         # rocprofv3 counts of one launch of this kernel at M = N = 16384, K = 256:

@@ -15,7 +15,7 @@ namespace gemm {
 // Tiled kernel
 // ---------------------------------------------------------------------------------
 template <int BM, int BN, int BK, int TM, int TN, bool TA, bool TB, bool SKIP0>
-__global__ __launch_bounds__((BM / TM) * (BN / TN), 2) void rgemm_tiled(
+__global__ __launch_bounds__((BM / TM) * (BN / TN), 3) void rgemm_tiled(
     int m, int n, int k, uint32_t alpha, const uint32_t* __restrict__ A, int lda,
     const uint32_t* __restrict__ B, int ldb, uint32_t beta, uint32_t* __restrict__ C, int ldc,
     int tiles_m, int tri, int urgent, const int* __restrict__ stop) {
@@ -124,7 +124,7 @@ template <int BM, int TM, bool TA, bool TB, bool SKIP0>
 static hipError_t launch_tiled_cfg(int m, int n, int k, uint32_t alpha, const uint32_t* A, int lda,
                                    const uint32_t* B, int ldb, uint32_t beta, uint32_t* C, int ldc,
                                    hipStream_t s, int tri, int urgent, const int* stop) {
-  constexpr int BN = BM, TN = TM, BK = 16;
+  constexpr int BN = BM == 64 ? 32 : BM, TN = BM == 64 ? 2 : TM, BK = 16;
   constexpr int NT = (BM / TM) * (BN / TN);
   int tm = (m + BM - 1) / BM, tn = (n + BN - 1) / BN;
   long long blocks = (long long)tm * tn;
