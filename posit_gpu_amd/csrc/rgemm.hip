@@ -37,7 +37,7 @@ __global__ __launch_bounds__((BM / TM) * (BN / TN), 3) void rgemm_tiled(
 // caller wants them scheduled, e.g. most expensive first).  Each tile is computed by the
 // same gemm_tile as the plain kernel, so every element gets the same bits as a plain call.
 template <int BM, int BN, int BK, int TM, int TN, bool TA, bool TB>
-__global__ __launch_bounds__((BM / TM) * (BN / TN), 2) void rgemm_batched_persistent(
+__global__ __launch_bounds__((BM / TM) * (BN / TN), 3) void rgemm_batched_persistent(
     const BatchProblem* __restrict__ probs, const int* __restrict__ tile_start, int nprob, int total_tiles,
     int* __restrict__ counter) {
   PGA_KERNEL_VGPR_FLOOR();
@@ -184,9 +184,11 @@ static hipError_t dispatch(bool ta, bool tb, int m, int n, int k, uint32_t alpha
 template <int BM, int TM, bool TA, bool TB>
 static hipError_t launch_persistent(int grid, const BatchProblem* probs, const int* tile_start, int nprob,
                                     int total_tiles, int* counter, hipStream_t s) {
-  constexpr int NT = (BM / TM) * (BM / TM);
+  // the 64 tile is 64 x 32 (4 x 2 per lane), as in the plain kernel
+  constexpr int BN = BM == 64 ? 32 : BM, TN = BM == 64 ? 2 : TM;
+  constexpr int NT = (BM / TM) * (BN / TN);
   hipLaunchKernelGGL(reset_counter, dim3(1), dim3(64), 0, s, counter);
-  hipLaunchKernelGGL((rgemm_batched_persistent<BM, BM, 16, TM, TM, TA, TB>), dim3(grid), dim3(NT), 0, s, probs,
+  hipLaunchKernelGGL((rgemm_batched_persistent<BM, BN, 16, TM, TN, TA, TB>), dim3(grid), dim3(NT), 0, s, probs,
                      tile_start, nprob, total_tiles, counter);
   return hipGetLastError();
 }
@@ -201,7 +203,11 @@ static hipError_t launch_persistent_tile(int tile, int grid, const BatchProblem*
 
 }  // namespace gemm
 
-int batched_tiles(int m, int n, int tile) { return ((m + tile - 1) / tile) * ((n + tile - 1) / tile); }
+// tiles of one problem for the persistent kernel's tile size `tile` (64 means 64 x 32)
+int batched_tiles(int m, int n, int tile) {
+  const int tn = tile == 64 ? 32 : tile;
+  return ((m + tile - 1) / tile) * ((n + tn - 1) / tn);
+}
 
 hipError_t batched_launch(bool ta, bool tb, int tile, int grid, const BatchProblem* probs, const int* tile_start,
                           int nprob, int total_tiles, int* counter, hipStream_t s) {
