@@ -1251,7 +1251,7 @@ static hipError_t potrf_overlap(const Ctx& c, bool upper, int n, uint32_t* a, in
 // T divides nb, so each T x T tile is wholly one or the other.  One workgroup per tile of
 // the lower trapezoid, tiles numbered column by column.
 template <int T>
-__global__ __launch_bounds__(WG, 2) void k_potrf_trail(int m, int ncols, int kb, const uint32_t* __restrict__ L,
+__global__ __launch_bounds__(WG, 3) void k_potrf_trail(int m, int ncols, int kb, const uint32_t* __restrict__ L,
                                                     int lda, uint32_t* __restrict__ C, int nb, int urgent,
                                                     const int* stop) {
   PGA_KERNEL_VGPR_FLOOR();
@@ -1268,12 +1268,24 @@ __global__ __launch_bounds__(WG, 2) void k_potrf_trail(int m, int ncols, int kb,
   }
   const int tj = lo, ti = tj + (t - (tj * tm - tj * (tj - 1) / 2));
   const int i0 = ti * T, j0 = tj * T;
-  if (i0 / nb == j0 / nb)
-    gemm::gemm_tile_in<T, T, 16, T / 16, T / 16, false, true, true>(m, ncols, kb, P_MONE, L, lda, L, lda, P_ONE, C,
-                                                                   lda, i0, j0, 1, lds);
-  else
-    gemm::gemm_tile_in<T, T, 16, T / 16, T / 16, false, true, false>(m, ncols, kb, P_MONE, L, lda, L, lda, P_ONE, C,
-                                                                    lda, i0, j0, 0, lds);
+  if constexpr (T == 64) {
+    // a 64 x 64 tile as two 64 x 32 halves (4 x 2 per lane: three waves per SIMD)
+    for (int jh = j0; jh < min(j0 + 64, ncols); jh += 32) {
+      if (i0 / nb == jh / nb)
+        gemm::gemm_tile_in<64, 32, 16, 4, 2, false, true, true>(m, ncols, kb, P_MONE, L, lda, L, lda, P_ONE, C, lda,
+                                                                i0, jh, 1, lds);
+      else
+        gemm::gemm_tile_in<64, 32, 16, 4, 2, false, true, false>(m, ncols, kb, P_MONE, L, lda, L, lda, P_ONE, C, lda,
+                                                                 i0, jh, 0, lds);
+    }
+  } else {
+    if (i0 / nb == j0 / nb)
+      gemm::gemm_tile_in<T, T, 16, T / 16, T / 16, false, true, true>(m, ncols, kb, P_MONE, L, lda, L, lda, P_ONE, C,
+                                                                     lda, i0, j0, 1, lds);
+    else
+      gemm::gemm_tile_in<T, T, 16, T / 16, T / 16, false, true, false>(m, ncols, kb, P_MONE, L, lda, L, lda, P_ONE,
+                                                                      C, lda, i0, j0, 0, lds);
+  }
 }
 
 // Rpotrf_opencl returns at the first failing diagonal block (Rpotrf_opencl.cpp:137-139):
