@@ -147,7 +147,7 @@ __global__ __launch_bounds__(PAIR_NT) void k_bb_pair(const FactorTask* __restric
 // The node's Rgemm (A22 -= A21 A12, Rgetrf2.cpp:155-159) of every problem's panel in one
 // launch of T x T tiles (workgroups take tiles from a counter).
 template <int T>
-__global__ __launch_bounds__(NT) void k_bb_node_gemm(const FactorTask* __restrict__ tasks,
+__global__ __launch_bounds__(NT, 3) void k_bb_node_gemm(const FactorTask* __restrict__ tasks,
                                                      const Item* __restrict__ items,
                                                      const int* __restrict__ tile_start, int nitems, int total,
                                                      int* __restrict__ counter, int s, int mid, int e) {
@@ -174,8 +174,15 @@ __global__ __launch_bounds__(NT) void k_bb_node_gemm(const FactorTask* __restric
     const uint32_t* B = tk.a + (j + s) + (size_t)(j + mid) * lda;
     uint32_t* C = tk.a + (j + mid) + (size_t)(j + mid) * lda;
     const int tmr = (m + T - 1) / T, tl = t - tile_start[lo];
-    gemm::gemm_tile_in<T, T, 16, T / 16, T / 16, false, false, false>(m, n2, k, 0xC0000000u, A, lda, B, lda, 0x40000000u,
-                                                                    C, lda, (tl % tmr) * T, (tl / tmr) * T, 0, lds);
+    const int i0 = (tl % tmr) * T, j0 = (tl / tmr) * T;
+    if constexpr (T == 64) {
+      for (int jh = j0; jh < min(j0 + 64, n2); jh += 32)  // two 64 x 32 halves
+        gemm::gemm_tile_in<64, 32, 16, 4, 2, false, false, false>(m, n2, k, 0xC0000000u, A, lda, B, lda, 0x40000000u,
+                                                                  C, lda, i0, jh, 0, lds);
+    } else {
+      gemm::gemm_tile_in<T, T, 16, T / 16, T / 16, false, false, false>(m, n2, k, 0xC0000000u, A, lda, B, lda,
+                                                                      0x40000000u, C, lda, i0, j0, 0, lds);
+    }
   }
 }
 
@@ -206,12 +213,21 @@ __global__ __launch_bounds__(NT) void k_bb_laswp_left(const FactorTask* __restri
 // device (a step of a few problems has few large tiles; K is never split).
 template <int T, bool TA, bool TB, bool SKIP0>
 __device__ __forceinline__ void bb_tile(const Gemm& p, int i0, int j0, uint32_t* lds) {
-  gemm::gemm_tile_in<T, T, 16, T / 16, T / 16, TA, TB, SKIP0>(p.m, p.n, p.k, p.alpha, p.a, p.lda, p.b, p.ldb, p.beta,
-                                                              p.c, p.ldc, i0, j0, p.tri, lds);
+  if constexpr (T == 64) {
+    // two 64 x 32 halves (4 x 2 per lane: three waves per SIMD)
+    for (int jh = j0; jh < min(j0 + 64, p.n); jh += 32) {
+      if (p.tri == 1 && i0 + 63 < jh) continue;
+      gemm::gemm_tile_in<64, 32, 16, 4, 2, TA, TB, SKIP0>(p.m, p.n, p.k, p.alpha, p.a, p.lda, p.b, p.ldb, p.beta, p.c,
+                                                          p.ldc, i0, jh, p.tri, lds);
+    }
+  } else {
+    gemm::gemm_tile_in<T, T, 16, T / 16, T / 16, TA, TB, SKIP0>(p.m, p.n, p.k, p.alpha, p.a, p.lda, p.b, p.ldb,
+                                                                p.beta, p.c, p.ldc, i0, j0, p.tri, lds);
+  }
 }
 
 template <int T>
-__global__ __launch_bounds__(NT) void k_bb_gemm(const Gemm* __restrict__ probs, const int* __restrict__ tile_start,
+__global__ __launch_bounds__(NT, 3) void k_bb_gemm(const Gemm* __restrict__ probs, const int* __restrict__ tile_start,
                                                 int nprob, int total, int* __restrict__ counter) {
   PGA_KERNEL_VGPR_FLOOR();
   __shared__ uint32_t lds[gemm::TileLds<T, T, 16>::words];
