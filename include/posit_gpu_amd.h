@@ -55,11 +55,11 @@ const char *pga_status_string(pga_status s);
 int pga_illegal_argument(void);
 pga_status pga_get_device_info(pga_device_info *info);
 
-/* GEMM implementation: 0 = tiled decoded kernel (default), 1 = exact one-element-per-
- * thread kernel (validation).  Also set by PGA_GEMM_IMPL=tiled|exact.  PGA_GEMM_TILE=16|32|64
- * (read at initialisation) fixes the tiled kernel's tile edge instead of the per-shape choice
- * (the role of the reference's OPENCL_GEMM_BLOCKSIZE, Rgemm_opencl.cpp:62-67); the bits do
- * not depend on it. */
+/* GEMM implementation: 0 = tiled kernel (default), 1 = exact one-element-per-thread kernel
+ * (validation).  Also set by PGA_GEMM_IMPL=tiled|exact.  PGA_GEMM_TILE=16|32|64 (read at
+ * initialisation; 64 selects the 64 x 32 tile) fixes the tiled kernel's tile instead of the
+ * per-shape choice (the role of the reference's OPENCL_GEMM_BLOCKSIZE,
+ * Rgemm_opencl.cpp:62-67); the bits do not depend on it. */
 pga_status pga_set_gemm_impl(int impl);
 
 /* ---- device memory and streams (plumbing for resident data) ------------------- */
