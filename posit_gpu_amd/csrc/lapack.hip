@@ -335,8 +335,8 @@ __global__ __launch_bounds__(WG) void k_trsm_right_sweep(int m, int n, uint32_t 
 // LDS as FP64 values: step k scales column k by 1/l_kk (the reciprocals formed once, in
 // parallel), then subtracts l_jk * b_k from every later column j (skipped when l_jk is
 // zero), each product and sum rounded to the posit grid as in the GEMM (gemm::fp64_mac).
-// A product on a posit midpoint, a scale outside the table or a NaR leaves the block
-// untouched in memory and the rows are solved again with the pattern-level sweep.
+// A scale outside the table or a NaR leaves the block untouched in memory and the rows are
+// solved again with the pattern-level sweep.
 constexpr int RSW_ROWS = 16, RSW_NMAX = 64;
 
 __global__ __launch_bounds__(WG) void k_trsm_rltn_fp64(int m, int n, const uint32_t* __restrict__ A, int lda,
@@ -359,25 +359,18 @@ __global__ __launch_bounds__(WG) void k_trsm_rltn_fp64(int m, int n, const uint3
   for (int e = tid; e < R * n; e += WG) sB[e / R][e % R] = to_double(b[e % R + (size_t)(e / R) * ldb]);
   if (tid < n) sR[tid] = to_double(div(P_ONE, A[tid + (size_t)tid * lda]));
   __syncthreads();
-  bool tie = false;
   for (int k = 0; k < n; ++k) {
-    if (tid < R) {
-      uint32_t chi;
-      const double ph = sR[k] * sB[k][tid];
-      const double y = gemm::grid_round(ph, chi);
-      tie |= __builtin_fabs(ph - y) == __builtin_fabs(__hiloint2double((int)(chi - (53u << 20)), 0));
-      sB[k][tid] = y;
-    }
+    if (tid < R) sB[k][tid] = gemm::fp64_posit_mul(sR[k], sB[k][tid]);
     __syncthreads();
     const int cnt = n - 1 - k;
     for (int e = tid; e < R * cnt; e += WG) {
       const int i = e % R, j = k + 1 + e / R;
       const double t = sL[k][j];
-      if (t != 0.0) sB[j][i] = gemm::fp64_mac(sB[j][i], -t, sB[k][i], tie);
+      if (t != 0.0) sB[j][i] = gemm::fp64_mac(sB[j][i], -t, sB[k][i]);
     }
     __syncthreads();
   }
-  bool bad = tie;
+  bool bad = false;
   for (int e = tid; e < R * n; e += WG) bad |= sB[e / R][e % R] != sB[e / R][e % R];  // NaN
   if (__syncthreads_or(bad)) {
     right_sweep_rows<false>(R, n, P_ONE, 1, A, lda, b, ldb);

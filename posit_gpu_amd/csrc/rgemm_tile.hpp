@@ -20,10 +20,9 @@ namespace gemm {
 // the top 12 bits of x: scales -108 .. 107 get their C; zero gets 1.0 (zero stays zero);
 // every other exponent gets an infinite C, which turns the result into NaN -- the slice
 // is then redone exactly.
-//   product: ph = t*a rounded to FP64 (it may need 56 bits), then rounded to the grid.
-//     FP64's rounding cannot change the posit rounding unless ph lands exactly on a
-//     posit midpoint (midpoints are FP64 numbers and rounding is monotone); that case,
-//     |ph - y| = q/2, is flagged and the slice redone exactly.
+//   product: t*a (up to 56 bits) is rounded once, by fma(t, a, C) - C, with C taken from
+//     the exponent of t*a rounded to FP64 (fp64_posit_mul).  The FMA keeps the exact
+//     product, so no double rounding arises.
 //   sum: c + y rounded to FP64 and then to the grid (fp64_posit_add: a sum that lands on
 //     a midpoint was exact, so ties round as the posit addition does).
 // ---------------------------------------------------------------------------------
@@ -59,16 +58,19 @@ PGA_D double grid_round(double x, uint32_t& chi) {
   return (x + c) - c;
 }
 
-// c <- round(c + round(t * a)); tie is set when the product's FP64 rounding may hide the
-// posit rounding decision
-PGA_D double fp64_mac(double c, double t, double a, bool& tie) {
-  const double ph = t * a;
+// round(t * a) on the posit grid, rounded once: fma(t, a, C) - C with C chosen from the
+// binade of t*a rounded to FP64 (if that rounding carried into the next binade, t*a lies
+// within 2^-52 of its power of two and both quanta round it there)
+PGA_D double fp64_posit_mul(double t, double a) {
+  const uint32_t chi = round_table[hi_word(t * a) >> 20];
+  const double c = __hiloint2double((int)chi, 0);
+  return __builtin_fma(t, a, c) - c;
+}
+
+// c <- round(c + round(t * a))
+PGA_D double fp64_mac(double c, double t, double a) {
   uint32_t chi;
-  const double y = grid_round(ph, chi);
-  const double half_q = __hiloint2double((int)(chi - (53u << 20)), 0);
-  tie |= __builtin_fabs(ph - y) == __builtin_fabs(half_q);
-  uint32_t chi2;
-  return grid_round(c + y, chi2);
+  return grid_round(c + fp64_posit_mul(t, a), chi);
 }
 
 // binary exponent of a finite nonzero double
@@ -459,7 +461,6 @@ __device__ __forceinline__ void gemm_tile_impl_fp64(int m, int n, int k, uint32_
       for (int r = 0; r < TM; ++r)
 #pragma unroll
         for (int c = 0; c < TN; ++c) saved[r][c] = acc[r][c];
-      bool tie = false;
       auto step = [&](int l) {
         double ad[TM], td[TN];
 #pragma unroll
@@ -469,7 +470,7 @@ __device__ __forceinline__ void gemm_tile_impl_fp64(int m, int n, int k, uint32_
 #pragma unroll
         for (int r = 0; r < TM; ++r)
 #pragma unroll
-          for (int c = 0; c < TN; ++c) acc[r][c] = fp64_mac(acc[r][c], td[c], ad[r], tie);
+          for (int c = 0; c < TN; ++c) acc[r][c] = fp64_mac(acc[r][c], td[c], ad[r]);
       };
       // two steps per iteration: the loop then carries no register rotation
       int l = 0;
@@ -478,7 +479,7 @@ __device__ __forceinline__ void gemm_tile_impl_fp64(int m, int n, int k, uint32_
         step(l + 1);
       }
       if (l < lmax) step(l);
-      bool bad = tie;
+      bool bad = false;
 #pragma unroll
       for (int r = 0; r < TM; ++r)
 #pragma unroll
