@@ -339,6 +339,21 @@ PGA_HD double fp64_posit_add(double t, double p, uint32_t& reach) {
 // reach values for which fp64_posit_add's results are the posit sums
 constexpr uint32_t FP64_ADD_REACH_MAX = 26;
 
+// High word of the rounding constant C for a double x whose high word is `hi` (only x's
+// sign and exponent field are used): C = +-2^(e + 52 - nf) for scales e in -108 .. 107,
+// whose FP64 ulp is the posit quantum there; 1.0 for zero (a zero stays zero); an infinite
+// C for every other exponent, which turns (x + C) - C into NaN.  The GEMM keeps these in
+// an LDS table indexed by hi >> 20.
+PGA_HD uint32_t fp64_round_chi(uint32_t hi) {
+  const uint32_t sign = hi & 0x80000000u;
+  const int eb = (int)((hi >> 20) & 0x7FFu);
+  const int e = eb - 1023;
+  if (eb == 0) return 0x3FF00000u;
+  if (e < -108 || e > 107) return sign | 0x7FF00000u;
+  const int ak = (e >> 2) ^ (e >> 31);
+  return sign | ((uint32_t)(eb + 25 + ak) << 20);
+}
+
 // ---------------------------------------------------------------------------------
 // Decoded-domain representation used by the GEMM inner loop.
 // ---------------------------------------------------------------------------------

@@ -32,21 +32,7 @@ static __shared__ uint32_t round_table[ROUND_TABLE_N];
 // Fills the table (every lane of the block takes part; a barrier must follow before use).
 // Refilling while other waves read it is harmless: the entries never change.
 PGA_D void round_table_fill(int tid, int nt) {
-  for (int idx = tid; idx < ROUND_TABLE_N; idx += nt) {
-    const uint32_t sign = (uint32_t)(idx >> 11) << 31;
-    const int eb = idx & 0x7FF;
-    const int e = eb - 1023;
-    uint32_t chi;
-    if (eb == 0) {
-      chi = 0x3FF00000u;  // C = 1: a zero stays zero
-    } else if (e >= dec::FAST_MIN && e <= dec::FAST_MAX) {
-      const int ak = (e >> 2) ^ (e >> 31);
-      chi = sign | ((uint32_t)(eb + 25 + ak) << 20);
-    } else {
-      chi = sign | 0x7FF00000u;  // infinite C: NaN result, the slice is redone
-    }
-    round_table[idx] = chi;
-  }
+  for (int idx = tid; idx < ROUND_TABLE_N; idx += nt) round_table[idx] = fp64_round_chi((uint32_t)idx << 20);
 }
 
 PGA_D uint32_t hi_word(double x) { return (uint32_t)(__double_as_longlong(x) >> 32); }

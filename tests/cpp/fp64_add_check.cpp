@@ -1,6 +1,7 @@
-// Host build of pga::fp64_posit_add (posit32_core.hpp), checked against the oracle's
-// pattern-level posit addition: wherever it does not ask for a redo, its result must be
-// the posit sum.  Pairs come from several families: values of uniform[-1,1) and of wider
+// Host build of pga::fp64_posit_add (posit32_core.hpp) and of the GEMM's FMA product
+// rounding (with pga::fp64_round_chi), checked against the oracle's pattern-level posit
+// addition and multiplication: wherever they do not ask for a redo, their results must be
+// the posit sum and product.  Pairs come from several families: values of uniform[-1,1) and of wider
 // ranges, short fractions (many exact midpoints), sums that cancel, terms far below the
 // accumulator, constructed midpoints and FP64 neighbours of midpoints, and the running
 // sums of long chains.  Prints the counts; exit status 0 when no result differs.
@@ -22,7 +23,34 @@ static uint64_t splitmix(uint64_t& s) {
 }
 static double unit(uint64_t& s) { return (double)(splitmix(s) >> 11) * 0x1.0p-53; }
 
-static long checked = 0, redone = 0, bad = 0;
+static long checked = 0, redone = 0, bad = 0, mul_checked = 0, mul_redone = 0;
+
+// the GEMM's product rounding (rgemm_tile.hpp: fp64_posit_mul) with its table entry
+// computed directly: fma(t, a, C) - C, C from the exponent of t*a rounded to FP64
+static double fp64_posit_mul_host(double t, double a) {
+  const double ph = t * a;
+  uint64_t b;
+  __builtin_memcpy(&b, &ph, sizeof b);
+  const uint64_t cb = (uint64_t)pga::fp64_round_chi((uint32_t)(b >> 32)) << 32;
+  double c;
+  __builtin_memcpy(&c, &cb, sizeof c);
+  return std::fma(t, a, c) - c;
+}
+
+static void check_mul(uint32_t a, uint32_t b) {
+  if (a == pga::P_NAR || b == pga::P_NAR) return;
+  const double y = fp64_posit_mul_host(pga::to_double(a), pga::to_double(b));
+  ++mul_checked;
+  if (y != y) {  // a scale outside the table: the caller redoes the step
+    ++mul_redone;
+    return;
+  }
+  const uint32_t want = orc_p32_mul(a, b);
+  if (y != pga::to_double(want)) {
+    if (bad < 8) printf("mul mismatch %08x * %08x: got %.17g want %08x\n", a, b, y, want);
+    ++bad;
+  }
+}
 
 static void check(uint32_t a, uint32_t b) {
   if (a == pga::P_NAR || b == pga::P_NAR) return;
@@ -63,6 +91,21 @@ int main(int argc, char** argv) {
           pga::from_double((double)((int)(splitmix(s) % 4001) - 2000) / 2048));
     check(pga::from_double(ldexp((double)(splitmix(s) % 1024), (int)(splitmix(s) % 40) - 20)),
           pga::from_double(ldexp((double)(splitmix(s) % 1024), (int)(splitmix(s) % 60) - 30)));
+    // products: uniform values (56-bit exact products), any patterns, short fractions, and
+    // factors whose product lands near a power of two
+    check_mul(a, b);
+    check_mul((uint32_t)splitmix(s), (uint32_t)splitmix(s));
+    check_mul(pga::from_double(ldexp(2 * unit(s) - 1, (int)(splitmix(s) % 120) - 60)),
+              pga::from_double(ldexp(2 * unit(s) - 1, (int)(splitmix(s) % 120) - 60)));
+    check_mul(pga::from_double((double)((int)(splitmix(s) % 4001) - 2000) / 64),
+              pga::from_double((double)((int)(splitmix(s) % 4001) - 2000) / 2048));
+    {
+      const double x = 1 + unit(s);
+      const uint32_t px = pga::from_double(x), py = pga::from_double(2.0 / x);
+      check_mul(px, py);
+      check_mul(px, py + 1u);
+      check_mul(px, py - 1u);
+    }
     // cancellation: b close to -a
     check(a, (0u - a) + (uint32_t)(splitmix(s) % 9) - 4u);
     // a term far below the accumulator
@@ -81,6 +124,18 @@ int main(int argc, char** argv) {
       }
     }
   }
+  // products whose FP64 rounding lands exactly on a posit midpoint while the exact
+  // product does not: mantissas m_t, m_a in [2^27, 2^28) with m_t * m_a = 2^27 + r
+  // (mod 2^28), r in 1..3, and a 56-bit product (the posit keeps its top 28 bits)
+  for (int i = 0; i < 200000; ++i) {
+    const uint64_t mt = (splitmix(s) & 0x7FFFFFFull) | 0x8000001ull;  // odd, top bit set
+    uint64_t inv = mt;  // inverse of mt modulo 2^28 by Newton's iteration
+    for (int it = 0; it < 5; ++it) inv = (inv * (2 - mt * inv)) & 0xFFFFFFFull;
+    const uint64_t r = 1 + splitmix(s) % 3;
+    const uint64_t ma = (((1ull << 27) + r) * inv) & 0xFFFFFFFull;
+    if (ma < (1ull << 27) || mt * ma < (1ull << 55)) continue;
+    check_mul(pga::from_double(ldexp((double)mt, -27)), pga::from_double(ldexp((double)ma, -27)));
+  }
   // running sums of chains like the solve's: t_{k+1} = round(t_k - a_k * b_k)
   for (int c = 0; c < 200; ++c) {
     uint32_t t = pga::from_double(2 * unit(s) - 1);
@@ -90,6 +145,7 @@ int main(int argc, char** argv) {
       t = orc_p32_add(t, p);
     }
   }
-  printf("checked %ld redone %ld mismatches %ld\n", checked, redone, bad);
+  printf("checked %ld redone %ld mismatches %ld products %ld redone %ld\n", checked, redone, bad, mul_checked,
+         mul_redone);
   return bad ? 1 : 0;
 }

@@ -172,10 +172,11 @@ def test_core_div_sqrt_host_build(tmp_path):
 
 def test_fp64_posit_add_host_build(tmp_path):
     """pga::fp64_posit_add (posit32_core.hpp: the posit sum rounded in FP64, for the
-    serial chains of the triangular solves), built for the host: wherever it does not ask
-    for a redo, its result equals the oracle's posit addition, over uniform values,
-    arbitrary patterns, short fractions, cancellations, tiny terms, constructed midpoints
-    and long running sums."""
+    serial chains of the triangular solves) and the GEMM's FMA product rounding, built for
+    the host: wherever they do not ask for a redo, their results equal the oracle's posit
+    addition and multiplication, over uniform values, arbitrary patterns, short fractions,
+    cancellations, tiny terms, constructed midpoints (products whose FP64 rounding lands on
+    a posit midpoint included) and long running sums."""
     import subprocess
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     exe = str(tmp_path / "fp64_add_check")
