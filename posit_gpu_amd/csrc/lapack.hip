@@ -602,36 +602,20 @@ static __device__ int potrf2_lower_regs(int n, uint32_t* a, int lda) {
   return 0;
 }
 
-// A decoded value with its mantissa renormalised after a rounding carry (2^31 becomes
-// 2^30 one scale up), as a multiply operand of gemm::mac must be.
-static __device__ __forceinline__ dec::Val normalised(dec::Val v) {
-  const uint32_t ov = v.m >> 31;
-  v.m >>= ov;
-  v.sc += (int)ov;
-  return v;
-}
-
-// Whether a product of nonzero factors with scales ts and as stays where gemm::mac rounds
-// exactly: its scale (ts + as, one more on a carry) in [FAST_MIN, FAST_MAX - 1].
-static __device__ __forceinline__ bool product_in_range(int ts, int as) {
-  return (uint32_t)(ts + as - dec::FAST_MIN) <= (uint32_t)(dec::FAST_MAX - 2 - dec::FAST_MIN);
-}
-
-// potrf2_lower_regs computed in the decoded domain (the GEMM's multiply-add, gemm::mac):
-// the same terms in the same order, each rounded as the pattern-level operations round
-// it as long as every product and sum stays in the fast scale range.  Each lane checks
-// its own steps; a step out of range, a NaR, or a failing diagonal (whose value may then
-// be meaningless) makes the call return -1 with the block unchanged, and the caller
-// redoes it with the pattern-level form.  sqrt and the reciprocal stay pattern-level.
+// potrf2_lower_regs computed with the GEMM's FP64 multiply-add (gemm::fp64_mac): the same
+// terms in the same order, each rounded as the pattern-level operations round it as long
+// as every product and sum stays in the table's scale range.  A result outside it (NaN),
+// a NaR, or a failing diagonal (whose value may then be meaningless) makes the call return
+// -1 with the block unchanged, and the caller redoes it with the pattern-level form.  sqrt
+// and the reciprocal stay pattern-level.
 template <int T>
 static __device__ int potrf2_lower_decoded(int n, uint32_t* a, int lda) {
   constexpr int SLOTS = (POTRF2_ONE_WG_MAX * (POTRF2_ONE_WG_MAX + 1) / 2 + T - 1) / T;
-  __shared__ uint32_t col_m[POTRF2_ONE_WG_MAX], col_g[POTRF2_ONE_WG_MAX];
-  __shared__ int col_s[POTRF2_ONE_WG_MAX];
+  __shared__ double col[POTRF2_ONE_WG_MAX];
   __shared__ uint32_t dg;
-  dec::Val c[SLOTS];
+  double c[SLOTS];  // posit values, exact in FP64 (NaR and out-of-range results: NaN)
   int ri[SLOTS], ci[SLOTS];
-  bool bad = false;
+  gemm::round_table_fill(threadIdx.x, T);
   {
     int j = 0, off = 0;
 #pragma unroll
@@ -643,8 +627,7 @@ static __device__ int potrf2_lower_decoded(int n, uint32_t* a, int lda) {
       }
       ri[q] = j + (e - off);
       ci[q] = j < n ? j : -1;
-      c[q] = dec::decode(j < n ? a[ri[q] + (size_t)j * lda] : 0u);
-      bad |= c[q].m != 0 && c[q].sc > dec::FAST_MAX;  // NaR and huge entries
+      c[q] = to_double(j < n ? a[ri[q] + (size_t)j * lda] : 0u);
     }
   }
   if (threadIdx.x == 0) dg = a[0];
@@ -653,26 +636,15 @@ static __device__ int potrf2_lower_decoded(int n, uint32_t* a, int lda) {
     const uint32_t v = dg;
     if (le(v, 0)) return -1;  // uniform; the pattern-level form decides
     const uint32_t d = pga::sqrt(v);
-    const dec::Val r = dec::decode(j + 1 < n ? div(P_ONE, d) : P_ONE);
+    const double r = to_double(j + 1 < n ? div(P_ONE, d) : P_ONE);
 #pragma unroll
     for (int q = 0; q < SLOTS; ++q) {
       if (ci[q] == j) {
         if (ri[q] == j) {
-          c[q] = dec::decode(d);
+          c[q] = to_double(d);
         } else {
-          if (c[q].m != 0) {  // r * 0 = 0 needs no step
-            const dec::Val x = normalised(c[q]);
-            dec::Val z;
-            z.m = 0;
-            z.sc = dec::ZERO_SC;
-            z.sg = 0;
-            bad |= !product_in_range(r.sc, x.sc);
-            bad |= gemm::mac(z, r.m, r.sc, r.sg, x.m, x.sc, x.sg);
-            c[q] = normalised(z);
-          }
-          col_m[ri[q]] = c[q].m;
-          col_s[ri[q]] = c[q].sc;
-          col_g[ri[q]] = c[q].sg;
+          c[q] = gemm::fp64_posit_mul(r, c[q]);
+          col[ri[q]] = c[q];
         }
       }
     }
@@ -680,24 +652,22 @@ static __device__ int potrf2_lower_decoded(int n, uint32_t* a, int lda) {
 #pragma unroll
     for (int q = 0; q < SLOTS; ++q) {
       if (ci[q] > j) {
-        const uint32_t mm = col_m[ci[q]], xm = col_m[ri[q]];
+        const double m = col[ci[q]];
         // a zero multiplier skips the term (Rsyrk.cpp:169, Rtrsm.cpp:272); a zero factor
         // makes it zero, which leaves the sum unchanged
-        if (mm != 0 && xm != 0) {
-          const int ms = col_s[ci[q]], xs = col_s[ri[q]];
-          bad |= !product_in_range(ms, xs);
-          bad |= gemm::mac(c[q], mm, ms, ~col_g[ci[q]], xm, xs, col_g[ri[q]]);
-          bad |= c[q].sc > dec::FAST_MAX;
-        }
-        if (ri[q] == j + 1 && ci[q] == j + 1) dg = dec::encode(c[q]);
+        if (m != 0.0) c[q] = gemm::fp64_mac(c[q], -m, col[ri[q]]);
+        if (ri[q] == j + 1 && ci[q] == j + 1) dg = from_double(c[q]);  // NaN -> NaR: redo
       }
     }
     __syncthreads();
   }
+  bool bad = false;
+#pragma unroll
+  for (int q = 0; q < SLOTS; ++q) bad |= c[q] != c[q];  // NaN: a scale left the table, or NaR
   if (__syncthreads_or(bad)) return -1;
 #pragma unroll
   for (int q = 0; q < SLOTS; ++q)
-    if (ci[q] >= 0) a[ri[q] + (size_t)ci[q] * lda] = dec::encode(c[q]);
+    if (ci[q] >= 0) a[ri[q] + (size_t)ci[q] * lda] = from_double(c[q]);
   return 0;
 }
 
