@@ -51,8 +51,6 @@ pga_status illegal(int i) {
   return PGA_ERROR_ILLEGAL_ARGUMENT;
 }
 
-std::recursive_mutex g_host_mutex;
-
 // Host<->device copies of a rows x cols column-major block with leading dimension ld
 // into a packed device buffer (leading dimension rows); synchronous.
 hipError_t put(uint32_t* d, const uint32_t* h, int rows, int cols, int ld, hipStream_t s) {
@@ -126,7 +124,7 @@ pga_status pga_rtrsm(char side, char uplo, char transa, char diag, int m, int n,
   PGA_CHECK_INIT();
   if (int bad = check_trsm(side, uplo, transa, diag, m, n, lda, ldb)) return illegal(bad);
   if (m == 0 || n == 0) return PGA_SUCCESS;
-  std::lock_guard<std::recursive_mutex> lk(g_host_mutex);
+  std::lock_guard<std::recursive_mutex> lk(host_call_mutex());
   hipStream_t s = runtime().stream;
   int na = lsame(side, 'L') ? m : n;
   pga_status st;
@@ -159,7 +157,7 @@ pga_status pga_rsyrk(char uplo, char trans, int n, int k, uint32_t alpha, const 
   PGA_CHECK_INIT();
   if (int bad = check_syrk(uplo, trans, n, k, lda, ldc)) return illegal(bad);
   if (n == 0) return PGA_SUCCESS;
-  std::lock_guard<std::recursive_mutex> lk(g_host_mutex);
+  std::lock_guard<std::recursive_mutex> lk(host_call_mutex());
   hipStream_t s = runtime().stream;
   bool tr = !lsame(trans, 'N');
   int ar = tr ? k : n, ac = tr ? n : k;
@@ -192,7 +190,7 @@ pga_status pga_rgemv(char trans, int m, int n, uint32_t alpha, const uint32_t* a
   PGA_CHECK_INIT();
   if (int bad = check_gemv(trans, m, n, lda, incx, incy)) return illegal(bad);
   if (m == 0 || n == 0) return PGA_SUCCESS;
-  std::lock_guard<std::recursive_mutex> lk(g_host_mutex);
+  std::lock_guard<std::recursive_mutex> lk(host_call_mutex());
   hipStream_t s = runtime().stream;
   bool tr = !lsame(trans, 'N');
   int lenx = tr ? m : n, leny = tr ? n : m;
@@ -223,7 +221,7 @@ pga_status pga_rscal_device(int n, uint32_t da, uint32_t* x, int incx, void* str
 pga_status pga_rscal(int n, uint32_t da, uint32_t* x, int incx) {
   PGA_CHECK_INIT();
   if (n <= 0 || incx <= 0) return PGA_SUCCESS;
-  std::lock_guard<std::recursive_mutex> lk(g_host_mutex);
+  std::lock_guard<std::recursive_mutex> lk(host_call_mutex());
   hipStream_t s = runtime().stream;
   size_t nx = (size_t)(n - 1) * incx + 1;
   pga_status st;
@@ -246,7 +244,7 @@ pga_status pga_iramax(int n, const uint32_t* x, int incx, int* result) {
   PGA_CHECK_INIT();
   *result = 0;
   if (n < 1 || incx <= 0) return PGA_SUCCESS;
-  std::lock_guard<std::recursive_mutex> lk(g_host_mutex);
+  std::lock_guard<std::recursive_mutex> lk(host_call_mutex());
   hipStream_t s = runtime().stream;
   size_t nx = (size_t)(n - 1) * incx + 1;
   pga_status st;
@@ -270,7 +268,7 @@ pga_status pga_rlaswp_device(int n, uint32_t* a, int lda, int k1, int k2, const 
 pga_status pga_rlaswp(int n, uint32_t* a, int lda, int k1, int k2, const int* ipiv, int incx) {
   PGA_CHECK_INIT();
   if (n <= 0 || incx == 0 || k2 < k1) return PGA_SUCCESS;
-  std::lock_guard<std::recursive_mutex> lk(g_host_mutex);
+  std::lock_guard<std::recursive_mutex> lk(host_call_mutex());
   hipStream_t s = runtime().stream;
   // rows touched: up to max(k2, max ipiv) in every column
   int rows = k2;
@@ -329,7 +327,7 @@ static pga_status host_getrf(bool recursive, int m, int n, uint32_t* a, int lda,
   }
   *info = 0;
   if (m == 0 || n == 0) return PGA_SUCCESS;
-  std::lock_guard<std::recursive_mutex> lk(g_host_mutex);
+  std::lock_guard<std::recursive_mutex> lk(host_call_mutex());
   hipStream_t s = runtime().stream;
   int mn = m < n ? m : n;
   pga_status st;
@@ -385,7 +383,7 @@ pga_status pga_rgetrs(char trans, int n, int nrhs, const uint32_t* a, int lda, c
   }
   *info = 0;
   if (n == 0 || nrhs == 0) return PGA_SUCCESS;
-  std::lock_guard<std::recursive_mutex> lk(g_host_mutex);
+  std::lock_guard<std::recursive_mutex> lk(host_call_mutex());
   hipStream_t s = runtime().stream;
   pga_status st;
   uint32_t* dA = (uint32_t*)scratch(0, (size_t)n * n * 4, &st);
@@ -442,7 +440,7 @@ static pga_status host_potrf(bool recursive, char uplo, int n, uint32_t* a, int 
   }
   *info = 0;
   if (n == 0) return PGA_SUCCESS;
-  std::lock_guard<std::recursive_mutex> lk(g_host_mutex);
+  std::lock_guard<std::recursive_mutex> lk(host_call_mutex());
   hipStream_t s = runtime().stream;
   pga_status st;
   uint32_t* dA = (uint32_t*)scratch(0, (size_t)n * n * 4, &st);
@@ -494,7 +492,7 @@ pga_status pga_rpotrs(char uplo, int n, int nrhs, const uint32_t* a, int lda, ui
   }
   *info = 0;
   if (n == 0 || nrhs == 0) return PGA_SUCCESS;
-  std::lock_guard<std::recursive_mutex> lk(g_host_mutex);
+  std::lock_guard<std::recursive_mutex> lk(host_call_mutex());
   hipStream_t s = runtime().stream;
   pga_status st;
   uint32_t* dA = (uint32_t*)scratch(0, (size_t)n * n * 4, &st);

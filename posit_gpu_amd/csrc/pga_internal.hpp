@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
+
 #include "../../include/posit_gpu_amd.h"
 
 namespace pga {
@@ -78,8 +80,12 @@ Runtime& runtime();
 void set_illegal_argument(int index);
 pga_status from_hip(hipError_t e);
 
-// Growable device scratch buffers used by the host-pointer entry points.
+// Growable device scratch buffers used by the host-pointer entry points and the backward
+// error.  Every caller holds host_call_mutex() from before its first scratch() call until
+// its last use of the slots has completed on the device, so no two calls share a slot and
+// no slot is freed while another call is filling or reading it.
 void* scratch(int slot, size_t bytes, pga_status* st);
+std::recursive_mutex& host_call_mutex();
 // Synchronous host <-> device copies of a column-major block through pinned staging
 // (host leading dimension ld, device leading dimension rows).
 hipError_t copy_h2d_block(uint32_t* d, const uint32_t* h, int rows, int cols, int ld, hipStream_t s);

@@ -63,6 +63,9 @@ extern "C" pga_status pga_backward_error_device(int n, const uint32_t* a, int ld
     return PGA_ERROR_ILLEGAL_ARGUMENT;
   }
   hipStream_t s = stream ? (hipStream_t)stream : runtime().stream;
+  // the accumulator lives in a shared scratch slot: calls from several threads take turns
+  // (the call is synchronous, so the lock is held until the result has been read back)
+  std::lock_guard<std::recursive_mutex> lk(host_call_mutex());
   pga_status st;
   double* tmp = (double*)scratch(6, 3 * sizeof(double), &st);
   if (st) return st;

@@ -17,10 +17,13 @@
 namespace pga {
 
 static Runtime g_rt;
-static std::recursive_mutex g_mutex;
 static thread_local int t_illegal = 0;
 
 Runtime& runtime() { return g_rt; }
+std::recursive_mutex& host_call_mutex() {
+  static std::recursive_mutex m;
+  return m;
+}
 void set_illegal_argument(int index) { t_illegal = index; }
 
 pga_status from_hip(hipError_t e) {
@@ -195,7 +198,7 @@ using namespace pga;
 extern "C" {
 
 pga_status pga_init(int device) {
-  std::lock_guard<std::recursive_mutex> lk(g_mutex);
+  std::lock_guard<std::recursive_mutex> lk(host_call_mutex());
   if (g_rt.initialized) return PGA_SUCCESS;
   int count = 0;
   if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
@@ -229,7 +232,7 @@ pga_status pga_init(int device) {
 }
 
 pga_status pga_finalize(void) {
-  std::lock_guard<std::recursive_mutex> lk(g_mutex);
+  std::lock_guard<std::recursive_mutex> lk(host_call_mutex());
   if (!g_rt.initialized) return PGA_SUCCESS;
   (void)hipStreamSynchronize(g_rt.stream);
   for (auto& s : g_scratch) {
@@ -388,7 +391,7 @@ pga_status pga_rgemm(char transa, char transb, int m, int n, int k, uint32_t alp
                      const uint32_t* a, int lda, const uint32_t* b, int ldb, uint32_t beta,
                      uint32_t* c, int ldc) {
   PGA_CHECK_INIT();
-  std::lock_guard<std::recursive_mutex> lk(g_mutex);
+  std::lock_guard<std::recursive_mutex> lk(host_call_mutex());
   bool ta, tb;
   int bad = check_gemm_args(transa, transb, m, n, k, lda, ldb, ldc, ta, tb);
   if (bad) {

@@ -22,8 +22,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <list>
@@ -159,8 +161,15 @@ struct pga_server {
     int* d_counter = nullptr;
     std::vector<Request*> batch;
     bool busy = false;
+    hipError_t err = hipSuccess;  // the launch of `batch` failed to enqueue
   };
   std::vector<Slot> slots;
+  std::vector<hipStream_t> worker_streams;  // one per worker thread
+  // PGA_SERVE_FAIL_EVERY=k (tests of the failure paths): every k-th launch of the batch
+  // slots and of the workers reports an error before it enqueues anything
+  int fail_every = 0;
+  std::atomic<int64_t> launches{0};
+  bool inject_failure() { return fail_every > 0 && ++launches % fail_every == 0; }
   std::vector<std::thread> workers;
   std::thread sched;
 
@@ -204,6 +213,7 @@ void finish(pga_server* S, Request* q, int status, int info) {
     if (is_new(q->r)) {
       f.ready = true;
       f.info = info;
+      if (status != PGA_REQ_OK) f.keep = false;  // a failed factorization is never re-used
     }
     f.refs--;
     f.last_use = ++S->tick;
@@ -418,10 +428,8 @@ pga_status run_blocked(pga_server* S, const std::vector<Request*>& batch, hipStr
 // Larger requests.  Plain mode: one request at a time through the per-call drivers.
 // Batched mode: every ready large request (up to max_batch) runs as one blocked batch;
 // requests beyond the blocked path's largest order run alone.
-void worker_main(pga_server* S) {
-  hipStream_t s = nullptr;
+void worker_main(pga_server* S, hipStream_t s) {
   (void)hipSetDevice(runtime().device);
-  (void)hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
   PlanBuf buf;
   std::vector<int> infos;
   std::unique_lock<std::mutex> lk(S->mu);
@@ -444,7 +452,9 @@ void worker_main(pga_server* S) {
       first->state = RUNNING;
       lk.unlock();
       int info = 0;
-      pga_status st = run_alone(S, first, s, &info);
+      pga_status st = S->inject_failure() ? PGA_ERROR_HIP : run_alone(S, first, s, &info);
+      // a failed request's memory returns to the arena: let whatever was queued finish first
+      if (st != PGA_SUCCESS) (void)hipStreamSynchronize(s);
       lk.lock();
       finish(S, first, st == PGA_SUCCESS ? 0 : PGA_REQ_FAILED, info);
       continue;
@@ -462,7 +472,8 @@ void worker_main(pga_server* S) {
     S->stats.batches++;
     S->stats.batched_requests += (int64_t)batch.size();
     lk.unlock();
-    pga_status st = run_blocked(S, batch, s, buf, infos);
+    pga_status st = S->inject_failure() ? PGA_ERROR_HIP : run_blocked(S, batch, s, buf, infos);
+    if (st != PGA_SUCCESS) (void)hipStreamSynchronize(s);
     lk.lock();
     for (size_t i = 0; i < batch.size(); ++i)
       finish(S, batch[i], st == PGA_SUCCESS ? 0 : PGA_REQ_FAILED, st == PGA_SUCCESS ? infos[i] : 0);
@@ -470,20 +481,24 @@ void worker_main(pga_server* S) {
   lk.unlock();
   (void)hipStreamSynchronize(s);
   if (buf.dev) (void)hipFree(buf.dev);
-  (void)hipStreamDestroy(s);
 }
 
-// Launch every admitted small request as one persistent launch on a free slot.
-void launch_batch(pga_server* S, pga_server::Slot& sl) {
+// Launch every admitted small request as one persistent launch on a free slot.  Returns
+// the first HIP error; after an error nothing more is enqueued and the work already
+// queued has finished, so the requests can be failed and their memory released.
+hipError_t launch_batch(pga_server* S, pga_server::Slot& sl) {
   hipStream_t s = sl.s;
   const int cnt = (int)sl.batch.size();
-  for (int i = 0; i < cnt; ++i) {
+  if (S->inject_failure()) return hipErrorLaunchFailure;
+  hipError_t e = hipSuccess;
+  for (int i = 0; i < cnt && e == hipSuccess; ++i) {
     Request* q = sl.batch[i];
     const pga_request& r = q->r;
     Factor& f = *q->f;
     const bool fresh = is_new(r);
-    if (fresh) (void)hipMemcpyAsync(factor_a(S, f), r.a, (size_t)r.n * r.n * 4, hipMemcpyHostToDevice, s);
-    if (r.nrhs > 0) (void)hipMemcpyAsync(S->dev(q->sol_off), r.b, (size_t)r.n * r.nrhs * 4, hipMemcpyHostToDevice, s);
+    if (fresh) e = hipMemcpyAsync(factor_a(S, f), r.a, (size_t)r.n * r.n * 4, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess && r.nrhs > 0)
+      e = hipMemcpyAsync(S->dev(q->sol_off), r.b, (size_t)r.n * r.nrhs * 4, hipMemcpyHostToDevice, s);
     FactorTask& t = sl.h_tasks[i];
     t = FactorTask{};
     t.kind = f.lu ? (fresh ? PGA_TASK_GESV : PGA_TASK_GETRS) : (fresh ? PGA_TASK_POSV : PGA_TASK_POTRS);
@@ -499,36 +514,50 @@ void launch_batch(pga_server* S, pga_server::Slot& sl) {
     t.info = factor_info(S, f);
     if (!fresh) t.info = nullptr;
   }
-  (void)hipMemcpyAsync(sl.d_tasks, sl.h_tasks, sizeof(FactorTask) * cnt, hipMemcpyHostToDevice, s);
-  (void)factor_tasks_launch(sl.d_tasks, cnt, sl.d_counter, factor_tasks_grid(cnt), s);
-  for (int i = 0; i < cnt; ++i) {
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(sl.d_tasks, sl.h_tasks, sizeof(FactorTask) * cnt, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = factor_tasks_launch(sl.d_tasks, cnt, sl.d_counter, factor_tasks_grid(cnt), s);
+  for (int i = 0; i < cnt && e == hipSuccess; ++i) {
     Request* q = sl.batch[i];
     const pga_request& r = q->r;
     if (r.nrhs > 0)
-      (void)hipMemcpyAsync(S->hbase + q->stage_off, S->dev(q->sol_off), (size_t)r.n * r.nrhs * 4,
-                           hipMemcpyDeviceToHost, s);
-    if (is_new(r))
-      (void)hipMemcpyAsync(S->h_info + (q->id % S->info_slots), factor_info(S, *q->f), sizeof(int),
-                           hipMemcpyDeviceToHost, s);
+      e = hipMemcpyAsync(S->hbase + q->stage_off, S->dev(q->sol_off), (size_t)r.n * r.nrhs * 4,
+                         hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess && is_new(r))
+      e = hipMemcpyAsync(S->h_info + (q->id % S->info_slots), factor_info(S, *q->f), sizeof(int),
+                         hipMemcpyDeviceToHost, s);
   }
-  (void)hipEventRecord(sl.ev, s);
+  if (e == hipSuccess) e = hipEventRecord(sl.ev, s);
+  if (e != hipSuccess) (void)hipStreamSynchronize(s);
+  return e;
 }
 
 void sched_main(pga_server* S) {
   (void)hipSetDevice(runtime().device);
   std::unique_lock<std::mutex> lk(S->mu);
   while (!S->stop) {
-    // completions
+    // completions; a launch that failed to enqueue, or whose event reports an error
+    // other than not-ready, finishes its requests as failed
     for (auto& sl : S->slots) {
-      if (!sl.busy || hipEventQuery(sl.ev) != hipSuccess) continue;
+      if (!sl.busy) continue;
+      hipError_t done = sl.err;
+      if (done == hipSuccess) {
+        done = hipEventQuery(sl.ev);
+        if (done == hipErrorNotReady) continue;
+      }
       for (Request* q : sl.batch) {
+        if (done != hipSuccess) {
+          finish(S, q, PGA_REQ_FAILED, 0);
+          continue;
+        }
         // a factor-and-solve writes the info of its new factor; a re-solve reports
         // the resident factor's info (zero, or it would not have been launched)
         const int info = is_new(q->r) ? S->h_info[q->id % S->info_slots] : q->f->info;
-        finish(S, q, 0, info);
+        finish(S, q, PGA_REQ_OK, info);
       }
       sl.batch.clear();
       sl.busy = false;
+      sl.err = hipSuccess;
     }
     admit(S);
     // launches
@@ -542,8 +571,9 @@ void sched_main(pga_server* S) {
       S->stats.batches++;
       S->stats.batched_requests += take;
       lk.unlock();
-      launch_batch(S, sl);
+      const hipError_t e = launch_batch(S, sl);
       lk.lock();
+      sl.err = e;
     }
     // Every wait releases the mutex: workers need it to finish their requests, which is
     // what blocked requests (memory, a factor still in production) are waiting for.
@@ -582,6 +612,7 @@ pga_status pga_server_create(pga_server** out, const pga_server_config* cfg) {
   }
   auto* S = new pga_server();
   S->cfg = c;
+  if (const char* fe = getenv("PGA_SERVE_FAIL_EVERY")) S->fail_every = atoi(fe);
   S->t0 = std::chrono::steady_clock::now();
   hipError_t e = hipMalloc((void**)&S->base, c.hbm_budget_bytes);
   if (e != hipSuccess) {
@@ -601,18 +632,26 @@ pga_status pga_server_create(pga_server** out, const pga_server_config* cfg) {
   S->harena.init(staging);
   S->slots.resize(c.batch_slots);
   for (auto& sl : S->slots) {
-    (void)hipStreamCreateWithFlags(&sl.s, hipStreamNonBlocking);
-    (void)hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming);
-    (void)hipHostMalloc((void**)&sl.h_tasks, sizeof(FactorTask) * c.max_batch, 0);
-    (void)hipMalloc((void**)&sl.d_tasks, sizeof(FactorTask) * c.max_batch);
-    (void)hipMalloc((void**)&sl.d_counter, sizeof(int));
-    if (!sl.s || !sl.ev || !sl.h_tasks || !sl.d_tasks || !sl.d_counter) {
+    if ((e = hipStreamCreateWithFlags(&sl.s, hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming)) != hipSuccess ||
+        (e = hipHostMalloc((void**)&sl.h_tasks, sizeof(FactorTask) * c.max_batch, 0)) != hipSuccess ||
+        (e = hipMalloc((void**)&sl.d_tasks, sizeof(FactorTask) * c.max_batch)) != hipSuccess ||
+        (e = hipMalloc((void**)&sl.d_counter, sizeof(int))) != hipSuccess) {
+      (void)hipGetLastError();
       pga_server_destroy(S);
-      return PGA_ERROR_OUT_OF_MEMORY;
+      return from_hip(e);
+    }
+  }
+  S->worker_streams.assign(c.workers, nullptr);
+  for (auto& ws : S->worker_streams) {
+    if ((e = hipStreamCreateWithFlags(&ws, hipStreamNonBlocking)) != hipSuccess) {
+      (void)hipGetLastError();
+      pga_server_destroy(S);
+      return from_hip(e);
     }
   }
   S->sched = std::thread(sched_main, S);
-  for (int i = 0; i < c.workers; ++i) S->workers.emplace_back(worker_main, S);
+  for (int i = 0; i < c.workers; ++i) S->workers.emplace_back(worker_main, S, S->worker_streams[i]);
   *out = S;
   return PGA_SUCCESS;
 }
@@ -704,6 +743,8 @@ pga_status pga_server_destroy(pga_server* S) {
   if (S->sched.joinable()) S->sched.join();
   for (auto& t : S->workers)
     if (t.joinable()) t.join();
+  for (hipStream_t ws : S->worker_streams)
+    if (ws) (void)hipStreamDestroy(ws);
   for (auto& sl : S->slots) {
     if (sl.s) (void)hipStreamSynchronize(sl.s);
     if (sl.s) (void)hipStreamDestroy(sl.s);

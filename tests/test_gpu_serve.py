@@ -155,3 +155,33 @@ def test_resolve_waits_for_its_factor(pga, oracle):
     want = plain.wait(w1)
     plain.close()
     assert np.array_equal(r2["x"], want["x"])
+
+
+def test_failed_launches_finish_as_failed(pga, oracle, monkeypatch):
+    """Launch failures (forced from the host before anything is enqueued, every second
+    launch of a batch slot or worker) finish their requests as failed without touching
+    the others: the rest stay bit-exact, a failed factor is never kept for re-solves, and
+    the server keeps serving afterwards."""
+    monkeypatch.setenv("PGA_SERVE_FAIL_EVERY", "2")
+    sizes = [30, 600, 120, 530, 64, 700]  # batch slots and workers alternate
+    reqs = _requests(oracle, 800, sizes)
+    s = pga.Server(256 * MiB, "batched", batch_max_n=512)
+    monkeypatch.delenv("PGA_SERVE_FAIL_EVERY")
+    ids, got = [], []
+    for kind, n, A, B in reqs:  # one at a time: one launch per request
+        ids.append(s.submit(kind, n, A, B))
+        got.append(s.wait(ids[-1]))
+    gone = s.wait(s.submit("resolve", sizes[1], b=reqs[1][3], factor=ids[1]))
+    after = s.wait(s.submit(*reqs[0]))  # the seventh launch is not forced to fail
+    s.close()
+    plain = pga.Server(256 * MiB, "plain")
+    want = [plain.wait(plain.submit(kind, n, A, B)) for kind, n, A, B in reqs]
+    plain.close()
+    for i, (g, w) in enumerate(zip(got, want)):
+        if i % 2:
+            assert g["status"] == pga.REQ_FAILED, i
+        else:
+            assert g["status"] == w["status"] == pga.REQ_OK, i
+            assert np.array_equal(g["x"], w["x"]), i
+    assert gone["status"] == pga.REQ_NOT_RESIDENT
+    assert after["status"] == pga.REQ_OK and np.array_equal(after["x"], want[0]["x"])
