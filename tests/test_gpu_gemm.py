@@ -213,3 +213,85 @@ def test_tile_override_same_bits(pga, tmp_path):
         outs[tile] = np.load(out)
     for tile in ("16", "32", "64"):
         assert np.array_equal(outs[tile], outs[""]), tile
+
+
+_TILE_SCRIPT = """
+import sys, numpy as np
+sys.path.insert(0, {root!r})
+import posit_gpu_amd as pga
+pga.init(0)
+d = np.load(sys.argv[1])
+out = {{}}
+for key in d.files:
+    if not key.startswith('C_'):
+        continue
+    case = key[2:]
+    kind, ta, tb, alpha, beta = case.split('-')
+    A, B, C = d['A_' + kind + ta], d['B_' + kind + tb], d[key]
+    out[case] = pga.rgemm(ta, tb, int(alpha), np.asfortranarray(A), np.asfortranarray(B), int(beta),
+                          np.asfortranarray(C))
+np.savez(sys.argv[2], **out)
+"""
+
+
+def _class_matrix(oracle, rng, kind, r, c, seed):
+    if kind == "patterns":
+        v = random_patterns(rng, r * c)
+    elif kind == "long_regimes":
+        v = random_patterns(rng, r * c, True)
+    elif kind == "sparse":
+        v = posit_matrix(oracle, seed, r, c).ravel(order="F").copy()
+        v[rng.random(v.size) < 0.7] = 0
+    elif kind == "nar":
+        v = posit_matrix(oracle, seed, r, c).ravel(order="F").copy()
+        v[rng.integers(0, v.size, 3)] = NAR
+    elif kind == "tiny":  # products near 2^-100, sums can cancel below 2^-108
+        v = oracle.from_double_v(rng.uniform(-1, 1, r * c) * 2.0 ** -50)
+    elif kind == "huge":  # products near 2^100: accumulators leave the fast range
+        v = oracle.from_double_v(rng.uniform(-1, 1, r * c) * 2.0 ** 52)
+    elif kind == "boundary":  # products 2^106..2^108: running sums cross 2^108 inside a slice
+        v = oracle.from_double_v(rng.choice([-1.0, 1.0], r * c) * rng.uniform(0.5, 1.0, r * c) * 2.0 ** 54)
+    else:  # cancel: few distinct values so that many sums cancel exactly
+        v = oracle.from_double_v(rng.choice([-1.0, -0.5, 0.5, 1.0, 0.25], r * c))
+    return np.asfortranarray(np.asarray(v, np.uint32).reshape(c, r).T)
+
+
+@pytest.mark.parametrize("tile", ["64", "32", "16"])
+def test_value_classes_at_each_tile(pga, oracle, tmp_path, tile):
+    """Every value class through the tile that serves large GEMMs: a process started with
+    PGA_GEMM_TILE fixed runs all four variants and three (alpha, beta) pairs, with K long
+    enough (288) that the one-element tile also takes the FP64 form.  This exercises the
+    FP64 form's slice redo (NaN from a scale outside the table, NaR) and the sum-first
+    epilogue of TN / TT at 4 x 2, 2 x 2 and 1 x 1 elements per lane, against the oracle."""
+    import subprocess
+    import sys
+    m, n, k = 70, 66, 288
+    kinds = ["patterns", "long_regimes", "sparse", "nar", "tiny", "huge", "boundary", "cancel"]
+    pairs = [(ONE, 0), (MONE, ONE), (0x3A000000, 0xC8000000)]
+    arrays, want = {}, {}
+    for ci, kind in enumerate(kinds):
+        rng = np.random.default_rng(500 + ci)
+        A = _class_matrix(oracle, rng, kind, m, k, 1)
+        B = _class_matrix(oracle, rng, kind, k, n, 2)
+        C = _class_matrix(oracle, rng, kind, m, n, 3)
+        arrays[f"A_{kind}N"], arrays[f"A_{kind}T"] = A, np.asfortranarray(A.T)
+        arrays[f"B_{kind}N"], arrays[f"B_{kind}T"] = B, np.asfortranarray(B.T)
+        for ta in "NT":
+            for tb in "NT":
+                for alpha, beta in pairs:
+                    case = f"{kind}-{ta}-{tb}-{alpha}-{beta}"
+                    arrays["C_" + case] = C
+                    w = C.copy(order="F")
+                    assert oracle.rgemm(ta, tb, m, n, k, alpha, arrays[f"A_{kind}{ta}"], arrays[f"B_{kind}{tb}"],
+                                        beta, w, nthreads=THREADS) == 0
+                    want[case] = w
+    np.savez(tmp_path / "in.npz", **arrays)
+    env = dict(os.environ, PGA_GEMM_TILE=tile)
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    subprocess.run([sys.executable, "-c", _TILE_SCRIPT.format(root=root), str(tmp_path / "in.npz"),
+                    str(tmp_path / "out.npz")], env=env, check=True, timeout=240)
+    got = np.load(tmp_path / "out.npz")
+    assert sorted(got.files) == sorted(want)
+    for case, w in want.items():
+        bad = np.argwhere(got[case] != w)
+        assert bad.size == 0, f"tile {tile} {case}: {len(bad)} mismatches, first {bad[:3].tolist()}"
