@@ -28,7 +28,8 @@ typedef enum {
     PGA_ERROR_HIP = 3,              /* a HIP runtime call failed */
     PGA_ERROR_OUT_OF_MEMORY = 4,
     PGA_ERROR_NO_DEVICE = 5,
-    PGA_ERROR_UNSUPPORTED = 6
+    PGA_ERROR_UNSUPPORTED = 6,
+    PGA_ERROR_EXCEEDS_SCRATCH = 7   /* more than a no-allocation entry point's fixed scratch holds */
 } pga_status;
 
 typedef struct {
@@ -181,6 +182,27 @@ pga_status pga_batch_plan_launch_graph(pga_batch_plan *plan, void *stream);
 pga_status pga_batch_plan_info(const pga_batch_plan *plan, int *tile, int *grid, int *total_tiles,
                                double *tune_ms);
 pga_status pga_batch_plan_destroy(pga_batch_plan *plan);
+
+/* No-allocation batched GEMM.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  One call does what pga_batch_plan_create (with the
+ * given tile, no autotuning) + pga_batch_plan_execute + pga_batch_plan_destroy do, with
+ * the same kernel and the same bits, but calls no hipMalloc, hipFree or other allocating
+ * routine: the problem descriptors, tile offsets and work counter of the batch live in a
+ * fixed device array of about 240 KiB that the library holds on each device it runs on
+ * until the process ends (set up by pga_init).  Arguments as in pga_batch_plan_create;
+ * tile 0 picks the largest of 64, 32, 16 that gives every CU four tiles, grid 0 two
+ * workgroups per CU.  At most PGA_NOALLOC_MAX_PROBLEMS problems (else
+ * PGA_ERROR_EXCEEDS_SCRATCH).  Asynchronous on `stream`.  Every call on a device uses
+ * that device's one array: calls on one stream run in order, but calls that overlap on
+ * one GPU (different streams or threads without ordering between them) race and may
+ * compute wrong results. */
+#define PGA_NOALLOC_MAX_PROBLEMS 4096
+pga_status pga_batch_gemm_noalloc(char transa, char transb, int count, const int *m, const int *n,
+                                  const int *k, const uint32_t *alpha, const uint32_t *const *a,
+                                  const int *lda, const uint32_t *const *b, const int *ldb,
+                                  const uint32_t *beta, uint32_t *const *c, const int *ldc, int tile,
+                                  int grid, void *stream);
 
 /* ---- batched factorizations and solves (one persistent launch, hipGraph-capturable) --
  * Each task is one of the per-call routines below on its own device matrix, computed by

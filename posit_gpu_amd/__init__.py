@@ -533,6 +533,26 @@ class BatchPlan:
             pass
 
 
+NOALLOC_MAX_PROBLEMS = 4096
+_sig("pga_batch_gemm_noalloc", [_C, _C, _I, _IP, _IP, _IP, ctypes.POINTER(_U32), _PP, _IP, _PP, _IP,
+                                ctypes.POINTER(_U32), _PP, _IP, _I, _I, _P])
+
+
+def batch_gemm_noalloc(transa, transb, problems, tile=0, grid=0, stream=None):
+    """pga_batch_gemm_noalloc: the batched GEMM of BatchPlan in one call that allocates no
+    device memory (descriptors in the library's fixed device array, at most
+    NOALLOC_MAX_PROBLEMS problems).  Asynchronous on `stream`; calls that overlap on one
+    GPU race."""
+    init()
+    cnt = len(problems)
+    ia = lambda key: (ctypes.c_int * max(1, cnt))(*[int(p[key]) for p in problems])  # noqa: E731
+    ua = lambda key: (_U32 * max(1, cnt))(*[int(p[key]) for p in problems])  # noqa: E731
+    pa = lambda key: (_P * max(1, cnt))(*[_dp(p[key]) for p in problems])  # noqa: E731
+    _check(_lib.pga_batch_gemm_noalloc(_ch(transa), _ch(transb), cnt, ia("m"), ia("n"), ia("k"), ua("alpha"),
+                                       pa("a"), ia("lda"), pa("b"), ia("ldb"), ua("beta"), pa("c"), ia("ldc"),
+                                       tile, grid, stream), "pga_batch_gemm_noalloc")
+
+
 # ---------------------------------------------------------------------------------
 # batched factorizations / solves
 # ---------------------------------------------------------------------------------

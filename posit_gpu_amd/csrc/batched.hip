@@ -31,6 +31,19 @@ int batched_tiles(int m, int n, int tile);
 
 }  // namespace pga
 
+// ---------------------------------------------------------------------------------
+// Device memory of pga_batch_gemm_noalloc: the descriptors, tile offsets and work
+// counter of one batch, held by every device the library runs on until the process
+// ends.  Host code never names this array as a device address: kernels and copies get
+// the address hipGetSymbolAddress returned for the current device (g_noalloc[dev].ptr).
+// ---------------------------------------------------------------------------------
+constexpr size_t NOALLOC_PROBS_BYTES = sizeof(pga::BatchProblem) * PGA_NOALLOC_MAX_PROBLEMS;
+constexpr size_t NOALLOC_START_OFF = (NOALLOC_PROBS_BYTES + 255) & ~(size_t)255;
+constexpr size_t NOALLOC_COUNTER_OFF = (NOALLOC_START_OFF + sizeof(int) * (PGA_NOALLOC_MAX_PROBLEMS + 1) + 255) &
+                                       ~(size_t)255;
+constexpr size_t NOALLOC_BYTES = NOALLOC_COUNTER_OFF + 256;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
 struct pga_batch_plan {
   bool ta = false, tb = false;
   int count = 0;
@@ -121,6 +134,40 @@ hipError_t run(pga_batch_plan& p, hipStream_t s) {
   return batched_launch(p.ta, p.tb, p.tile, p.grid, p.d_probs, p.d_tile_start, p.count, p.total_tiles,
                         p.d_counter, s);
 }
+
+// Per device: the address of `scratch` and an event marking the end of the last call's
+// descriptor copies (the host staging below is rewritten only after it).
+constexpr int NOALLOC_MAX_DEVICES = 64;
+struct NoallocDevice {
+  unsigned char* ptr = nullptr;
+  hipEvent_t copied = nullptr;
+  bool pending = false;  // `copied` has been recorded and may not have completed yet
+};
+NoallocDevice g_noalloc[NOALLOC_MAX_DEVICES];
+// Host staging of the descriptors; one call at a time fills it (g_noalloc_mu).
+std::mutex g_noalloc_mu;
+BatchProblem g_noalloc_probs[PGA_NOALLOC_MAX_PROBLEMS];
+int g_noalloc_start[PGA_NOALLOC_MAX_PROBLEMS + 1];
+int g_noalloc_order[PGA_NOALLOC_MAX_PROBLEMS];
+
+}  // namespace
+
+// Set-up of the no-allocation entry point on the current device `device` (called by
+// pga_init once the device is current; later calls for the same device do nothing).
+hipError_t batched_noalloc_setup(int device) {
+  if (device < 0 || device >= NOALLOC_MAX_DEVICES) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  NoallocDevice& d = g_noalloc[device];
+  if (d.ptr) return hipSuccess;
+  void* p = nullptr;
+  hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(::scratch));
+  if (e != hipSuccess) return e;
+  if ((e = hipEventCreateWithFlags(&d.copied, hipEventDisableTiming)) != hipSuccess) return e;
+  d.ptr = static_cast<unsigned char*>(p);
+  return hipSuccess;
+}
+
+namespace {
 
 // Time every (tile, grid) candidate on scratch copies of the output blocks, keep the best.
 hipError_t autotune(pga_batch_plan& p) {
@@ -328,6 +375,94 @@ pga_status pga_batch_plan_destroy(pga_batch_plan* p) {
   if (p->d_tile_start) (void)hipFree(p->d_tile_start);
   if (p->d_counter) (void)hipFree(p->d_counter);
   delete p;
+  return PGA_SUCCESS;
+}
+
+// No-allocation form of pga_batch_plan_create + pga_batch_plan_execute: the same
+// problems, the same persistent kernel, the same bits, with the descriptors in the
+// library's fixed device array instead of memory allocated per plan.
+pga_status pga_batch_gemm_noalloc(char transa, char transb, int count, const int* m, const int* n, const int* k,
+                                  const uint32_t* alpha, const uint32_t* const* a, const int* lda,
+                                  const uint32_t* const* b, const int* ldb, const uint32_t* beta,
+                                  uint32_t* const* c, const int* ldc, int tile, int grid, void* stream) {
+  PGA_CHECK_INIT();
+  auto up = [](char ch) { return (ch >= 'a' && ch <= 'z') ? (char)(ch - 32) : ch; };
+  const char A = up(transa), B = up(transb);
+  if (A != 'N' && A != 'T' && A != 'C') { set_illegal_argument(1); return PGA_ERROR_ILLEGAL_ARGUMENT; }
+  if (B != 'N' && B != 'T' && B != 'C') { set_illegal_argument(2); return PGA_ERROR_ILLEGAL_ARGUMENT; }
+  if (count < 0) { set_illegal_argument(3); return PGA_ERROR_ILLEGAL_ARGUMENT; }
+  if (count > PGA_NOALLOC_MAX_PROBLEMS) return PGA_ERROR_EXCEEDS_SCRATCH;
+  if (tile != 0 && tile != 16 && tile != 32 && tile != 64) { set_illegal_argument(15); return PGA_ERROR_ILLEGAL_ARGUMENT; }
+  const bool ta = (A != 'N'), tb = (B != 'N');
+  const int dev = runtime().device;
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES || !g_noalloc[dev].ptr) return PGA_ERROR_NOT_INITIALIZED;
+  NoallocDevice& d = g_noalloc[dev];
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  // the checks and quick returns of pga_batch_plan_create, problem by problem
+  int live = 0;
+  for (int i = 0; i < count; ++i) {
+    const int nra = ta ? k[i] : m[i], nrb = tb ? n[i] : k[i];
+    if (m[i] < 0 || n[i] < 0 || k[i] < 0 || lda[i] < std::max(1, nra) || ldb[i] < std::max(1, nrb) ||
+        ldc[i] < std::max(1, m[i])) {
+      set_illegal_argument(4);
+      return PGA_ERROR_ILLEGAL_ARGUMENT;
+    }
+    if (m[i] == 0 || n[i] == 0) continue;
+    if (alpha[i] == 0 || k[i] == 0) {
+      if (beta[i] == 0x40000000u) continue;  // posit32 1.0: C unchanged
+      set_illegal_argument(7);
+      return PGA_ERROR_UNSUPPORTED;
+    }
+    g_noalloc_order[live++] = i;
+  }
+  if (live == 0) return PGA_SUCCESS;
+  // most expensive first; ties in the caller's order (std::sort allocates nothing)
+  std::sort(g_noalloc_order, g_noalloc_order + live, [&](int x, int y) {
+    const double wx = (double)m[x] * n[x] * k[x], wy = (double)m[y] * n[y] * k[y];
+    return wx != wy ? wx > wy : x < y;
+  });
+  if (tile == 0) {  // the largest tile that still gives every CU four tiles
+    tile = 16;
+    for (int t : {64, 32}) {
+      long long tiles = 0;
+      for (int j = 0; j < live; ++j) tiles += batched_tiles(m[g_noalloc_order[j]], n[g_noalloc_order[j]], t);
+      if (tiles >= 4LL * runtime().cus) {
+        tile = t;
+        break;
+      }
+    }
+  }
+  if (grid <= 0) grid = 2 * runtime().cus;
+  hipStream_t s = stream ? (hipStream_t)stream : runtime().stream;
+  // the previous call's copies read the staging: wait for them before rewriting it
+  if (d.pending) PGA_HIP(hipEventSynchronize(d.copied));
+  d.pending = false;
+  g_noalloc_start[0] = 0;
+  for (int j = 0; j < live; ++j) {
+    const int i = g_noalloc_order[j];
+    BatchProblem& q = g_noalloc_probs[j];
+    q.m = m[i];
+    q.n = n[i];
+    q.k = k[i];
+    q.lda = lda[i];
+    q.ldb = ldb[i];
+    q.ldc = ldc[i];
+    q.alpha = alpha[i];
+    q.beta = beta[i];
+    q.a = a[i];
+    q.b = b[i];
+    q.c = c[i];
+    g_noalloc_start[j + 1] = g_noalloc_start[j] + batched_tiles(q.m, q.n, tile);
+  }
+  auto* probs = reinterpret_cast<BatchProblem*>(d.ptr);
+  auto* start = reinterpret_cast<int*>(d.ptr + NOALLOC_START_OFF);
+  auto* counter = reinterpret_cast<int*>(d.ptr + NOALLOC_COUNTER_OFF);
+  PGA_HIP(hipMemcpyAsync(probs, g_noalloc_probs, sizeof(BatchProblem) * live, hipMemcpyHostToDevice, s));
+  PGA_HIP(hipMemcpyAsync(start, g_noalloc_start, sizeof(int) * (live + 1), hipMemcpyHostToDevice, s));
+  PGA_HIP(hipEventRecord(d.copied, s));
+  d.pending = true;
+  // the counter is cleared by a kernel of the launch sequence, not by a memset
+  PGA_HIP(batched_launch(ta, tb, tile, grid, probs, start, live, g_noalloc_start[live], counter, s));
   return PGA_SUCCESS;
 }
 

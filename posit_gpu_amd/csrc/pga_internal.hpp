@@ -44,6 +44,10 @@ struct FactorTask {
   int* info;
 };
 
+// Looks up this device's address of the no-allocation batched GEMM's fixed device
+// array (batched.hip); the device must be current.
+hipError_t batched_noalloc_setup(int device);
+
 // Persistent task kernel of batched_factor.hip
 int factor_tasks_grid(int count);
 hipError_t factor_tasks_launch(const FactorTask* d_tasks, int count, int* d_counter, int grid, hipStream_t s);

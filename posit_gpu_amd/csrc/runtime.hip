@@ -217,6 +217,8 @@ pga_status pga_init(int device) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) g_rt.cus = prop.multiProcessorCount;
   }
+  // the fixed device array of the no-allocation batched GEMM, for this device
+  PGA_HIP(batched_noalloc_setup(device));
   const char* impl = getenv("PGA_GEMM_IMPL");
   if (impl && strcmp(impl, "exact") == 0) g_rt.gemm_impl = GemmImpl::Exact;
   // the tile edge of the plain GEMM (the role of OPENCL_GEMM_BLOCKSIZE in the reference):
@@ -257,6 +259,7 @@ const char* pga_status_string(pga_status s) {
     case PGA_ERROR_OUT_OF_MEMORY: return "out of device memory";
     case PGA_ERROR_NO_DEVICE: return "no usable HIP device";
     case PGA_ERROR_UNSUPPORTED: return "unsupported";
+    case PGA_ERROR_EXCEEDS_SCRATCH: return "exceeds the fixed scratch of a no-allocation entry point";
   }
   return "unknown status";
 }
