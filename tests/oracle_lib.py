@@ -9,7 +9,9 @@ import os
 import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-ORC_PATH = os.path.join(ROOT, "oracle", "liborc.so")
+# PGA_ORACLE_LIB names another build of the oracle (the mutation tests load planted-mistake
+# copies this way)
+ORC_PATH = os.environ.get("PGA_ORACLE_LIB") or os.path.join(ROOT, "oracle", "liborc.so")
 REF_PATH = os.path.join(ROOT, "oracle", "_ref", "libposit_ref.so")
 
 _P = ctypes.c_void_p
