@@ -14,8 +14,18 @@ Besides the JSON fields of the run contract the line carries:
                 peak = CUs * 4 SIMD * 16 lanes * clock; traffic = HBM bytes per launch from
                 FETCH_SIZE + WRITE_SIZE in the same file (scaled by a streaming kernel's
                 counts, bench.py --config copycal).
+  batched_gemm, batched_factor, serving
+                the other parts of the metric, each with its bitwise-mismatch count:
+                4096 GEMMs (M, N, K on {16..512}) in one persistent launch from a
+                hipGraph against per-call launches; 2048 LU + 2048 lower Cholesky
+                factorizations (n in [32, 512]) likewise; a request stream (50% LU / 50%
+                Cholesky, n log-uniform [64, 4096], 1-16 RHS, 20% re-solves) served with
+                continuous batching against plain per-request launches, as capacity and
+                at a fixed Poisson rate (solves/s, p50, p99).  --no-extras leaves them out.
   cpu_baseline  the oracle's restatement of the reference CPU Rgemm (OpenMP over columns,
-                as libRpack/Rgemm.cpp:168) on a bounded sample of the 16384 shape.
+                as libRpack/Rgemm.cpp:168) on a bounded sample of the 16384 shape, on every
+                thread the process is given and on one thread (medians of three), with the
+                host's core counts and CPU model.
 """
 import argparse
 import json
@@ -33,42 +43,75 @@ SHAPES = [16384, 12288, 8192, 4096]
 K = 256
 P_ONE, P_MONE = 0x40000000, 0xC0000000
 COUNTS = os.path.join(ROOT, "profiles", "rgemm_valu_counts.json")
+METRIC = "posit32 GEMM Gop/s (2MNK/t), %INT VALU peak; batched solves/s + p99 latency"
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(seconds_target=12.0):
-    """Time the oracle Rgemm (port of libRpack/Rgemm.cpp, NN branch) on column samples of
-    the M=16384, K=256 trailing shape with all host cores (capped at 16)."""
+def _cpu_facts():
+    """Host cores this process may use, the OpenMP thread count it is given and the CPU
+    model (the GPU box assigns one GPU's share of its cores: OMP_NUM_THREADS there)."""
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    omp = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "affinity_cpus": avail, "omp_num_threads": omp or None,
+            "threads": min(avail, omp) if omp > 0 else avail, "cpu_model": model}
+
+
+def cpu_baseline(seconds_target=8.0):
+    """The oracle's restatement of the reference CPU Rgemm (NN branch of libRpack/Rgemm.cpp,
+    OpenMP over columns as at Rgemm.cpp:168) on column samples of the M=16384, K=256
+    trailing shape: the median of three runs on every thread this process is given, and
+    the median of three single-thread runs."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from datagen import posit_matrix
     from oracle_lib import Oracle
 
     O = Oracle()
-    cores = min(16, os.cpu_count() or 1)
+    facts = _cpu_facts()
+    threads = facts["threads"]
     M = SHAPES[0]
     A = posit_matrix(O, 501, M, K)
-    ncols = 4
-    elapsed = 0.0
-    while True:
+
+    def run(ncols, nthreads):
         B = posit_matrix(O, 502, K, ncols)
         C = posit_matrix(O, 503, M, ncols)
         t0 = time.perf_counter()
-        O.rgemm("N", "N", M, ncols, K, P_MONE, A, B, P_ONE, C, nthreads=cores)
-        elapsed = time.perf_counter() - t0
-        if elapsed > seconds_target / 4 or ncols >= 4096:
-            break
-        ncols = int(ncols * max(2.0, min(16.0, (seconds_target / 4) / max(elapsed, 1e-3))))
-    gops = 2.0 * M * ncols * K / elapsed / 1e9
+        O.rgemm("N", "N", M, ncols, K, P_MONE, A, B, P_ONE, C, nthreads=nthreads)
+        return time.perf_counter() - t0
+
+    def sample(nthreads, target):
+        ncols = 2
+        while True:  # grow the sample until one run takes about target / 3
+            t = run(ncols, nthreads)
+            if t > target / 6 or ncols >= 8192:
+                break
+            ncols = int(ncols * max(2.0, min(16.0, (target / 3) / max(t, 1e-3))))
+        ts = sorted(run(ncols, nthreads) for _ in range(3))
+        return ncols, ts[1]
+
+    ncols, t_mt = sample(threads, seconds_target)
+    ncols1, t_1 = sample(1, seconds_target / 2)
     return {
-        "value": round(gops, 4),
+        "value": round(2.0 * M * ncols * K / t_mt / 1e9, 4),
         "unit": "Gop/s",
-        "cores": cores,
+        "cores": threads,
         "kind": "port",
-        "sample": f"oracle Rgemm NN, M={M}, N={ncols} columns, K={K}, alpha=-1, beta=1, "
-                  f"{elapsed:.2f} s on {cores} OpenMP threads",
+        "sample": f"oracle Rgemm NN, M={M}, K={K}, alpha=-1, beta=1, N={ncols} columns on {threads} OpenMP "
+                  f"threads (median of 3: {t_mt:.2f} s) and N={ncols1} columns on 1 thread (median of 3: "
+                  f"{t_1:.2f} s)",
+        "single_thread": round(2.0 * M * ncols1 * K / t_1 / 1e9, 4),
+        **{k: v for k, v in facts.items() if k != "threads"},
     }
 
 
@@ -78,6 +121,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="default run: leave out the batched and served sub-measurements")
     ap.add_argument("--config", default="gemm",
                     choices=["gemm", "dropin", "lu", "cholesky", "batched", "batched_factor", "blocked", "serve",
                              "copycal"],
@@ -193,7 +238,7 @@ def main():
         roof["counts_from"] = os.path.relpath(COUNTS, ROOT)
 
     out = {
-        "metric": "posit32 GEMM Gop/s (2MNK/t)",
+        "metric": METRIC,
         "value": round(gops, 2),
         "unit": "Gop/s",
         "n_gpus": 1,
@@ -215,6 +260,19 @@ def main():
         },
         "roofline": roof,
     }
+    for b in bufs:
+        for d in b[1:]:
+            d.free()
+    bufs.clear()
+    if not args.no_extras:
+        # the other measured parts of the metric: batched GEMMs, batched factorizations
+        # (both one persistent launch replayed from a hipGraph) and the served form
+        log("batched GEMMs ...")
+        out["batched_gemm"] = run_batched_gemm(pga, 3)
+        log("batched factorizations ...")
+        out["batched_factor"] = run_batched_factor(pga, 2)
+        log("serving ...")
+        out["serving"] = run_serving(pga, args.requests, args.qps)
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline()
@@ -269,7 +327,7 @@ def bench_dropin(pga, info, args):
     if not args.no_cpu_baseline:
         # the oracle's Rgemm (port of Rgemm.cpp NN) on the same operands
         O = _oracle()
-        cores = min(16, os.cpu_count() or 1)
+        cores = _cpu_facts()["threads"]
         Cc = np.zeros((n, n), np.uint32, order="F")
         t0 = time.perf_counter()
         O.rgemm("N", "N", n, n, n, P_ONE, np.asfortranarray(A), np.asfortranarray(B), 0, Cc, nthreads=cores)
@@ -324,7 +382,7 @@ def bench_lu(pga, info, args):
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         from datagen import posit_matrix
 
-        cores = min(16, os.cpu_count() or 1)
+        cores = _cpu_facts()["threads"]
         n_cpu = 1536
         Ac = posit_matrix(O, 4243, n_cpu, n_cpu)
         t0 = time.perf_counter()
@@ -384,7 +442,7 @@ def bench_cholesky(pga, info, args):
     if not args.no_cpu_baseline:
         # the oracle's blocked lower Cholesky (port of Rpotrf_opencl / Rpotrf2) at n_cpu
         O = _oracle()
-        cores = min(16, os.cpu_count() or 1)
+        cores = _cpu_facts()["threads"]
         n_cpu = 2048
         Mc = uniform_doubles(4301, n_cpu * n_cpu).reshape(n_cpu, n_cpu)
         Sc = Mc @ Mc.T / n_cpu + np.eye(n_cpu)
@@ -406,53 +464,79 @@ def bench_cholesky(pga, info, args):
     print(json.dumps(out), flush=True)
 
 
-def bench_batched(pga, info, args):
-    """4096 independent GEMMs, M, N, K uniform on multiples of 16 in [16, 512], alpha=beta=1,
-    one persistent launch (autotuned) captured in a hipGraph, vs per-call launches."""
+def run_batched_gemm(pga, reps):
+    """4096 independent GEMMs, M, N, K uniform on multiples of 16 in [16, 512], alpha =
+    beta = 1: one persistent launch (autotuned plan) replayed from a hipGraph, against
+    per-call launches one after another; every C compared bit for bit."""
     rng = np.random.default_rng(4096)
     count = 4096
     shapes = rng.integers(1, 33, (count, 3)) * 16
-    probs, bufs = [], []
+    offs = np.zeros((count + 1, 3), np.int64)
+    for i, (m, n, k) in enumerate(shapes):
+        offs[i + 1] = offs[i] + [m * k, k * n, m * n]
+    dA = pga.DeviceArray(int(offs[-1, 0]), np.uint32, "C")
+    dB = pga.DeviceArray(int(offs[-1, 1]), np.uint32, "C")
+    dC0 = pga.DeviceArray(int(offs[-1, 2]), np.uint32, "C")
+    dC = pga.DeviceArray(int(offs[-1, 2]), np.uint32, "C")
+    pga.fill_uniform(dA, 41)
+    pga.fill_uniform(dB, 42)
+    pga.fill_uniform(dC0, 43)
+    probs = []
     total_ops = 0.0
     for i, (m, n, k) in enumerate(shapes):
         m, n, k = int(m), int(n), int(k)
-        dA, dB, dC = pga.DeviceArray((m, k)), pga.DeviceArray((k, n)), pga.DeviceArray((m, n))
-        pga.fill_uniform(dA, 10 * i + 1)
-        pga.fill_uniform(dB, 10 * i + 2)
-        pga.fill_uniform(dC, 10 * i + 3)
-        probs.append(dict(m=m, n=n, k=k, alpha=P_ONE, beta=P_ONE, a=dA, lda=m, b=dB, ldb=k, c=dC, ldc=m))
-        bufs.append((dA, dB, dC))
+        probs.append(dict(m=m, n=n, k=k, alpha=P_ONE, beta=P_ONE, a=dA.ptr + 4 * int(offs[i, 0]), lda=m,
+                          b=dB.ptr + 4 * int(offs[i, 1]), ldb=k, c=dC.ptr + 4 * int(offs[i, 2]), ldc=m))
         total_ops += 2.0 * m * n * k
+    dC.copy_from(dC0)
     pga.synchronize()
     t0 = time.perf_counter()
     plan = pga.BatchPlan("N", "N", probs, tile=0)
     t_tune = time.perf_counter() - t0
     plan.capture_graph()
+    dC.copy_from(dC0)
     plan.launch_graph()
     pga.synchronize()
-    reps = max(1, args.steps)
-    t_graph = min(_timed(pga, plan.launch_graph) for _ in range(reps))
+    got = dC.to_host()
+    t_graph = min(_timed(pga, plan.launch_graph) for _ in range(max(1, reps)))
+
     def per_call():
         for p in probs:
             pga.rgemm_device("N", "N", p["m"], p["n"], p["k"], P_ONE, p["a"], p["lda"], p["b"], p["ldb"], P_ONE,
                              p["c"], p["ldc"])
-    t_plain = min(_timed(pga, per_call) for _ in range(max(1, reps // 2)))
-    out = {
-        "metric": "batched posit32 GEMM Gop/s (sum 2MNK/t)", "value": round(total_ops / t_graph / 1e9, 2),
-        "unit": "Gop/s", "n_gpus": 1, "higher_is_better": True, "dtype": "posit32 (VALU: integer and exact FP64 operations)",
-        "data": "synthetic: seeded uniform[-1,1) posit32, shapes from a seeded generator",
-        "config": {"workload": "4096 GEMMs, M,N,K uniform on {16,...,512}, alpha=beta=1", "plan": plan.info(),
-                   "graph_s": round(t_graph, 5), "per_call_s": round(t_plain, 5),
-                   "speedup_vs_per_call": round(t_plain / t_graph, 2), "plan_build_s": round(t_tune, 2),
-                   "device": info["name"]},
-    }
+
+    dC.copy_from(dC0)
+    pga.synchronize()
+    t_plain = _timed(pga, per_call)
+    want = dC.to_host()
+    mism = sum(0 if np.array_equal(got[offs[i, 2]:offs[i + 1, 2]], want[offs[i, 2]:offs[i + 1, 2]]) else 1
+               for i in range(count))
+    info = plan.info()
+    del plan
+    for d in (dA, dB, dC0, dC):
+        d.free()
+    return {"workload": "4096 GEMMs C += A*B, M,N,K uniform on {16,32,...,512}, alpha=beta=1, one persistent "
+                        "launch replayed from a hipGraph",
+            "value": round(total_ops / t_graph / 1e9, 2), "unit": "Gop/s",
+            "gemms_per_s": round(count / t_graph, 1), "graph_s": round(t_graph, 5),
+            "per_call_s": round(t_plain, 5), "speedup_vs_per_call": round(t_plain / t_graph, 2),
+            "plan": info, "plan_build_s": round(t_tune, 2), "bitwise_mismatches_vs_per_call": mism}
+
+
+def bench_batched(pga, info, args):
+    r = run_batched_gemm(pga, max(1, args.steps))
+    out = {"metric": "batched posit32 GEMM Gop/s (sum 2MNK/t)", "value": r["value"], "unit": "Gop/s", "n_gpus": 1,
+           "higher_is_better": True, "dtype": "posit32 (VALU: integer and exact FP64 operations)",
+           "data": "synthetic: seeded uniform[-1,1) posit32, shapes from a seeded generator",
+           "config": dict(r, device=info["name"])}
     print(json.dumps(out), flush=True)
 
 
-def bench_batched_factor(pga, info, args):
+def run_batched_factor(pga, reps):
     """2048 LU (Rgetrf_opencl, nb = 64) and 2048 lower Cholesky (Rpotrf_opencl, nb = 64)
     factorizations, n uniform in [32, 512], in one persistent launch replayed from a
-    hipGraph, against the per-call device routines one problem after another."""
+    hipGraph, against the per-call device routines one problem after another; factors,
+    pivots and info compared bit for bit."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from datagen import uniform_doubles
 
@@ -493,7 +577,7 @@ def bench_batched_factor(pga, info, args):
     plan.capture_graph()
     t_build = time.perf_counter() - t0
     times = []
-    for _ in range(max(1, min(args.steps, 3))):
+    for _ in range(max(1, min(reps, 3))):
         restore()
         times.append(_timed(pga, plan.launch_graph))
     t_graph = min(times)
@@ -517,17 +601,26 @@ def bench_batched_factor(pga, info, args):
         if piv is not None:
             same = same and np.array_equal(piv, it["ipiv"].to_host())
         mismatches += 0 if same else 1
-    out = {
-        "metric": "batched posit32 factorizations/s", "value": round(len(items) / t_graph, 1),
-        "unit": "factorizations/s", "n_gpus": 1, "higher_is_better": True, "dtype": "posit32 (VALU: integer and exact FP64 operations)",
-        "data": "synthetic: LU inputs uniform[-1,1), Cholesky inputs M*M^T/n + I, posit32",
-        "config": {"workload": f"{count} Rgetrf_opencl (nb=64) + {count} lower Rpotrf_opencl (nb=64), n uniform "
-                               "in [32, 512], one persistent launch in a hipGraph",
-                   "graph_s": round(t_graph, 4), "per_call_s": round(t_plain, 4),
-                   "speedup_vs_per_call": round(t_plain / t_graph, 2), "gops": round(ops / t_graph / 1e9, 2),
-                   "plan": plan.info(), "plan_build_s": round(t_build, 3),
-                   "bitwise_mismatches_vs_per_call": mismatches, "device": info["name"]},
-    }
+    pinfo = plan.info()
+    del plan
+    for it in items:
+        for key in ("orig", "a", "ipiv", "info"):
+            if it[key] is not None:
+                it[key].free()
+    return {"workload": f"{count} Rgetrf_opencl (nb=64) + {count} lower Rpotrf_opencl (nb=64), n uniform "
+                        "in [32, 512], one persistent launch replayed from a hipGraph",
+            "value": round(len(items) / t_graph, 1), "unit": "factorizations/s",
+            "graph_s": round(t_graph, 4), "per_call_s": round(t_plain, 4),
+            "speedup_vs_per_call": round(t_plain / t_graph, 2), "gops": round(ops / t_graph / 1e9, 2),
+            "plan": pinfo, "plan_build_s": round(t_build, 3), "bitwise_mismatches_vs_per_call": mismatches}
+
+
+def bench_batched_factor(pga, info, args):
+    r = run_batched_factor(pga, args.steps)
+    out = {"metric": "batched posit32 factorizations/s", "value": r["value"], "unit": "factorizations/s",
+           "n_gpus": 1, "higher_is_better": True, "dtype": "posit32 (VALU: integer and exact FP64 operations)",
+           "data": "synthetic: LU inputs uniform[-1,1), Cholesky inputs M*M^T/n + I, posit32",
+           "config": dict(r, device=info["name"])}
     print(json.dumps(out), flush=True)
 
 
@@ -692,10 +785,11 @@ def _serve_run(pga, reqs, mode, arrivals, batch_max_n=512, batch_slots=2, worker
                  "peak_bytes": st["bytes_peak"], "evictions": st["evictions"]}
 
 
-def bench_serve(pga, info, args):
+def run_serving(pga, requests, qps):
     """Serving: capacity (every request submitted at once) and latency under a Poisson
     arrival stream at a fixed rate, for the served form (continuous batching) and for
-    plain per-request launches; every solution compared bit for bit between the two."""
+    plain per-request launches; every solution compared bit for bit across the four runs."""
+    args = argparse.Namespace(requests=requests, qps=qps)
     reqs = _serve_requests(pga, args.requests, 777)
     log(f"{len(reqs)} requests generated")
     zero = [0.0] * len(reqs)
@@ -716,18 +810,23 @@ def bench_serve(pga, info, args):
         if len(infos) != 1 or any(not np.array_equal(xs[0], x) for x in xs[1:]):
             mism += 1
     sizes = [r["n"] for r in reqs]
-    out = {
-        "metric": "served posit32 solves/s", "value": cap_s_st["solves_per_s"], "unit": "solves/s", "n_gpus": 1,
-        "higher_is_better": True, "dtype": "posit32 (VALU: integer and exact FP64 operations)",
-        "data": "synthetic: general uniform[-1,1) and SPD M*M^T/n + I blocks, RHS uniform[-1,1), posit32",
-        "config": {"workload": f"{len(reqs)} solve requests: 50% LU / 50% Cholesky (upper), n log-uniform "
-                               "[64, 4096], 1-16 RHS, 20% re-solves on resident factors",
-                   "n_median": int(np.median(sizes)), "resolves": sum(1 for r in reqs if r["kind"] == "resolve"),
-                   "served_capacity": cap_s_st, "plain_capacity": cap_p_st,
-                   "speedup_vs_plain": round(cap_s_st["solves_per_s"] / cap_p_st["solves_per_s"], 2),
-                   "qps": args.qps, "served_at_qps": lat_s_st, "plain_at_qps": lat_p_st,
-                   "bitwise_mismatches_across_runs": mism, "device": info["name"]},
-    }
+    return {"workload": f"{len(reqs)} solve requests: 50% LU / 50% Cholesky (upper), n log-uniform "
+                        "[64, 4096], 1-16 RHS, 20% re-solves on resident factors",
+            "value": cap_s_st["solves_per_s"], "unit": "solves/s",
+            "n_median": int(np.median(sizes)), "resolves": sum(1 for r in reqs if r["kind"] == "resolve"),
+            "served_capacity": cap_s_st, "plain_capacity": cap_p_st,
+            "speedup_vs_plain": round(cap_s_st["solves_per_s"] / cap_p_st["solves_per_s"], 2),
+            "qps": args.qps, "served_at_qps": lat_s_st, "plain_at_qps": lat_p_st,
+            "speedup_at_qps": round(lat_s_st["solves_per_s"] / lat_p_st["solves_per_s"], 2),
+            "bitwise_mismatches_across_runs": mism}
+
+
+def bench_serve(pga, info, args):
+    r = run_serving(pga, args.requests, args.qps)
+    out = {"metric": "served posit32 solves/s", "value": r["value"], "unit": "solves/s", "n_gpus": 1,
+           "higher_is_better": True, "dtype": "posit32 (VALU: integer and exact FP64 operations)",
+           "data": "synthetic: general uniform[-1,1) and SPD M*M^T/n + I blocks, RHS uniform[-1,1), posit32",
+           "config": dict(r, device=info["name"])}
     print(json.dumps(out), flush=True)
 
 
