@@ -152,7 +152,7 @@ __global__ __launch_bounds__(NT, 3) void k_bb_node_gemm(const FactorTask* __rest
                                                      const int* __restrict__ tile_start, int nitems, int total,
                                                      int* __restrict__ counter, int s, int mid, int e) {
   PGA_KERNEL_VGPR_FLOOR();
-  __shared__ uint32_t lds[gemm::TileLds<T, T, 16>::words];
+  __shared__ __align__(16) uint32_t lds[gemm::TileLds<T, T, 16>::words];
   __shared__ int s_tile;
   for (;;) {
     if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1);
@@ -230,7 +230,7 @@ template <int T>
 __global__ __launch_bounds__(NT, 3) void k_bb_gemm(const Gemm* __restrict__ probs, const int* __restrict__ tile_start,
                                                 int nprob, int total, int* __restrict__ counter) {
   PGA_KERNEL_VGPR_FLOOR();
-  __shared__ uint32_t lds[gemm::TileLds<T, T, 16>::words];
+  __shared__ __align__(16) uint32_t lds[gemm::TileLds<T, T, 16>::words];
   __shared__ int s_tile;
   for (;;) {
     if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1);

@@ -1219,7 +1219,7 @@ __global__ __launch_bounds__(WG, 3) void k_potrf_trail(int m, int ncols, int kb,
                                                     const int* stop) {
   PGA_KERNEL_VGPR_FLOOR();
   if (urgent) PGA_URGENT();
-  __shared__ uint32_t lds[gemm::TileLds<T, T, 16>::words];
+  __shared__ __align__(16) uint32_t lds[gemm::TileLds<T, T, 16>::words];
   if (stopped(stop)) return;
   const int tm = (m + T - 1) / T, tc = (ncols + T - 1) / T, t = blockIdx.x;
   // column q holds tiles q .. tm-1: it starts at q*tm - q*(q-1)/2

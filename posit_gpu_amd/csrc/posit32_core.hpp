@@ -354,6 +354,24 @@ PGA_HD uint32_t fp64_round_chi(uint32_t hi) {
   return sign | ((uint32_t)(eb + 25 + ak) << 20);
 }
 
+// High word of the sign-free rounding constant C = 1.5 * 2^(e + 52 - nf) for a double x
+// whose high word is `hi` (only x's exponent field is used).  Its FP64 ulp is the posit
+// quantum q = 2^(e - nf) at x's binade, its last bit is zero, and x + C stays in C's
+// binade for either sign of x (|x| < 2^(e+1) <= 2^-24 * C), so (x + C) - C rounds x to a
+// multiple of q with ties to the even multiple -- the even posit pattern -- whatever x's
+// sign.  Scales -108 .. 107 get their C, zero gets 1.5 (a zero stays zero), every other
+// exponent +infinity, which turns (x + C) - C into NaN.  The GEMM keeps these in an LDS
+// table indexed by the exponent field alone, so that values of opposite signs read the
+// same entry.
+PGA_HD uint32_t fp64_grid_chi(uint32_t hi) {
+  const int eb = (int)((hi >> 20) & 0x7FFu);
+  const int e = eb - 1023;
+  if (eb == 0) return 0x3FF80000u;
+  if (e < -108 || e > 107) return 0x7FF00000u;
+  const int ak = (e >> 2) ^ (e >> 31);
+  return ((uint32_t)(eb + 25 + ak) << 20) | 0x00080000u;
+}
+
 // ---------------------------------------------------------------------------------
 // Decoded-domain representation used by the GEMM inner loop.
 // ---------------------------------------------------------------------------------

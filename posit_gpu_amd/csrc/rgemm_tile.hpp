@@ -12,35 +12,42 @@ namespace gemm {
 // ---------------------------------------------------------------------------------
 // The multiply-add in FP64.  Posit values with regimes up to 27 bits long have at most
 // 28 significant bits, so every operand and accumulator is exact in FP64.  A value x is
-// rounded to the posit grid of its binade by y = (x + C) - C with C = +-2^(e + 52 - nf)
+// rounded to the posit grid of its binade by y = (x + C) - C with C = 1.5 * 2^(e + 52 - nf)
 // (nf = 27 - |k|' fraction bits at scale e, |k|' = k for k >= 0 and -k - 1 below), whose
 // FP64 ulp is the posit quantum: round-to-nearest-even rounds to a multiple of it, ties
-// to the even one, which is the even posit pattern (see fp64_posit_add).  C's high word
-// depends only on x's sign and exponent field, so it comes from a table in LDS indexed by
-// the top 12 bits of x: scales -108 .. 107 get their C; zero gets 1.0 (zero stays zero);
-// every other exponent gets an infinite C, which turns the result into NaN -- the slice
-// is then redone exactly.
+// to the even one, which is the even posit pattern (see fp64_posit_add); with the factor
+// 1.5, x + C stays in C's binade for either sign of x (fp64_grid_chi).  C depends only on
+// x's exponent field, so it comes from a table in LDS indexed by those 11 bits: scales
+// -108 .. 107 get their C; zero gets 1.5 (zero stays zero); every other exponent gets an
+// infinite C, which turns the result into NaN -- the slice is then redone exactly.
 //   product: t*a (up to 56 bits) is rounded once, by fma(t, a, C) - C, with C taken from
 //     the exponent of t*a rounded to FP64 (fp64_posit_mul).  The FMA keeps the exact
 //     product, so no double rounding arises.
 //   sum: c + y rounded to FP64 and then to the grid (fp64_posit_add: a sum that lands on
 //     a midpoint was exact, so ties round as the posit addition does).
 // ---------------------------------------------------------------------------------
-constexpr int ROUND_TABLE_N = 4096;
-static __shared__ uint32_t round_table[ROUND_TABLE_N];
+// The table holds C itself (8 bytes, low word zero), so that one 64-bit LDS read yields
+// the FP64 constant with no register moves; values of opposite signs read the same entry,
+// so they never meet in one LDS bank at different addresses.
+constexpr int ROUND_TABLE_N = 2048;
+static __shared__ double round_table[ROUND_TABLE_N];
 
 // Fills the table (every lane of the block takes part; a barrier must follow before use).
 // Refilling while other waves read it is harmless: the entries never change.
 PGA_D void round_table_fill(int tid, int nt) {
-  for (int idx = tid; idx < ROUND_TABLE_N; idx += nt) round_table[idx] = fp64_round_chi((uint32_t)idx << 20);
+  for (int idx = tid; idx < ROUND_TABLE_N; idx += nt)
+    round_table[idx] = __hiloint2double((int)fp64_grid_chi((uint32_t)idx << 20), 0);
 }
 
 PGA_D uint32_t hi_word(double x) { return (uint32_t)(__double_as_longlong(x) >> 32); }
 
+// C for x: the entry of x's exponent field
+PGA_D double grid_c(double x) { return round_table[(hi_word(x) >> 20) & 0x7FFu]; }
+
 // y = x rounded to the posit grid (NaN outside the table's scales); chi is C's high word
 PGA_D double grid_round(double x, uint32_t& chi) {
-  chi = round_table[hi_word(x) >> 20];
-  const double c = __hiloint2double((int)chi, 0);
+  const double c = grid_c(x);
+  chi = hi_word(c);
   return (x + c) - c;
 }
 
@@ -48,8 +55,7 @@ PGA_D double grid_round(double x, uint32_t& chi) {
 // binade of t*a rounded to FP64 (if that rounding carried into the next binade, t*a lies
 // within 2^-52 of its power of two and both quanta round it there)
 PGA_D double fp64_posit_mul(double t, double a) {
-  const uint32_t chi = round_table[hi_word(t * a) >> 20];
-  const double c = __hiloint2double((int)chi, 0);
+  const double c = grid_c(t * a);
   return __builtin_fma(t, a, c) - c;
 }
 
@@ -62,9 +68,10 @@ PGA_D double fp64_mac(double c, double t, double a) {
 // binary exponent of a finite nonzero double
 PGA_D int fp64_scale(double x) { return (int)((hi_word(x) >> 20) & 0x7FFu) - 1023; }
 
-// LDS needed by gemm_tile<BM, BN, BK, ...>, in 32-bit words: three decoded planes
-// (mantissa, scale, sign) of the BK x BM A-slice and of the BK x BN B-side slice, plus
-// the per-wave slice bounds.
+// LDS needed by gemm_tile<BM, BN, BK, ...>, in 32-bit words, for either form of the tile:
+// the integer form keeps three decoded planes (mantissa, scale, sign) of the BK x BM
+// A-slice and of the BK x BN B-side slice, the FP64 form each element as one 8-byte
+// value; both add the per-wave slice bounds.  The buffer must be 8-byte aligned.
 template <int BM, int BN, int BK>
 struct TileLds {
   static constexpr int words = 3 * BK * BM + 3 * BK * BN + 16 * 4;
@@ -313,9 +320,7 @@ __device__ __forceinline__ void gemm_tile_impl_fp64(int m, int n, int k, uint32_
                                                const uint32_t* __restrict__ A, int lda,
                                                const uint32_t* __restrict__ B, int ldb, uint32_t beta,
                                                uint32_t* __restrict__ C, int ldc, int i0, int j0, int tri,
-                                               uint32_t (*__restrict__ sAm)[BM], int (*__restrict__ sAs)[BM],
-                                               uint32_t (*__restrict__ sAg)[BM], uint32_t (*__restrict__ sTm)[BN],
-                                               int (*__restrict__ sTs)[BN], uint32_t (*__restrict__ sTg)[BN],
+                                               double (*__restrict__ sA)[BM], double (*__restrict__ sT)[BN],
                                                int (*__restrict__ sBound)[4]) {
   constexpr int NX = BM / TM;
   constexpr int NY = BN / TN;
@@ -371,8 +376,7 @@ __device__ __forceinline__ void gemm_tile_impl_fp64(int m, int n, int k, uint32_
       uint32_t v = 0;
       if (i < m && l < k) v = TA ? A[l + (size_t)i * lda] : A[i + (size_t)l * lda];
       const double dv = pga::to_double(v);  // exact; NaR -> NaN
-      sAm[ll][ii] = (uint32_t)__double_as_longlong(dv);
-      sAs[ll][ii] = (int)(__double_as_longlong(dv) >> 32);
+      sA[ll][ii] = dv;
       const int sc = (v == 0) ? dec::ZERO_SC : (v == P_NAR ? dec::NAR_SC : fp64_scale(dv));
       amax = max(amax, sc);
       if (v != 0) amin = min(amin, sc);
@@ -395,8 +399,7 @@ __device__ __forceinline__ void gemm_tile_impl_fp64(int m, int n, int k, uint32_
       if (!SUM_FIRST && !(SKIP0 && v == 0))
         v = alpha_one ? v : (alpha_mone ? pga::neg(v) : pga::mul(alpha, v));
       const double dv = pga::to_double(v);
-      sTm[ll][jj] = (uint32_t)__double_as_longlong(dv);
-      sTs[ll][jj] = (int)(__double_as_longlong(dv) >> 32);
+      sT[ll][jj] = dv;
       const int sc = (v == 0) ? dec::ZERO_SC : (v == P_NAR ? dec::NAR_SC : fp64_scale(dv));
       tmax = max(tmax, sc);
       if (v != 0) tmin = min(tmin, sc);
@@ -447,16 +450,34 @@ __device__ __forceinline__ void gemm_tile_impl_fp64(int m, int n, int k, uint32_
       for (int r = 0; r < TM; ++r)
 #pragma unroll
         for (int c = 0; c < TN; ++c) saved[r][c] = acc[r][c];
+      // fp64_mac on every element, in phases: all products and their table reads, then
+      // all roundings of the products and the sums, then all sums' table reads, then the
+      // sums' roundings -- every table read of a phase is in flight before the first
+      // result is used (the scheduling barriers keep the compiler from interleaving them
+      // one multiply-add at a time)
       auto step = [&](int l) {
-        double ad[TM], td[TN];
+        double ad[TM], td[TN], y[TM][TN], cc[TM][TN];
 #pragma unroll
-        for (int r = 0; r < TM; ++r) ad[r] = __hiloint2double(sAs[l][tx + r * NX], (int)sAm[l][tx + r * NX]);
+        for (int r = 0; r < TM; ++r) ad[r] = sA[l][tx + r * NX];
 #pragma unroll
-        for (int c = 0; c < TN; ++c) td[c] = __hiloint2double(sTs[l][ty + c * NY], (int)sTm[l][ty + c * NY]);
+        for (int c = 0; c < TN; ++c) td[c] = sT[l][ty + c * NY];
 #pragma unroll
         for (int r = 0; r < TM; ++r)
 #pragma unroll
-          for (int c = 0; c < TN; ++c) acc[r][c] = fp64_mac(acc[r][c], td[c], ad[r]);
+          for (int c = 0; c < TN; ++c) cc[r][c] = grid_c(td[c] * ad[r]);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int r = 0; r < TM; ++r)
+#pragma unroll
+          for (int c = 0; c < TN; ++c) {
+            y[r][c] = acc[r][c] + (__builtin_fma(td[c], ad[r], cc[r][c]) - cc[r][c]);
+            cc[r][c] = grid_c(y[r][c]);
+          }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int r = 0; r < TM; ++r)
+#pragma unroll
+          for (int c = 0; c < TN; ++c) acc[r][c] = (y[r][c] + cc[r][c]) - cc[r][c];
       };
       // two steps per iteration: the loop then carries no register rotation
       int l = 0;
@@ -486,8 +507,8 @@ __device__ __forceinline__ void gemm_tile_impl_fp64(int m, int n, int k, uint32_
         for (int c = 0; c < TN; ++c) {
           uint32_t pc = pga::from_double(acc[r][c]);  // exact; NaN -> NaR
           for (int l = 0; l < lmax; ++l) {
-            const uint32_t ap = pga::from_double(__hiloint2double(sAs[l][tx + r * NX], (int)sAm[l][tx + r * NX]));
-            const uint32_t tp = pga::from_double(__hiloint2double(sTs[l][ty + c * NY], (int)sTm[l][ty + c * NY]));
+            const uint32_t ap = pga::from_double(sA[l][tx + r * NX]);
+            const uint32_t tp = pga::from_double(sT[l][ty + c * NY]);
             if (SKIP0 && tp == 0) continue;  // Rsyrk.cpp:149,169 skip zero multipliers
             pc = pga::add(pc, pga::mul(tp, ap));
           }
@@ -518,56 +539,49 @@ __device__ __forceinline__ void gemm_tile_impl_fp64(int m, int n, int k, uint32_
 }
 
 // One BM x BN tile of C at (i0, j0): the body shared by the plain and the batched kernel.
-// Both forms give every element the reference's bits.
+// Both forms give every element the reference's bits.  `lds` holds TileLds<BM, BN, BK>
+// words, 8-byte aligned; each form lays out its slices there.
 template <int BM, int BN, int BK, int TM, int TN, bool TA, bool TB, bool SKIP0>
 __device__ __forceinline__ void gemm_tile_impl(int m, int n, int k, uint32_t alpha,
                                                const uint32_t* __restrict__ A, int lda,
                                                const uint32_t* __restrict__ B, int ldb, uint32_t beta,
                                                uint32_t* __restrict__ C, int ldc, int i0, int j0, int tri,
-                                               uint32_t (*__restrict__ sAm)[BM], int (*__restrict__ sAs)[BM],
-                                               uint32_t (*__restrict__ sAg)[BM], uint32_t (*__restrict__ sTm)[BN],
-                                               int (*__restrict__ sTs)[BN], uint32_t (*__restrict__ sTg)[BN],
-                                               int (*__restrict__ sBound)[4]) {
+                                               uint32_t* lds) {
   // one element per lane: the FP64 form pays for its table and its LDS round trips only
   // over long K loops (a 512^3 GEMM: 1.45x the integer form; the LU panel's node updates,
   // K <= 128: 0.35x)
-  if (TM * TN >= 4 || k >= 256)
-    gemm_tile_impl_fp64<BM, BN, BK, TM, TN, TA, TB, SKIP0>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, i0, j0, tri,
-                                                           sAm, sAs, sAg, sTm, sTs, sTg, sBound);
-  else
-    gemm_tile_impl_int<BM, BN, BK, TM, TN, TA, TB, SKIP0>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, i0, j0, tri,
-                                                          sAm, sAs, sAg, sTm, sTs, sTg, sBound);
+  if (TM * TN >= 4 || k >= 256) {
+    double* d = reinterpret_cast<double*>(lds);
+    gemm_tile_impl_fp64<BM, BN, BK, TM, TN, TA, TB, SKIP0>(
+        m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, i0, j0, tri, reinterpret_cast<double(*)[BM]>(d),
+        reinterpret_cast<double(*)[BN]>(d + BK * BM), reinterpret_cast<int(*)[4]>(d + BK * BM + BK * BN));
+  } else {
+    gemm_tile_impl_int<BM, BN, BK, TM, TN, TA, TB, SKIP0>(
+        m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, i0, j0, tri, reinterpret_cast<uint32_t(*)[BM]>(lds),
+        reinterpret_cast<int(*)[BM]>(lds + BK * BM), reinterpret_cast<uint32_t(*)[BM]>(lds + 2 * BK * BM),
+        reinterpret_cast<uint32_t(*)[BN]>(lds + 3 * BK * BM), reinterpret_cast<int(*)[BN]>(lds + 3 * BK * BM + BK * BN),
+        reinterpret_cast<uint32_t(*)[BN]>(lds + 3 * BK * BM + 2 * BK * BN),
+        reinterpret_cast<int(*)[4]>(lds + 3 * BK * BM + 3 * BK * BN));
+  }
 }
 
-// The tile with its own LDS planes (separate objects), for kernels that run one tile shape.
+// The tile with its own LDS buffer, for kernels that run one tile shape.
 template <int BM, int BN, int BK, int TM, int TN, bool TA, bool TB, bool SKIP0>
 __device__ __forceinline__ void gemm_tile(int m, int n, int k, uint32_t alpha, const uint32_t* __restrict__ A,
                                           int lda, const uint32_t* __restrict__ B, int ldb, uint32_t beta,
                                           uint32_t* __restrict__ C, int ldc, int i0, int j0, int tri) {
-  __shared__ uint32_t sAm[BK][BM];
-  __shared__ int sAs[BK][BM];
-  __shared__ uint32_t sAg[BK][BM];
-  __shared__ uint32_t sTm[BK][BN];
-  __shared__ int sTs[BK][BN];
-  __shared__ uint32_t sTg[BK][BN];
-  __shared__ int sBound[(BM / TM) * (BN / TN) / 64][4];
-  gemm_tile_impl<BM, BN, BK, TM, TN, TA, TB, SKIP0>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, i0, j0, tri, sAm,
-                                                    sAs, sAg, sTm, sTs, sTg, sBound);
+  __shared__ __align__(16) uint32_t lds[TileLds<BM, BN, BK>::words];
+  gemm_tile_impl<BM, BN, BK, TM, TN, TA, TB, SKIP0>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, i0, j0, tri, lds);
 }
 
-// The tile with its planes carved from a caller's buffer of TileLds<BM, BN, BK>::words
-// words, for kernels that run several tile shapes out of one LDS allocation.
+// The tile in a caller's buffer of TileLds<BM, BN, BK>::words words (8-byte aligned), for
+// kernels that run several tile shapes out of one LDS allocation.
 template <int BM, int BN, int BK, int TM, int TN, bool TA, bool TB, bool SKIP0>
 __device__ __forceinline__ void gemm_tile_in(int m, int n, int k, uint32_t alpha, const uint32_t* __restrict__ A,
                                              int lda, const uint32_t* __restrict__ B, int ldb, uint32_t beta,
                                              uint32_t* __restrict__ C, int ldc, int i0, int j0, int tri,
                                              uint32_t* lds) {
-  gemm_tile_impl<BM, BN, BK, TM, TN, TA, TB, SKIP0>(
-      m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, i0, j0, tri, reinterpret_cast<uint32_t(*)[BM]>(lds),
-      reinterpret_cast<int(*)[BM]>(lds + BK * BM), reinterpret_cast<uint32_t(*)[BM]>(lds + 2 * BK * BM),
-      reinterpret_cast<uint32_t(*)[BN]>(lds + 3 * BK * BM), reinterpret_cast<int(*)[BN]>(lds + 3 * BK * BM + BK * BN),
-      reinterpret_cast<uint32_t(*)[BN]>(lds + 3 * BK * BM + 2 * BK * BN),
-      reinterpret_cast<int(*)[4]>(lds + 3 * BK * BM + 3 * BK * BN));
+  gemm_tile_impl<BM, BN, BK, TM, TN, TA, TB, SKIP0>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, i0, j0, tri, lds);
 }
 
 }  // namespace gemm

@@ -36,7 +36,7 @@ constexpr int MAX_N = PGA_FACTOR_TASK_MAX_N;
 typedef __attribute__((address_space(3))) uint32_t lds_u32;
 
 struct Shared {
-  uint32_t tile[gemm::TileLds<64, 64, 16>::words];  // GEMM tiles of every shape
+  alignas(16) uint32_t tile[gemm::TileLds<64, 64, 16>::words];  // GEMM tiles of every shape
   int perm[MAX_N];                                 // row interchanges of right-hand sides
   int red_v[NT / 64];
   int red_i[NT / 64];

@@ -1,7 +1,7 @@
-// Host build of pga::fp64_posit_add (posit32_core.hpp) and of the GEMM's FMA product
-// rounding (with pga::fp64_round_chi), checked against the oracle's pattern-level posit
-// addition and multiplication: wherever they do not ask for a redo, their results must be
-// the posit sum and product.  Pairs come from several families: values of uniform[-1,1) and of wider
+// Host build of pga::fp64_posit_add (posit32_core.hpp) and of the GEMM's FMA product and
+// sum roundings (with pga::fp64_grid_chi, the sign-free C = 1.5 * 2^E of its LDS table),
+// checked against the oracle's pattern-level posit addition and multiplication: wherever
+// they do not ask for a redo, their results must be the posit sum and product.  Pairs come from several families: values of uniform[-1,1) and of wider
 // ranges, short fractions (many exact midpoints), sums that cancel, terms far below the
 // accumulator, constructed midpoints and FP64 neighbours of midpoints, and the running
 // sums of long chains.  Prints the counts; exit status 0 when no result differs.
@@ -23,18 +23,41 @@ static uint64_t splitmix(uint64_t& s) {
 }
 static double unit(uint64_t& s) { return (double)(splitmix(s) >> 11) * 0x1.0p-53; }
 
-static long checked = 0, redone = 0, bad = 0, mul_checked = 0, mul_redone = 0;
+static long checked = 0, redone = 0, bad = 0, mul_checked = 0, mul_redone = 0, gsum_checked = 0, gsum_redone = 0;
 
-// the GEMM's product rounding (rgemm_tile.hpp: fp64_posit_mul) with its table entry
-// computed directly: fma(t, a, C) - C, C from the exponent of t*a rounded to FP64
-static double fp64_posit_mul_host(double t, double a) {
-  const double ph = t * a;
+// the GEMM's rounding constant for x (rgemm_tile.hpp: grid_c) computed directly
+static double grid_c_host(double x) {
   uint64_t b;
-  __builtin_memcpy(&b, &ph, sizeof b);
-  const uint64_t cb = (uint64_t)pga::fp64_round_chi((uint32_t)(b >> 32)) << 32;
+  __builtin_memcpy(&b, &x, sizeof b);
+  const uint64_t cb = (uint64_t)pga::fp64_grid_chi((uint32_t)(b >> 32)) << 32;
   double c;
   __builtin_memcpy(&c, &cb, sizeof c);
+  return c;
+}
+
+// the GEMM's product rounding (rgemm_tile.hpp: fp64_posit_mul): fma(t, a, C) - C, C from
+// the exponent of t*a rounded to FP64
+static double fp64_posit_mul_host(double t, double a) {
+  const double c = grid_c_host(t * a);
   return std::fma(t, a, c) - c;
+}
+
+// the GEMM's sum rounding (rgemm_tile.hpp: grid_round of c + y): (x + C) - C
+static void check_gemm_sum(uint32_t a, uint32_t b) {
+  if (a == pga::P_NAR || b == pga::P_NAR) return;
+  const double x = pga::to_double(a) + pga::to_double(b);
+  const double c = grid_c_host(x);
+  const double y = (x + c) - c;
+  ++gsum_checked;
+  if (y != y) {  // a scale outside the table: the caller redoes the slice
+    ++gsum_redone;
+    return;
+  }
+  const uint32_t want = orc_p32_add(a, b);
+  if (y != pga::to_double(want)) {
+    if (bad < 8) printf("gemm sum mismatch %08x + %08x: got %.17g want %08x\n", a, b, y, want);
+    ++bad;
+  }
 }
 
 static void check_mul(uint32_t a, uint32_t b) {
@@ -53,6 +76,7 @@ static void check_mul(uint32_t a, uint32_t b) {
 }
 
 static void check(uint32_t a, uint32_t b) {
+  check_gemm_sum(a, b);
   if (a == pga::P_NAR || b == pga::P_NAR) return;
   uint32_t reach = 0;
   const double y = pga::fp64_posit_add(pga::to_double(a), pga::to_double(b), reach);
@@ -145,7 +169,7 @@ int main(int argc, char** argv) {
       t = orc_p32_add(t, p);
     }
   }
-  printf("checked %ld redone %ld mismatches %ld products %ld redone %ld\n", checked, redone, bad, mul_checked,
-         mul_redone);
+  printf("checked %ld redone %ld mismatches %ld products %ld redone %ld gemm-sums %ld redone %ld\n", checked,
+         redone, bad, mul_checked, mul_redone, gsum_checked, gsum_redone);
   return bad ? 1 : 0;
 }
