@@ -315,27 +315,26 @@ __device__ __forceinline__ void gemm_tile_impl_int(int m, int n, int k, uint32_t
 }
 
 // The tile body with the FP64 multiply-add (fp64_mac), for several elements per lane.
+// No range checks run before a slice: a product or sum whose scale leaves -108 .. 107,
+// and any NaR, reads an infinite rounding constant and turns into NaN, which stays NaN
+// to the end of the slice; a wave that finds a NaN redoes the slice from its saved
+// accumulators with the pattern-level operations.
 template <int BM, int BN, int BK, int TM, int TN, bool TA, bool TB, bool SKIP0>
 __device__ __forceinline__ void gemm_tile_impl_fp64(int m, int n, int k, uint32_t alpha,
                                                const uint32_t* __restrict__ A, int lda,
                                                const uint32_t* __restrict__ B, int ldb, uint32_t beta,
                                                uint32_t* __restrict__ C, int ldc, int i0, int j0, int tri,
-                                               double (*__restrict__ sA)[BM], double (*__restrict__ sT)[BN],
-                                               int (*__restrict__ sBound)[4]) {
+                                               double (*__restrict__ sA)[BM], double (*__restrict__ sT)[BN]) {
   constexpr int NX = BM / TM;
   constexpr int NY = BN / TN;
   constexpr int NT = NX * NY;
-  constexpr int NW = NT / 64;
   constexpr bool SUM_FIRST = TA;  // TN and TT sum a*b from zero, then scale
-  constexpr int U = dec::FAST_MAX - growth_bits(BK);
   static_assert((BM * BK) % NT == 0 && (BN * BK) % NT == 0, "tile loads");
   static_assert(NT % 64 == 0, "whole waves");
-  static_assert(NW <= 16, "slice-bound slots in TileLds");
 
   const int tid = threadIdx.x;
   const int tx = tid % NX;
   const int ty = tid / NX;
-  const int wave = tid / 64;
 
   round_table_fill(tid, NT);  // visible after the first slice's barrier
 
@@ -359,8 +358,7 @@ __device__ __forceinline__ void gemm_tile_impl_fp64(int m, int n, int k, uint32_
   const bool alpha_mone = (alpha == P_MONE);
 
   for (int kt = 0; kt < k; kt += BK) {
-    // ---- stage the A slice (BM x BK) and the B-side slice (BK x BN), decoded ----
-    int amax = INT32_MIN, amin = INT32_MAX, tmax = INT32_MIN, tmin = INT32_MAX;
+    // ---- stage the A slice (BM x BK) and the B-side slice (BK x BN) as FP64 values ----
 #pragma unroll
     for (int q = 0; q < (BM * BK) / NT; ++q) {
       int e = tid + q * NT;
@@ -375,11 +373,7 @@ __device__ __forceinline__ void gemm_tile_impl_fp64(int m, int n, int k, uint32_
       int i = i0 + ii, l = kt + ll;
       uint32_t v = 0;
       if (i < m && l < k) v = TA ? A[l + (size_t)i * lda] : A[i + (size_t)l * lda];
-      const double dv = pga::to_double(v);  // exact; NaR -> NaN
-      sA[ll][ii] = dv;
-      const int sc = (v == 0) ? dec::ZERO_SC : (v == P_NAR ? dec::NAR_SC : fp64_scale(dv));
-      amax = max(amax, sc);
-      if (v != 0) amin = min(amin, sc);
+      sA[ll][ii] = pga::to_double(v);  // exact; NaR -> NaN
     }
 #pragma unroll
     for (int q = 0; q < (BN * BK) / NT; ++q) {
@@ -398,117 +392,67 @@ __device__ __forceinline__ void gemm_tile_impl_fp64(int m, int n, int k, uint32_
       // alpha*b; with SKIP0 a zero b stays zero (it marks a skipped step even if alpha is NaR)
       if (!SUM_FIRST && !(SKIP0 && v == 0))
         v = alpha_one ? v : (alpha_mone ? pga::neg(v) : pga::mul(alpha, v));
-      const double dv = pga::to_double(v);
-      sT[ll][jj] = dv;
-      const int sc = (v == 0) ? dec::ZERO_SC : (v == P_NAR ? dec::NAR_SC : fp64_scale(dv));
-      tmax = max(tmax, sc);
-      if (v != 0) tmin = min(tmin, sc);
-    }
-    // wave reductions of the slice bounds
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      amax = max(amax, __shfl_xor(amax, off));
-      amin = min(amin, __shfl_xor(amin, off));
-      tmax = max(tmax, __shfl_xor(tmax, off));
-      tmin = min(tmin, __shfl_xor(tmin, off));
-    }
-    if ((tid & 63) == 0) {
-      sBound[wave][0] = amax;
-      sBound[wave][1] = amin;
-      sBound[wave][2] = tmax;
-      sBound[wave][3] = tmin;
+      sT[ll][jj] = pga::to_double(v);
     }
     __syncthreads();
-#pragma unroll
-    for (int w = 0; w < NW; ++w) {
-      amax = max(amax, sBound[w][0]);
-      amin = min(amin, sBound[w][1]);
-      tmax = max(tmax, sBound[w][2]);
-      tmin = min(tmin, sBound[w][3]);
-    }
-    // products of nonzero operands lie in [amin + tmin, amax + tmax + 1]
-    bool fast = (amax + tmax + 1 <= U);
-    if (amin != INT32_MAX && tmin != INT32_MAX) fast = fast && (amin + tmin >= dec::FAST_MIN);
-    // accumulators of this lane (zeros have scales far below every bound)
-    int cmax = INT32_MIN;
+
+    const int lmax = min(BK, k - kt);
+    double saved[TM][TN];
 #pragma unroll
     for (int r = 0; r < TM; ++r)
 #pragma unroll
-      for (int c = 0; c < TN; ++c) {
-        // NaN (NaR) has exponent field 2047: above every bound
-        const int sc = (acc[r][c] == 0.0) ? dec::ZERO_SC : fp64_scale(acc[r][c]);
-        cmax = max(cmax, sc + 1);
-      }
-    fast = fast && (cmax <= U);
-    const bool wave_fast = __all(fast);
-
-    const int lmax = min(BK, k - kt);
-    bool redo = !wave_fast;
-    if (wave_fast) {
-      double saved[TM][TN];
+      for (int c = 0; c < TN; ++c) saved[r][c] = acc[r][c];
+    // fp64_mac on every element, in phases: all products and their table reads, then
+    // all roundings of the products and the sums, then all sums' table reads, then the
+    // sums' roundings -- every table read of a phase is in flight before the first
+    // result is used (the scheduling barriers keep the compiler from interleaving them
+    // one multiply-add at a time)
+    auto step = [&](int l) {
+      double ad[TM], td[TN], y[TM][TN], cc[TM][TN];
+#pragma unroll
+      for (int r = 0; r < TM; ++r) ad[r] = sA[l][tx + r * NX];
+#pragma unroll
+      for (int c = 0; c < TN; ++c) td[c] = sT[l][ty + c * NY];
 #pragma unroll
       for (int r = 0; r < TM; ++r)
 #pragma unroll
-        for (int c = 0; c < TN; ++c) saved[r][c] = acc[r][c];
-      // fp64_mac on every element, in phases: all products and their table reads, then
-      // all roundings of the products and the sums, then all sums' table reads, then the
-      // sums' roundings -- every table read of a phase is in flight before the first
-      // result is used (the scheduling barriers keep the compiler from interleaving them
-      // one multiply-add at a time)
-      auto step = [&](int l) {
-        double ad[TM], td[TN], y[TM][TN], cc[TM][TN];
-#pragma unroll
-        for (int r = 0; r < TM; ++r) ad[r] = sA[l][tx + r * NX];
-#pragma unroll
-        for (int c = 0; c < TN; ++c) td[c] = sT[l][ty + c * NY];
-#pragma unroll
-        for (int r = 0; r < TM; ++r)
-#pragma unroll
-          for (int c = 0; c < TN; ++c) cc[r][c] = grid_c(td[c] * ad[r]);
-        __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-        for (int r = 0; r < TM; ++r)
-#pragma unroll
-          for (int c = 0; c < TN; ++c) {
-            y[r][c] = acc[r][c] + (__builtin_fma(td[c], ad[r], cc[r][c]) - cc[r][c]);
-            cc[r][c] = grid_c(y[r][c]);
-          }
-        __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-        for (int r = 0; r < TM; ++r)
-#pragma unroll
-          for (int c = 0; c < TN; ++c) acc[r][c] = (y[r][c] + cc[r][c]) - cc[r][c];
-      };
-      // two steps per iteration: the loop then carries no register rotation
-      int l = 0;
-      for (; l + 1 < lmax; l += 2) {
-        step(l);
-        step(l + 1);
-      }
-      if (l < lmax) step(l);
-      bool bad = false;
+        for (int c = 0; c < TN; ++c) cc[r][c] = grid_c(td[c] * ad[r]);
+      __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int r = 0; r < TM; ++r)
 #pragma unroll
-        for (int c = 0; c < TN; ++c) bad |= acc[r][c] != acc[r][c];  // NaN: a scale left the table
-      if (__any(bad)) {
-        redo = true;
+        for (int c = 0; c < TN; ++c) {
+          y[r][c] = acc[r][c] + (__builtin_fma(td[c], ad[r], cc[r][c]) - cc[r][c]);
+          cc[r][c] = grid_c(y[r][c]);
+        }
+      __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int r = 0; r < TM; ++r)
+      for (int r = 0; r < TM; ++r)
 #pragma unroll
-          for (int c = 0; c < TN; ++c) acc[r][c] = saved[r][c];
-      }
+        for (int c = 0; c < TN; ++c) acc[r][c] = (y[r][c] + cc[r][c]) - cc[r][c];
+    };
+    // two steps per iteration: the loop then carries no register rotation
+    int l = 0;
+    for (; l + 1 < lmax; l += 2) {
+      step(l);
+      step(l + 1);
     }
-    if (redo) {
+    if (l < lmax) step(l);
+    bool bad = false;
+#pragma unroll
+    for (int r = 0; r < TM; ++r)
+#pragma unroll
+      for (int c = 0; c < TN; ++c) bad |= acc[r][c] != acc[r][c];  // NaN: a scale left the table, or NaR
+    if (__any(bad)) {
       // exact path for this slice: pattern-level SoftPosit operations
 #pragma unroll
       for (int r = 0; r < TM; ++r) {
 #pragma unroll
         for (int c = 0; c < TN; ++c) {
-          uint32_t pc = pga::from_double(acc[r][c]);  // exact; NaN -> NaR
-          for (int l = 0; l < lmax; ++l) {
-            const uint32_t ap = pga::from_double(sA[l][tx + r * NX]);
-            const uint32_t tp = pga::from_double(sT[l][ty + c * NY]);
+          uint32_t pc = pga::from_double(saved[r][c]);  // exact; NaN -> NaR
+          for (int l2 = 0; l2 < lmax; ++l2) {
+            const uint32_t ap = pga::from_double(sA[l2][tx + r * NX]);
+            const uint32_t tp = pga::from_double(sT[l2][ty + c * NY]);
             if (SKIP0 && tp == 0) continue;  // Rsyrk.cpp:149,169 skip zero multipliers
             pc = pga::add(pc, pga::mul(tp, ap));
           }
@@ -554,7 +498,7 @@ __device__ __forceinline__ void gemm_tile_impl(int m, int n, int k, uint32_t alp
     double* d = reinterpret_cast<double*>(lds);
     gemm_tile_impl_fp64<BM, BN, BK, TM, TN, TA, TB, SKIP0>(
         m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, i0, j0, tri, reinterpret_cast<double(*)[BM]>(d),
-        reinterpret_cast<double(*)[BN]>(d + BK * BM), reinterpret_cast<int(*)[4]>(d + BK * BM + BK * BN));
+        reinterpret_cast<double(*)[BN]>(d + BK * BM));
   } else {
     gemm_tile_impl_int<BM, BN, BK, TM, TN, TA, TB, SKIP0>(
         m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, i0, j0, tri, reinterpret_cast<uint32_t(*)[BM]>(lds),
