@@ -60,7 +60,10 @@ pga_status pga_get_device_info(pga_device_info *info);
  * (validation).  Also set by PGA_GEMM_IMPL=tiled|exact.  PGA_GEMM_TILE=16|32|64 (read at
  * initialisation; 64 selects the 64 x 32 tile) fixes the tiled kernel's tile instead of the
  * per-shape choice (the role of the reference's OPENCL_GEMM_BLOCKSIZE,
- * Rgemm_opencl.cpp:62-67); the bits do not depend on it. */
+ * Rgemm_opencl.cpp:62-67); the bits do not depend on it.  The 64 x 32 tile of GEMMs with
+ * m*n*k >= 2^30 (PGA_GEMM_PANEL_MIN) converts A and alpha*B to FP64 operand panels once,
+ * in workspace from a library memory pool, instead of in every tile; PGA_GEMM_PANELS=0
+ * turns that off (same bits either way). */
 pga_status pga_set_gemm_impl(int impl);
 
 /* ---- device memory and streams (plumbing for resident data) ------------------- */

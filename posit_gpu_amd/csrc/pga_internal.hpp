@@ -78,6 +78,9 @@ struct Runtime {
   GemmImpl gemm_impl = GemmImpl::Tiled;
   int cus = 256;
   int gemm_tile = 0;  // PGA_GEMM_TILE: 0 per shape, else the tile edge of the plain GEMM
+  bool gemm_panels = true;         // PGA_GEMM_PANELS=0: large GEMMs decode in every tile
+  long long panel_min_work = 1LL << 30;  // m*n*k from which the 64 x 32 tile uses panels
+  hipMemPool_t pool = nullptr;     // stream-ordered workspace of the GEMM panels
 };
 Runtime& runtime();
 

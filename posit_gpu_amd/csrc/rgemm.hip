@@ -63,6 +63,56 @@ __global__ __launch_bounds__((BM / TM) * (BN / TN), 3) void rgemm_batched_persis
   }
 }
 
+// Panel form of the 64 x 32 tile: the slices come from FP64 operand panels (Panels).
+template <bool TA, bool TB, bool SKIP0>
+__global__ __launch_bounds__(256, 3) void rgemm_tiled_panel(int m, int n, int k, uint32_t alpha,
+                                                         const uint32_t* __restrict__ A, int lda,
+                                                         const uint32_t* __restrict__ B, int ldb, uint32_t beta,
+                                                         uint32_t* __restrict__ C, int ldc, int tiles_m, int tri,
+                                                         int urgent, const int* __restrict__ stop, Panels pn) {
+  PGA_KERNEL_VGPR_FLOOR();
+  if (urgent) PGA_URGENT();
+  const int bid = blockIdx.x;
+  const int i0 = (bid % tiles_m) * 64;
+  const int j0 = (bid / tiles_m) * 32;
+  if (stop && *stop) return;
+  if (tri == 1 && i0 + 63 < j0) return;
+  if (tri == 2 && j0 + 31 < i0) return;
+  __shared__ __align__(16) double sA[16][64];
+  __shared__ __align__(16) double sT[16][32];
+  gemm_tile_impl_fp64<64, 32, 16, 4, 2, TA, TB, SKIP0, true>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, i0, j0,
+                                                              tri, sA, sT, pn);
+}
+
+// The panels: one workgroup per slice (i-block or j-block, k-slice), lanes walking the
+// source matrix's leading dimension so that the reads coalesce.
+template <bool TA>
+__global__ __launch_bounds__(256) void k_panel_a(int m, int k, const uint32_t* __restrict__ A, int lda,
+                                                 double* __restrict__ pa, int ks) {
+  PGA_KERNEL_VGPR_FLOOR();
+  const int ib = blockIdx.x / ks, s = blockIdx.x % ks;
+  double* out = pa + (size_t)blockIdx.x * (16 * 64);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int e = threadIdx.x + q * 256;
+    const int ll = TA ? e % 16 : e / 64, ii = TA ? e / 16 : e % 64;
+    out[ll * 64 + ii] = panel_a_elem<64, 16, TA>(m, k, A, lda, ib, s, ll, ii);
+  }
+}
+template <bool TB, bool SUM_FIRST, bool SKIP0>
+__global__ __launch_bounds__(256) void k_panel_t(int n, int k, uint32_t alpha, const uint32_t* __restrict__ B,
+                                                 int ldb, double* __restrict__ pt, int ks) {
+  PGA_KERNEL_VGPR_FLOOR();
+  const int jb = blockIdx.x / ks, s = blockIdx.x % ks;
+  double* out = pt + (size_t)blockIdx.x * (16 * 32);
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const int e = threadIdx.x + q * 256;
+    const int ll = TB ? e / 32 : e % 16, jj = TB ? e % 32 : e / 16;
+    out[ll * 32 + jj] = panel_t_elem<32, 16, TB, SUM_FIRST, SKIP0>(n, k, alpha, B, ldb, jb, s, ll, jj);
+  }
+}
+
 __global__ void reset_counter(int* counter) {
   PGA_KERNEL_VGPR_FLOOR();
   if (threadIdx.x == 0) *counter = 0;
@@ -144,9 +194,45 @@ static int pick_tile(int m, int n, int cus) {
 }
 
 template <bool TA, bool TB, bool SKIP0>
+static hipError_t launch_panel(int m, int n, int k, uint32_t alpha, const uint32_t* A, int lda, const uint32_t* B,
+                               int ldb, uint32_t beta, uint32_t* C, int ldc, hipStream_t s, int tri, int urgent,
+                               const int* stop, bool& done) {
+  done = false;
+  hipMemPool_t pool = runtime().pool;
+  if (!pool) return hipSuccess;
+  const int tm = (m + 63) / 64, tn = (n + 31) / 32, ks = (k + 15) / 16;
+  const long long blocks = (long long)tm * tn;
+  if (blocks > 0x7FFFFFFF || (long long)tm * ks > 0x7FFFFFFF || (long long)tn * ks > 0x7FFFFFFF) return hipSuccess;
+  const size_t a_elems = (size_t)tm * ks * 16 * 64, t_elems = (size_t)tn * ks * 16 * 32;
+  void* ws = nullptr;
+  if (hipMallocFromPoolAsync(&ws, (a_elems + t_elems) * sizeof(double), pool, s) != hipSuccess) {
+    (void)hipGetLastError();  // no workspace: the tile decodes its own slices
+    return hipSuccess;
+  }
+  done = true;
+  Panels pn{static_cast<double*>(ws), static_cast<double*>(ws) + a_elems, ks};
+  hipLaunchKernelGGL((k_panel_a<TA>), dim3(tm * ks), dim3(256), 0, s, m, k, A, lda, const_cast<double*>(pn.a), ks);
+  hipLaunchKernelGGL((k_panel_t<TB, TA, SKIP0>), dim3(tn * ks), dim3(256), 0, s, n, k, alpha, B, ldb,
+                     const_cast<double*>(pn.t), ks);
+  hipLaunchKernelGGL((rgemm_tiled_panel<TA, TB, SKIP0>), dim3((unsigned)blocks), dim3(256), 0, s, m, n, k, alpha, A,
+                     lda, B, ldb, beta, C, ldc, tm, tri, urgent, stop, pn);
+  hipError_t e = hipGetLastError();
+  hipError_t f = hipFreeAsync(ws, s);
+  return e != hipSuccess ? e : f;
+}
+
+template <bool TA, bool TB, bool SKIP0>
 static hipError_t launch_tiled(int m, int n, int k, uint32_t alpha, const uint32_t* A, int lda,
                                const uint32_t* B, int ldb, uint32_t beta, uint32_t* C, int ldc,
                                hipStream_t s, int tri, int urgent, const int* stop, int tile) {
+  // panels from m*n*k = 2^30 on (runtime().panel_min_work): below it the two extra launches
+  // and the workspace cost more than the staging they save
+  if (tile == 64 && runtime().gemm_panels && (long long)m * n * k >= runtime().panel_min_work) {
+    bool done;
+    hipError_t e = launch_panel<TA, TB, SKIP0>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, tri, urgent, stop,
+                                               done);
+    if (done || e != hipSuccess) return e;
+  }
   if (tile == 64)
     return launch_tiled_cfg<64, 4, TA, TB, SKIP0>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, tri, urgent, stop);
   if (tile == 32)

@@ -314,17 +314,32 @@ __device__ __forceinline__ void gemm_tile_impl_int(int m, int n, int k, uint32_t
   }
 }
 
+// Operand panels (PANEL form): A and the B side (alpha*B for NN / NT, B for TN / TT) held
+// as FP64 values in HBM, cut into the slices the tile stages, each slice contiguous in
+// the tile's LDS order ([l][i] for A, [l][j] for the B side) and zero-padded at the edges:
+// A's slice (i-block ib, k-slice s) starts at PA + (ib * ks + s) * BK * BM, the B side's
+// (j-block jb, s) at PT + (jb * ks + s) * BK * BN, with ks = ceil(k / BK).  A GEMM whose A
+// slice serves N / BN tiles and whose B slice serves M / BM tiles then converts each
+// operand once instead of once per tile.
+struct Panels {
+  const double* a;
+  const double* t;
+  int ks;
+};
+
 // The tile body with the FP64 multiply-add (fp64_mac), for several elements per lane.
 // No range checks run before a slice: a product or sum whose scale leaves -108 .. 107,
 // and any NaR, reads an infinite rounding constant and turns into NaN, which stays NaN
 // to the end of the slice; a wave that finds a NaN redoes the slice from its saved
-// accumulators with the pattern-level operations.
-template <int BM, int BN, int BK, int TM, int TN, bool TA, bool TB, bool SKIP0>
+// accumulators with the pattern-level operations.  With PANEL the slices are copied from
+// pre-converted operand panels (pn) instead of decoded from A and B.
+template <int BM, int BN, int BK, int TM, int TN, bool TA, bool TB, bool SKIP0, bool PANEL = false>
 __device__ __forceinline__ void gemm_tile_impl_fp64(int m, int n, int k, uint32_t alpha,
                                                const uint32_t* __restrict__ A, int lda,
                                                const uint32_t* __restrict__ B, int ldb, uint32_t beta,
                                                uint32_t* __restrict__ C, int ldc, int i0, int j0, int tri,
-                                               double (*__restrict__ sA)[BM], double (*__restrict__ sT)[BN]) {
+                                               double (*__restrict__ sA)[BM], double (*__restrict__ sT)[BN],
+                                               Panels pn = Panels{}) {
   constexpr int NX = BM / TM;
   constexpr int NY = BN / TN;
   constexpr int NT = NX * NY;
@@ -358,6 +373,24 @@ __device__ __forceinline__ void gemm_tile_impl_fp64(int m, int n, int k, uint32_
   const bool alpha_mone = (alpha == P_MONE);
 
   for (int kt = 0; kt < k; kt += BK) {
+    if constexpr (PANEL) {
+      // ---- copy the slices from the panels: 16-byte pieces, in the LDS order ----
+      static_assert((BM * BK) % (2 * NT) == 0 && (BN * BK) % (2 * NT) == 0, "panel copies");
+      const double2* pa = reinterpret_cast<const double2*>(pn.a + ((size_t)(i0 / BM) * pn.ks + kt / BK) * BK * BM);
+      const double2* pt = reinterpret_cast<const double2*>(pn.t + ((size_t)(j0 / BN) * pn.ks + kt / BK) * BK * BN);
+      double2* la = reinterpret_cast<double2*>(&sA[0][0]);
+      double2* lt = reinterpret_cast<double2*>(&sT[0][0]);
+      double2 va[(BM * BK) / (2 * NT)], vt[(BN * BK) / (2 * NT)];
+#pragma unroll
+      for (int q = 0; q < (BM * BK) / (2 * NT); ++q) va[q] = pa[tid + q * NT];
+#pragma unroll
+      for (int q = 0; q < (BN * BK) / (2 * NT); ++q) vt[q] = pt[tid + q * NT];
+#pragma unroll
+      for (int q = 0; q < (BM * BK) / (2 * NT); ++q) la[tid + q * NT] = va[q];
+#pragma unroll
+      for (int q = 0; q < (BN * BK) / (2 * NT); ++q) lt[tid + q * NT] = vt[q];
+      __syncthreads();
+    } else {
     // ---- stage the A slice (BM x BK) and the B-side slice (BK x BN) as FP64 values ----
 #pragma unroll
     for (int q = 0; q < (BM * BK) / NT; ++q) {
@@ -395,6 +428,7 @@ __device__ __forceinline__ void gemm_tile_impl_fp64(int m, int n, int k, uint32_
       sT[ll][jj] = pga::to_double(v);
     }
     __syncthreads();
+    }
 
     const int lmax = min(BK, k - kt);
     double saved[TM][TN];
@@ -480,6 +514,28 @@ __device__ __forceinline__ void gemm_tile_impl_fp64(int m, int n, int k, uint32_
       }
     }
   }
+}
+
+// One element of the panels: A's slice element (ll, ii) of i-block ib, k-slice s, or the
+// B side's (ll, jj) of j-block jb -- zero outside the matrix, alpha applied as the tile
+// applies it.
+template <int BM, int BK, bool TA>
+__device__ __forceinline__ double panel_a_elem(int m, int k, const uint32_t* __restrict__ A, int lda, int ib, int s,
+                                               int ll, int ii) {
+  const int i = ib * BM + ii, l = s * BK + ll;
+  uint32_t v = 0;
+  if (i < m && l < k) v = TA ? A[l + (size_t)i * lda] : A[i + (size_t)l * lda];
+  return pga::to_double(v);
+}
+template <int BN, int BK, bool TB, bool SUM_FIRST, bool SKIP0>
+__device__ __forceinline__ double panel_t_elem(int n, int k, uint32_t alpha, const uint32_t* __restrict__ B, int ldb,
+                                               int jb, int s, int ll, int jj) {
+  const int j = jb * BN + jj, l = s * BK + ll;
+  uint32_t v = 0;
+  if (j < n && l < k) v = TB ? B[j + (size_t)l * ldb] : B[l + (size_t)j * ldb];
+  if (!SUM_FIRST && !(SKIP0 && v == 0))
+    v = (alpha == P_ONE) ? v : (alpha == P_MONE ? pga::neg(v) : pga::mul(alpha, v));
+  return pga::to_double(v);
 }
 
 // One BM x BN tile of C at (i0, j0): the body shared by the plain and the batched kernel.

@@ -219,6 +219,28 @@ pga_status pga_init(int device) {
   }
   // the fixed device array of the no-allocation batched GEMM, for this device
   PGA_HIP(batched_noalloc_setup(device));
+  // a memory pool for the GEMM's operand panels that keeps what it has reserved (workspace
+  // taken and returned in stream order on every large GEMM); without it the GEMM decodes
+  // its operands in every tile
+  {
+    hipMemPoolProps props = {};
+    props.allocType = hipMemAllocationTypePinned;
+    props.handleTypes = hipMemHandleTypeNone;
+    props.location.type = hipMemLocationTypeDevice;
+    props.location.id = device;
+    if (hipMemPoolCreate(&g_rt.pool, &props) == hipSuccess) {
+      uint64_t keep = ~0ull;
+      if (hipMemPoolSetAttribute(g_rt.pool, hipMemPoolAttrReleaseThreshold, &keep) != hipSuccess) {
+        (void)hipMemPoolDestroy(g_rt.pool);
+        g_rt.pool = nullptr;
+      }
+    } else {
+      g_rt.pool = nullptr;
+    }
+    (void)hipGetLastError();
+  }
+  if (const char* pn = getenv("PGA_GEMM_PANELS")) g_rt.gemm_panels = atoi(pn) != 0;
+  if (const char* pm = getenv("PGA_GEMM_PANEL_MIN")) g_rt.panel_min_work = atoll(pm);
   const char* impl = getenv("PGA_GEMM_IMPL");
   if (impl && strcmp(impl, "exact") == 0) g_rt.gemm_impl = GemmImpl::Exact;
   // the tile edge of the plain GEMM (the role of OPENCL_GEMM_BLOCKSIZE in the reference):
@@ -241,6 +263,11 @@ pga_status pga_finalize(void) {
     if (s.ptr) (void)hipFree(s.ptr);
     s.ptr = nullptr;
     s.bytes = 0;
+  }
+  if (g_rt.pool) {
+    (void)hipDeviceSynchronize();
+    (void)hipMemPoolDestroy(g_rt.pool);
+    g_rt.pool = nullptr;
   }
   (void)hipStreamDestroy(g_rt.stream);
   g_rt.stream = nullptr;

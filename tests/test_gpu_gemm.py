@@ -256,7 +256,7 @@ def _class_matrix(oracle, rng, kind, r, c, seed):
     return np.asfortranarray(np.asarray(v, np.uint32).reshape(c, r).T)
 
 
-@pytest.mark.parametrize("tile", ["64", "32", "16"])
+@pytest.mark.parametrize("tile", ["64", "32", "16", "64-panels"])
 def test_value_classes_at_each_tile(pga, oracle, tmp_path, tile):
     """Every value class through the tile that serves large GEMMs: a process started with
     PGA_GEMM_TILE fixed runs all four variants and three (alpha, beta) pairs, with K long
@@ -286,7 +286,9 @@ def test_value_classes_at_each_tile(pga, oracle, tmp_path, tile):
                                         beta, w, nthreads=THREADS) == 0
                     want[case] = w
     np.savez(tmp_path / "in.npz", **arrays)
-    env = dict(os.environ, PGA_GEMM_TILE=tile)
+    env = dict(os.environ, PGA_GEMM_TILE=tile.split("-")[0])
+    if tile.endswith("panels"):  # the 64 x 32 tile with its operands from FP64 panels
+        env["PGA_GEMM_PANEL_MIN"] = "0"
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     subprocess.run([sys.executable, "-c", _TILE_SCRIPT.format(root=root), str(tmp_path / "in.npz"),
                     str(tmp_path / "out.npz")], env=env, check=True, timeout=240)
@@ -295,3 +297,16 @@ def test_value_classes_at_each_tile(pga, oracle, tmp_path, tile):
     for case, w in want.items():
         bad = np.argwhere(got[case] != w)
         assert bad.size == 0, f"tile {tile} {case}: {len(bad)} mismatches, first {bad[:3].tolist()}"
+
+
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("N", "T"), ("T", "N"), ("T", "T")])
+def test_panels_ragged_vs_oracle(pga, oracle, ta, tb):
+    """A GEMM above the panel threshold (m*n*k >= 2^30) with ragged edges in m, n and K:
+    the 64 x 32 tile takes its operands from FP64 panels; every element against the oracle."""
+    m, n, k = 1500, 1500, 500
+    A = posit_matrix(oracle, 61, k if ta == "T" else m, m if ta == "T" else k)
+    B = posit_matrix(oracle, 62, n if tb == "T" else k, k if tb == "T" else n)
+    C = posit_matrix(oracle, 63, m, n)
+    for alpha, beta in [(MONE, ONE), (0x3A000000, 0xC8000000)]:
+        got, want = _both(oracle, pga, ta, tb, alpha, beta, A, B, C)
+        _assert_same(got, want)
