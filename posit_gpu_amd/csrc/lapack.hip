@@ -1016,37 +1016,43 @@ static hipError_t getrf_update(const Ctx& c, int m, uint32_t* a, int lda, const 
 // Orders from which the LU runs with one panel of lookahead
 constexpr int GETRF_LOOKAHEAD_MIN = 1024;
 
-// Rgetrf_opencl with one panel of lookahead.  After panel j, the caller's stream updates
+// Rgetrf_opencl with one panel of lookahead.  After panel j, the chain stream updates
 // only block column j+1 and factors it as panel j+1, while a second stream updates the
 // columns from j+2 on (the bulk of the work, one large GEMM) and applies panel j's
 // interchanges to the columns left of it.  Each element sees the same operations in the
 // same order as in the sequential driver: block column j+1 waits for the second stream's
 // update by block j-1, and the left interchanges of panel j run after that update has
-// read panel j-1.
-static hipError_t getrf_lookahead(const Ctx& c, int m, int n, uint32_t* a, int lda, int* ipiv, int* info,
-                                  int nb) {
+// read panel j-1.  `join` and `done` are two events owned by this call.
+static hipError_t getrf_lookahead_on(const Ctx& c, hipStream_t wide, hipEvent_t join, hipEvent_t done, int m, int n,
+                                     uint32_t* a, int lda, int* ipiv, int* info, int nb) {
   const int mn = m < n ? m : n;
-  Aux x;
-  PGA_TRY_E(aux_take(x));
-  const Ctx cx{x.s, c.impl};
+  const Ctx cx{wide, c.impl};
   hipError_t e = getrf2(c, m, std::min(nb, mn), a, lda, ipiv, info, 0, 0);
-  bool recorded = false;  // the aux stream has recorded x.done at least once
+  bool recorded = false;  // the wide stream has recorded `done` at least once
   for (int j = 0; j < mn && e == hipSuccess; j += nb) {
     const int jb = std::min(nb, mn - j), c0 = j + jb, w1 = std::min(nb, n - c0), c1 = c0 + std::max(w1, 0);
-    if ((e = hipEventRecord(x.join, c.s)) != hipSuccess) break;
-    if ((e = hipStreamWaitEvent(x.s, x.join, 0)) != hipSuccess) break;
+    if ((e = hipEventRecord(join, c.s)) != hipSuccess) break;
+    if ((e = hipStreamWaitEvent(wide, join, 0)) != hipSuccess) break;
     if (j > 0 && (e = laswp(cx, j, a, lda, j + 1, j + jb, ipiv, 1, 0, nullptr)) != hipSuccess) break;
     if ((e = getrf_update(cx, m, a, lda, ipiv, j, jb, c1, n - c1, false)) != hipSuccess) break;
-    if (recorded && (e = hipStreamWaitEvent(c.s, x.done, 0)) != hipSuccess) break;
-    if ((e = hipEventRecord(x.done, x.s)) != hipSuccess) break;
+    if (recorded && (e = hipStreamWaitEvent(c.s, done, 0)) != hipSuccess) break;
+    if ((e = hipEventRecord(done, wide)) != hipSuccess) break;
     recorded = true;
     if ((e = getrf_update(c, m, a, lda, ipiv, j, jb, c0, w1, true)) != hipSuccess) break;
     if (c0 < mn) e = getrf2(c, m - c0, std::min(nb, mn - c0), a + c0 + (size_t)c0 * lda, lda, ipiv + c0, info, c0, c0);
   }
   if (recorded) {
-    hipError_t w = hipStreamWaitEvent(c.s, x.done, 0);
+    hipError_t w = hipStreamWaitEvent(c.s, done, 0);
     if (e == hipSuccess) e = w;
   }
+  return e;
+}
+
+// The lookahead driver on the caller's stream and an Aux stream.
+static hipError_t getrf_lookahead(const Ctx& c, int m, int n, uint32_t* a, int lda, int* ipiv, int* info, int nb) {
+  Aux x;
+  PGA_TRY_E(aux_take(x));
+  hipError_t e = getrf_lookahead_on(c, x.s, x.join, x.done, m, n, a, lda, ipiv, info, nb);
   aux_give(x);
   return e;
 }
