@@ -83,17 +83,19 @@ __global__ __launch_bounds__(WG) void k_trsm_left(int m, int n, uint32_t alpha, 
 
 // Left, lower, transposed (Rtrsm.cpp:174-186), one right-hand side per workgroup:
 // temp_i = alpha*b_i - sum_{k>i} a_ki*b_k in ascending k, i descending.  Row i's first
-// term needs b_{i+1}, so the rows form one chain of m^2/2 rounded additions, and a
-// single wave's instruction issue bounds it.  Per row the lanes form the negated
-// products (posits, exact in FP64) in LDS, CHAIN_CHUNK terms at a time; wave 0 then
-// runs the additions, every lane computing the same values, eight terms a group while
-// the next group loads.
+// term needs b_{i+1}, so the rows form one chain of m^2/2 rounded additions.  Per row
+// the lanes form the negated products (posits, exact in FP64) in LDS, CHAIN_CHUNK terms
+// at a time; wave 0 then runs the additions 64 terms a block, one term per lane.
 //
-// One addition is fp64_posit_add (posit32_core.hpp): round(t + p) on the posit grid in
-// three FP64 additions.  A group in which one of them reports a case it does not round
-// (a scale outside -108 .. 107, an exact zero) is redone from its start with the exact
-// decoded additions.  A NaR term or start value makes the row NaR, as
-// NaR absorbs every addition.
+// While the running sum stays in one binade the additions are integer steps on that
+// binade's posit grid (chain_grid_*, posit32_core.hpp): each lane turns its term into a
+// step d (and a tie flag), one wave-wide prefix sum gives every lane the grid value
+// after its term, ties are settled in order (each rounds to the even value), and every
+// lane checks that its value lies strictly inside the binade.  The values before the
+// first lane that fails are the posit sums; that lane's sum is formed on its own with
+// fp64_posit_add (the exact decoded addition where that does not round it), the grid
+// is taken anew from the result, and the next block starts with the following term.
+// A NaR term or start value makes the row NaR, as NaR absorbs every addition.
 constexpr int CHAIN_CHUNK = 4096;
 
 // t <- t + p exactly for posit values t and p (finite), through the decoded form
@@ -103,51 +105,71 @@ __device__ __forceinline__ void chain_add_exact(double& t, double p) {
   t = to_double(dec::encode(v));
 }
 
-struct ChainGroup {
-  static constexpr int G = 8;
-  double p[G];
-  __device__ __forceinline__ void load(const double* sp, int k) {
-#pragma unroll
-    for (int u = 0; u < G; ++u) p[u] = sp[k + u];
-  }
-  __device__ __forceinline__ void apply(double& t) {
-    // keep the terms in VGPRs: uniform copies would be moved to SGPRs one word at a time
-#pragma unroll
-    for (int u = 0; u < G; ++u) asm volatile("" : "+v"(p[u]));
-    const double save = t;
-    uint32_t reach = 0;
-#pragma unroll
-    for (int u = 0; u < G; ++u) t = fp64_posit_add(t, p[u], reach);
-    if (__builtin_amdgcn_readfirstlane(reach) > FP64_ADD_REACH_MAX) {
-      t = save;
-#pragma unroll
-      for (int u = 0; u < G; ++u) chain_add_exact(t, p[u]);
-    }
-  }
-};
+// one posit addition of the chain, every lane alike
+__device__ __forceinline__ double chain_add_one(double t, double p) {
+  uint32_t reach = 0;
+  const double r = fp64_posit_add(t, p, reach);
+  if (__builtin_amdgcn_readfirstlane(reach) <= FP64_ADD_REACH_MAX) return r;
+  chain_add_exact(t, p);
+  return t;
+}
 
-// Two groups in flight: one loads from LDS while the other is added.
+// Inclusive prefix sum over the 64 lanes of a wave: shifts by 1, 2, 4 and 8 lanes within
+// each row of 16, then the last lane of row 0 added to row 1 and of row 2 to row 3, then
+// the last lane of row 1 to rows 2 and 3 (DPP row_shr / row_bcast; lanes without a source
+// add the 0 passed as the old value).
+__device__ __forceinline__ int wave_prefix_sum(int v) {
+  v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);  // row_shr:1
+  v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);  // row_shr:2
+  v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);  // row_shr:4
+  v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);  // row_shr:8
+  v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);  // row_bcast:15
+  v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);  // row_bcast:31
+  return v;
+}
+
+// The chain's additions over the kn terms in sp (LDS), by one full wave.
 __device__ __forceinline__ void chain_terms(double& t, const double* __restrict__ sp, int kn) {
-  constexpr int G = ChainGroup::G;
-  const int groups = kn / G;
-  ChainGroup x, y;
-  if (groups > 0) x.load(sp, 0);
-  for (int g = 0; g < groups; g += 2) {
-    if (g + 1 < groups) y.load(sp, (g + 1) * G);
-    x.apply(t);
-    if (g + 1 >= groups) break;
-    if (g + 2 < groups) x.load(sp, (g + 2) * G);
-    y.apply(t);
-  }
-  for (int k = groups * G; k < kn; ++k) {
-    const double save = t;
-    uint32_t reach = 0;
-    t = fp64_posit_add(t, sp[k], reach);
-    if (__builtin_amdgcn_readfirstlane(reach) > FP64_ADD_REACH_MAX) {
-      t = save;
-      chain_add_exact(t, sp[k]);
+  const int lane = (int)__builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+  ChainGrid g;
+  bool live = __builtin_amdgcn_readfirstlane((uint32_t)chain_grid_setup(t, g)) != 0;
+  int k = 0;
+  while (k < kn) {
+    if (!live) {  // zero or a scale outside the grids' range: one term on its own
+      t = chain_add_one(t, sp[k]);
+      ++k;
+      live = __builtin_amdgcn_readfirstlane((uint32_t)chain_grid_setup(t, g)) != 0;
+      continue;
     }
+    const int cnt = min(64, kn - k);
+    const bool valid = lane < cnt;
+    const double p = valid ? sp[k + lane] : 0.0;
+    int d;
+    bool tie;
+    chain_grid_term(p, g.qinv, d, tie);
+    const int D = wave_prefix_sum(valid ? d : 0);
+    // ties in lane order: tie w rounds m0 + D_w + (the ties before it) up to even
+    int c = 0, cs = 0;
+    for (uint64_t T = __builtin_amdgcn_ballot_w64(tie && valid); T; T &= T - 1) {
+      const int w = __builtin_ctzll(T);
+      const int up = (g.m0 + __builtin_amdgcn_readlane(D, w) + cs) & 1;
+      cs += up;
+      c += (up && lane >= w) ? 1 : 0;
+    }
+    const int m = g.m0 + D + c;
+    const uint64_t F = __builtin_amdgcn_ballot_w64(valid && !(m > g.lo && m < g.hi));
+    if (F == 0) {
+      g.m0 = __builtin_amdgcn_readlane(m, cnt - 1);
+      k += cnt;
+      continue;
+    }
+    const int us = __builtin_ctzll(F);  // first term whose sum left the binade
+    if (us > 0) g.m0 = __builtin_amdgcn_readlane(m, us - 1);
+    t = chain_add_one(chain_grid_value(g, g.m0), sp[k + us]);
+    k += us + 1;
+    live = __builtin_amdgcn_readfirstlane((uint32_t)chain_grid_setup(t, g)) != 0;
   }
+  if (live) t = chain_grid_value(g, g.m0);
 }
 
 __global__ __launch_bounds__(WG) void k_trsm_lltn(int m, int n, uint32_t alpha, int nounit,

@@ -372,6 +372,60 @@ PGA_HD uint32_t fp64_grid_chi(uint32_t hi) {
   return ((uint32_t)(eb + 25 + ak) << 20) | 0x00080000u;
 }
 
+// Serial sums of posit values on the grid of one binade, for the long addition chains of
+// the triangular solves.  For t in binade e (2^e <= |t| < 2^(e+1)) at scales -108 .. 107
+// the posits of the binade are the multiples of the quantum q = 2^(e - nf), nf = 27 - |k|'
+// fraction bits (fp64_posit_add), so t = m * q for an integer m with 2^nf <= |m| <
+// 2^(nf+1).  A sum t + p whose exact value lies inside the binade rounds to
+// (m + d) * q, d = floor(p/q) plus one where p/q's fraction exceeds 1/2; where the
+// fraction is exactly 1/2 (a tie) it rounds to the even one of m + floor(p/q) and that
+// plus one, which is the even posit pattern.  A run of sums is thus integer arithmetic
+// on m, and a sum whose result (m + d) lies strictly inside the binade's range had its
+// exact value inside the binade (within half a unit of it), so it was the posit sum.
+struct ChainGrid {
+  int m0;      // t = m0 * 2^qe
+  int qe;      // exponent of the quantum q
+  int lo, hi;  // the binade on t's side of zero: lo < m < hi strictly inside it
+  double qinv; // 1/q
+};
+
+// The grid of t's binade; false for zero and for scales outside -108 .. 107.
+PGA_HD bool chain_grid_setup(double t, ChainGrid& g) {
+  uint64_t tb;
+  __builtin_memcpy(&tb, &t, sizeof tb);
+  const uint32_t th = (uint32_t)(tb >> 32);
+  const int eb = (int)((th >> 20) & 0x7FFu), e = eb - 1023;
+  if (eb == 0 || e < -108 || e > 107) return false;
+  const int ak = (e >> 2) ^ (e >> 31), nf = 27 - ak;
+  g.qe = e - nf;
+  const uint64_t qib = (uint64_t)(1023 - g.qe) << 52;
+  __builtin_memcpy(&g.qinv, &qib, sizeof g.qinv);
+  g.m0 = (int)(t * g.qinv);  // exact: t is a multiple of q, |m0| < 2^28
+  const int a = 1 << nf, b = 2 << nf;
+  g.lo = (int32_t)th < 0 ? -b : a;
+  g.hi = (int32_t)th < 0 ? -a : b;
+  return true;
+}
+
+// m * 2^qe, exact for |m| < 2^53
+PGA_HD double chain_grid_value(const ChainGrid& g, int m) {
+  const uint64_t qb = (uint64_t)(1023 + g.qe) << 52;
+  double q;
+  __builtin_memcpy(&q, &qb, sizeof q);
+  return (double)m * q;
+}
+
+// A term's step on the grid: d = floor(p/q) (clamped to +-2^29, which any step leaving
+// the binade exceeds by far), plus one where the fraction of p/q exceeds 1/2; `tie`
+// where it is exactly 1/2 (the step is then d or d + 1, whichever makes m even).
+// p/q, its floor and fraction are exact (q is a power of two).
+PGA_HD void chain_grid_term(double p, double qinv, int& d, bool& tie) {
+  const double y = p * qinv, fl = __builtin_floor(y), f = y - fl;
+  const double c = fl < -536870912.0 ? -536870912.0 : (fl > 536870912.0 ? 536870912.0 : fl);
+  d = (int)c + (f > 0.5 ? 1 : 0);
+  tie = f == 0.5;
+}
+
 // ---------------------------------------------------------------------------------
 // Decoded-domain representation used by the GEMM inner loop.
 // ---------------------------------------------------------------------------------

@@ -1,4 +1,5 @@
-// Host build of pga::fp64_posit_add (posit32_core.hpp) and of the GEMM's FMA product and
+// Host build of pga::fp64_posit_add and of the solve chain's grid steps (pga::chain_grid_*,
+// posit32_core.hpp) and of the GEMM's FMA product and
 // sum roundings (with pga::fp64_grid_chi, the sign-free C = 1.5 * 2^E of its LDS table),
 // checked against the oracle's pattern-level posit addition and multiplication: wherever
 // they do not ask for a redo, their results must be the posit sum and product.  Pairs come from several families: values of uniform[-1,1) and of wider
@@ -72,6 +73,59 @@ static void check_mul(uint32_t a, uint32_t b) {
   if (y != pga::to_double(want)) {
     if (bad < 8) printf("mul mismatch %08x * %08x: got %.17g want %08x\n", a, b, y, want);
     ++bad;
+  }
+}
+
+// The solve chain's grid steps (pga::chain_grid_setup / chain_grid_term /
+// chain_grid_value), applied the way the device applies them to a block of up to 64
+// terms: steps d summed in term order, ties settled in order (each rounds to the even
+// value), every running value checked strictly inside the binade.  Each running value
+// before the first one outside must be the posit sum of the oracle; returns the number
+// of terms consumed (that first term is formed with the oracle's addition), and the sum.
+static long grid_terms = 0, grid_exits = 0, grid_ties = 0;
+static int check_grid_block(uint32_t& t, const uint32_t* p, int cnt) {
+  pga::ChainGrid g;
+  if (!pga::chain_grid_setup(pga::to_double(t), g)) {
+    t = orc_p32_add(t, p[0]);
+    return 1;
+  }
+  if (pga::chain_grid_value(g, g.m0) != pga::to_double(t)) {
+    if (bad < 8) printf("grid value of %08x differs\n", t);
+    ++bad;
+  }
+  int D = 0, cs = 0, used = cnt;
+  uint32_t want = t;
+  for (int u = 0; u < cnt; ++u) {
+    int d;
+    bool tie;
+    pga::chain_grid_term(pga::to_double(p[u]), g.qinv, d, tie);
+    D += d;
+    if (tie) {  // the device settles ties in lane order after the prefix sum
+      const int up = (g.m0 + D + cs) & 1;
+      cs += up;
+      ++grid_ties;
+    }
+    const long m = (long)g.m0 + D + cs;
+    want = orc_p32_add(want, p[u]);
+    ++grid_terms;
+    if (!(m > g.lo && m < g.hi)) {
+      ++grid_exits;
+      used = u + 1;
+      break;
+    }
+    if (pga::chain_grid_value(g, (int)m) != pga::to_double(want)) {
+      if (bad < 8) printf("grid step mismatch t=%08x term %d (%08x): m %ld want %08x\n", t, u, p[u], m, want);
+      ++bad;
+    }
+  }
+  t = want;
+  return used;
+}
+
+static void check_grid_chain(uint32_t t, const uint32_t* p, int n) {
+  for (int k = 0; k < n;) {
+    if (t == pga::P_NAR) return;
+    k += check_grid_block(t, p + k, n - k < 64 ? n - k : 64);
   }
 }
 
@@ -169,7 +223,41 @@ int main(int argc, char** argv) {
       t = orc_p32_add(t, p);
     }
   }
-  printf("checked %ld redone %ld mismatches %ld products %ld redone %ld gemm-sums %ld redone %ld\n", checked,
-         redone, bad, mul_checked, mul_redone, gsum_checked, gsum_redone);
+  // grid chains: the solve's (products of uniform values), chains that start on or next
+  // to a power of two and step across it either way, exact midpoints (ties), sign
+  // changes, terms far below or far above the sum, arbitrary patterns, the range's ends
+  {
+    static uint32_t p[4096];
+    for (int c = 0; c < 100; ++c) {
+      for (int k = 0; k < 4096; ++k)
+        p[k] = 0u - orc_p32_mul(pga::from_double((2 * unit(s) - 1) * 0.02), pga::from_double(2 * unit(s) - 1));
+      check_grid_chain(pga::from_double(2 * unit(s) - 1), p, 4096);
+    }
+    for (long i = 0; i < n / 8; ++i) {
+      const int e = (int)(splitmix(s) % 200) - 100;
+      const double q = ldexp(1.0, e - 27 + ((e >> 2) ^ (e >> 31)));  // posit quantum at scale e
+      const double sg = (splitmix(s) & 1) ? 1.0 : -1.0;
+      const int off = (int)(splitmix(s) % 9) - 4;
+      const uint32_t t = pga::from_double(sg * (ldexp(1.0, e + (int)(splitmix(s) & 1)) + off * q));
+      const int fam = (int)(splitmix(s) % 6);
+      for (int u = 0; u < 64; ++u) {
+        if (fam == 0)  // a few (half-)quanta either way
+          p[u] = pga::from_double(((int)(splitmix(s) % 17) - 8) * q * 0.5 * ((splitmix(s) & 3) ? 1 : 0.25));
+        else if (fam == 1)  // any size at nearby scales
+          p[u] = pga::from_double(ldexp(2 * unit(s) - 1, e - (int)(splitmix(s) % 30)));
+        else if (fam == 2)  // odd multiples of half a quantum: ties
+          p[u] = pga::from_double(q * 0.5 * (double)(2 * (splitmix(s) % 5) + 1) * ((splitmix(s) & 1) ? 1 : -1));
+        else if (fam == 3)  // across zero, or over the binade
+          p[u] = pga::from_double(-sg * ldexp(unit(s), e + (int)(splitmix(s) % 3) - 1));
+        else if (fam == 4)
+          p[u] = (uint32_t)splitmix(s);
+        else  // far beyond the sum, now and then
+          p[u] = pga::from_double(ldexp(2 * unit(s) - 1, e + ((splitmix(s) % 16) ? -3 : 40)));
+      }
+      check_grid_chain(fam == 4 ? (uint32_t)splitmix(s) : t, p, 64);
+    }
+  }
+  printf("checked %ld redone %ld mismatches %ld products %ld redone %ld gemm-sums %ld redone %ld grid-terms %ld exits %ld ties %ld\n",
+         checked, redone, bad, mul_checked, mul_redone, gsum_checked, gsum_redone, grid_terms, grid_exits, grid_ties);
   return bad ? 1 : 0;
 }

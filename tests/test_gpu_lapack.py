@@ -109,6 +109,38 @@ def test_trsm_left_lower_trans_chain(pga, oracle, diag):
         _same(got, want)
 
 
+@pytest.mark.parametrize("kind", ["short", "wide", "tiny", "cancel"])
+@pytest.mark.parametrize("diag", ["N", "U"])
+def test_trsm_left_lower_trans_chain_classes(pga, oracle, kind, diag):
+    """The chain's integer grid steps on data that stresses them: short fractions (terms
+    that tie between two posits of the running sum's binade, and exact sums), magnitudes
+    spread over 2^-40 .. 2^40 (sums that leave their binade often), values near the ends
+    of the posit range (scales outside -108 .. 107, summed exactly), and sums that cancel
+    to zero."""
+    m, n = 700, 3
+    rng = np.random.default_rng({"short": 1, "wide": 2, "tiny": 3, "cancel": 4}[kind])
+    if kind == "short":
+        Ad = rng.integers(-8, 9, (m, m)) / 8.0
+        Bd = rng.integers(-16, 17, (m, n)).astype(float)
+    elif kind == "wide":
+        Ad = rng.uniform(-1, 1, (m, m)) * 2.0 ** rng.integers(-40, 41, (m, m))
+        Bd = rng.uniform(-1, 1, (m, n)) * 2.0 ** rng.integers(-20, 21, (m, n))
+    elif kind == "tiny":
+        Ad = rng.uniform(-1, 1, (m, m)) * 2.0 ** -100
+        Bd = rng.uniform(-1, 1, (m, n)) * 2.0 ** rng.choice([-20, -8, 0], (m, n))
+    else:
+        Ad = np.ones((m, m))
+        Bd = np.where(np.arange(m)[:, None] % 2 == 0, 1.0, -1.0) * np.ones((m, n))
+    Ad[np.diag_indices(m)] = 1.0 + np.arange(m) % 3
+    A = np.asfortranarray(oracle.from_double_v(Ad.ravel()).reshape(m, m))
+    B = np.asfortranarray(oracle.from_double_v(Bd.ravel()).reshape(m, n))
+    want = B.copy(order="F")
+    assert oracle.rtrsm("L", "L", "T", diag, m, n, ONE, A, want) == 0
+    got = B.copy(order="F")
+    pga.Rtrsm("L", "L", "T", diag, m, n, ONE, A, m, got, m)
+    _same(got, want)
+
+
 @pytest.mark.parametrize("uplo", ["U", "L"])
 @pytest.mark.parametrize("diag", ["N", "U"])
 def test_trsm_left_wide(pga, oracle, uplo, diag):

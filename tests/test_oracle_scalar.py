@@ -171,12 +171,15 @@ def test_core_div_sqrt_host_build(tmp_path):
 
 
 def test_fp64_posit_add_host_build(tmp_path):
-    """pga::fp64_posit_add (posit32_core.hpp: the posit sum rounded in FP64, for the
-    serial chains of the triangular solves) and the GEMM's FMA product rounding, built for
-    the host: wherever they do not ask for a redo, their results equal the oracle's posit
-    addition and multiplication, over uniform values, arbitrary patterns, short fractions,
-    cancellations, tiny terms, constructed midpoints (products whose FP64 rounding lands on
-    a posit midpoint included) and long running sums."""
+    """pga::fp64_posit_add (posit32_core.hpp: the posit sum rounded in FP64), the solve
+    chain's grid steps (pga::chain_grid_*: running sums as integer steps on one binade's
+    posit grid, ties settled in order, a block ended by the first sum outside the binade)
+    and the GEMM's FMA product rounding, built for the host: wherever they do not ask for
+    a redo, their results equal the oracle's posit additions and multiplication, over
+    uniform values, arbitrary patterns, short fractions, cancellations, tiny terms,
+    constructed midpoints (products whose FP64 rounding lands on a posit midpoint
+    included), long running sums, and chains that start on a power of two, cross it,
+    change sign, tie or jump far."""
     import subprocess
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     exe = str(tmp_path / "fp64_add_check")
