@@ -206,6 +206,60 @@ __global__ __launch_bounds__(WG) void k_trsm_lltn(int m, int n, uint32_t alpha, 
   }
 }
 
+// The same solve with the products of the next row formed while wave 0 runs a row's
+// chain, for rows of up to LLTN_PIPE_CAP terms (m <= LLTN_PIPE_CAP + 1).  In the step
+// for row i, wave 0 adds row i's terms: the first (k = i+1, whose b_{i+1} the step
+// before made final) formed by itself, the rest (k = i+2 .. m-1) from the LDS buffer
+// the other waves filled in the step before; meanwhile the other three waves form row
+// i-1's terms k = i+1 .. m-1 into the other buffer (b_k final for k > i).  One barrier
+// a row hands the buffers over.
+constexpr int LLTN_PIPE_CAP = 8192;
+__global__ __launch_bounds__(WG) void k_trsm_lltn_pipe(int m, int n, uint32_t alpha, int nounit,
+                                                       const uint32_t* __restrict__ A, int lda,
+                                                       uint32_t* __restrict__ B, int ldb, const int* stop) {
+  PGA_KERNEL_VGPR_FLOOR();
+  __shared__ double sp[2][LLTN_PIPE_CAP];
+  __shared__ int row_nar[2];  // a NaR among the buffer's terms
+  const int j = blockIdx.x;
+  if (j >= n || stopped(stop)) return;
+  uint32_t* b = B + (size_t)j * ldb;
+  const int tid = threadIdx.x;
+  if (tid < 2) row_nar[tid] = 0;
+  __syncthreads();
+  for (int i = m - 1; i >= 0; --i) {
+    if (tid < 64) {
+      const uint32_t t0 = mul(alpha, b[i]);  // every lane of wave 0 reads the same values
+      bool nar = t0 == P_NAR || row_nar[i & 1] != 0;
+      double t = nar ? 0.0 : to_double(t0);
+      if (i + 1 < m) {
+        const uint32_t v = neg(mul(A[i + 1 + (size_t)i * lda], b[i + 1]));
+        nar |= v == P_NAR;
+        if (!nar) t = chain_add_one(t, to_double(v));
+      }
+      if (!nar && m - i - 2 > 0) chain_terms(t, sp[i & 1], m - i - 2);
+      if (tid == 0) {
+        uint32_t tp = nar ? P_NAR : from_double(t);
+        if (nounit) tp = div(tp, A[i + (size_t)i * lda]);
+        b[i] = tp;
+        row_nar[i & 1] = 0;  // the buffer's next row (i-2) is formed after the barrier
+      }
+    } else if (i >= 1) {
+      const int r = i - 1, cnt = m - r - 2;  // row r's terms k = r+2 .. m-1
+      double* out = sp[r & 1];
+      const uint32_t* ar = A + (size_t)r * lda + r + 2;
+      const uint32_t* bk = b + r + 2;
+      bool term_nar = false;
+      for (int u = tid - 64; u < cnt; u += WG - 64) {
+        const uint32_t v = neg(mul(ar[u], bk[u]));
+        term_nar |= v == P_NAR;
+        out[u] = v == P_NAR ? 0.0 : to_double(v);
+      }
+      if (term_nar) row_nar[r & 1] = 1;
+    }
+    __syncthreads();  // b_i and row i-1's terms are final
+  }
+}
+
 // TRSM, left side, not transposed (Rtrsm.cpp:115-155), `cpw` columns of B per
 // workgroup: the same column sweep with the (row, column) pairs of a step spread over
 // the lanes, so wide right-hand sides take fewer workgroup-steps.
@@ -891,6 +945,9 @@ static hipError_t trsm(const Ctx& c, bool left, bool upper, bool trans, bool nou
     if (upper && !trans) PGA_TL(true, false);
     else if (!upper && !trans) PGA_TL(false, false);
     else if (upper && trans) PGA_TL(true, true);
+    else if (m <= LLTN_PIPE_CAP + 1)
+      hipLaunchKernelGGL(k_trsm_lltn_pipe, dim3(n), dim3(WG), 0, c.s, m, n, alpha, (int)nounit, A, lda, B, ldb,
+                         stop);
     else
       hipLaunchKernelGGL(k_trsm_lltn, dim3(n), dim3(WG), 0, c.s, m, n, alpha, (int)nounit, A, lda, B, ldb, stop);
 #undef PGA_TL

@@ -109,6 +109,19 @@ def test_trsm_left_lower_trans_chain(pga, oracle, diag):
         _same(got, want)
 
 
+def test_trsm_left_lower_trans_chain_long(pga, oracle):
+    """Rows longer than the pipelined kernel's buffer (8192 terms): the solve that forms
+    each row's terms in chunks of 4096 before wave 0 adds them."""
+    m, n = 8300, 1
+    A = _tri(oracle, 76, m, False)
+    B = posit_matrix(oracle, 77, m, n)
+    want = B.copy(order="F")
+    assert oracle.rtrsm("L", "L", "T", "N", m, n, ONE, A, want) == 0
+    got = B.copy(order="F")
+    pga.Rtrsm("L", "L", "T", "N", m, n, ONE, A, m, got, m)
+    _same(got, want)
+
+
 @pytest.mark.parametrize("kind", ["short", "wide", "tiny", "cancel"])
 @pytest.mark.parametrize("diag", ["N", "U"])
 def test_trsm_left_lower_trans_chain_classes(pga, oracle, kind, diag):
