@@ -14,6 +14,23 @@ namespace gemm {
 // ---------------------------------------------------------------------------------
 // Tiled kernel
 // ---------------------------------------------------------------------------------
+// Workgroup -> tile of C.  The hardware hands consecutive workgroups to the 8 XCDs in
+// turn, and each XCD has its own L2.  The workgroups of one XCD (bid % 8) are given one
+// contiguous part of an order that walks bands of GROUP_M tile rows column by column, so
+// the tiles that run at once on an XCD share a few slices of A (and their slices of B)
+// in its L2; in plain column-major order an XCD would get every eighth tile row and
+// reuse no slice of A.  Workgroups past the last multiple of 8 keep their place.
+__device__ __forceinline__ void xcd_tile(int bid, int tiles_m, int tiles_n, int& ti, int& tj) {
+  constexpr int XCDS = 8, GROUP_M = 8;
+  const int total = tiles_m * tiles_n, per = total / XCDS;
+  const int o = bid < per * XCDS ? (bid % XCDS) * per + bid / XCDS : bid;
+  const int band = GROUP_M * tiles_n;
+  const int g = o / band, r = o % band;
+  const int gm = min(GROUP_M, tiles_m - g * GROUP_M);  // tile rows in this band
+  tj = r / gm;
+  ti = g * GROUP_M + r % gm;
+}
+
 template <int BM, int BN, int BK, int TM, int TN, bool TA, bool TB, bool SKIP0>
 __global__ __launch_bounds__((BM / TM) * (BN / TN), 3) void rgemm_tiled(
     int m, int n, int k, uint32_t alpha, const uint32_t* __restrict__ A, int lda,
@@ -21,10 +38,9 @@ __global__ __launch_bounds__((BM / TM) * (BN / TN), 3) void rgemm_tiled(
     int tiles_m, int tri, int urgent, const int* __restrict__ stop) {
   PGA_KERNEL_VGPR_FLOOR();
   if (urgent) PGA_URGENT();
-  // tiles in column-major order of the tile grid: consecutive workgroups share B slices
-  const int bid = blockIdx.x;
-  const int i0 = (bid % tiles_m) * BM;
-  const int j0 = (bid / tiles_m) * BN;
+  int ti, tj;
+  xcd_tile(blockIdx.x, tiles_m, (n + BN - 1) / BN, ti, tj);
+  const int i0 = ti * BM, j0 = tj * BN;
   if (stop && *stop) return;  // an earlier step of a factorization failed
   // triangular update (Rsyrk): skip tiles with no element in the stored triangle
   if (tri == 1 && i0 + BM - 1 < j0) return;
@@ -72,9 +88,9 @@ __global__ __launch_bounds__(256, 3) void rgemm_tiled_panel(int m, int n, int k,
                                                          int urgent, const int* __restrict__ stop, Panels pn) {
   PGA_KERNEL_VGPR_FLOOR();
   if (urgent) PGA_URGENT();
-  const int bid = blockIdx.x;
-  const int i0 = (bid % tiles_m) * 64;
-  const int j0 = (bid / tiles_m) * 32;
+  int ti, tj;
+  xcd_tile(blockIdx.x, tiles_m, (n + 31) / 32, ti, tj);
+  const int i0 = ti * 64, j0 = tj * 32;
   if (stop && *stop) return;
   if (tri == 1 && i0 + 63 < j0) return;
   if (tri == 2 && j0 + 31 < i0) return;
