@@ -7,9 +7,10 @@ alpha = -1, beta = 1; A, B, C uniform[-1,1) in FP64 converted to posit32 (seeded
 on the device), all resident in HBM before timing.  value = sum(2*M*N*K) / time.
 
 Besides the JSON fields of the run contract the line carries:
-  roofline      integer-VALU roofline of the dominant kernel (the M = N = 16384 GEMM):
-                achieved = VALU lane-ops per launch / its average duration (HIP events on
-                the library stream), where VALU lane-ops per launch = 64 * SQ_INSTS_VALU
+  roofline      integer-VALU roofline of the dominant kernel (the M = N = 16384 GEMM: its
+                two FP64 operand-panel kernels and the 64 x 32 tile kernel):
+                achieved = VALU lane-ops per call / its average duration (HIP events on
+                the library stream), where VALU lane-ops per call = 64 * SQ_INSTS_VALU
                 measured with rocprofv3 (profiles/rgemm_valu_counts.json, scaled by MNK);
                 peak = CUs * 4 SIMD * 16 lanes * clock; traffic = HBM bytes per launch from
                 FETCH_SIZE + WRITE_SIZE in the same file (scaled by a streaming kernel's
@@ -218,7 +219,7 @@ def main():
     macs = float(n0) * n0 * K
     peak = info["int32_valu_peak"]
     roof = {"bound": "valu", "achieved": None, "peak": round(peak / 1e12, 3), "unit": "Tlane-op/s",
-            "frac": None, "traffic": None, "kernel": "rgemm_tiled NN 64x32x16",
+            "frac": None, "traffic": None, "kernel": "rgemm_tiled_panel NN 64x32 + k_panel_a + k_panel_t",
             "avg_ms": round(per_shape_ms[0], 4), "gops": round(2 * macs / t_dom / 1e9, 2)}
     if os.path.exists(COUNTS):
         # rocprofv3 counts of one launch of this kernel at M = N = 16384, K = 256:
