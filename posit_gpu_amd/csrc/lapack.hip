@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <mutex>
 #include <vector>
 
@@ -584,6 +585,21 @@ __global__ __launch_bounds__(LEAF_NT) void k_getrf2_pair(int m_root, int w_root,
                                                          ts, tm, te);
 }
 
+// The same node on G workgroups (lu_leaf.hpp: getrf2_pair_mc), RPL rows per lane.
+template <int RPL>
+__global__ __launch_bounds__(256) void k_getrf2_pair_mc(int m_root, int w_root, uint32_t* root, int lda, int j,
+                                                        int* ipiv, int* info, int row_off, int col_off, int ts,
+                                                        int tm, int te, leaf::PairSync* sy, unsigned base) {
+  PGA_KERNEL_VGPR_FLOOR();
+  PGA_URGENT();
+  leaf::getrf2_pair_mc<RPL, LEAF_TRSM_N1, LEAF_TRSM_N2>(m_root, w_root, root, lda, j, ipiv, info, row_off, col_off,
+                                                         ts, tm, te, sy, base);
+}
+
+__global__ void k_zero_words(unsigned* p, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0u;
+}
+
 // ---------------------------------------------------------------------------------
 // Rpotrf2 leaf (Rpotrf2.cpp:86-100): not positive (or NaR) -> info, else sqrt
 // ---------------------------------------------------------------------------------
@@ -784,9 +800,17 @@ static unsigned blocks_for(long n) { return (unsigned)((n + WG - 1) / WG); }
     if (_e != hipSuccess) return _e;       \
   } while (0)
 
+// The device state of the multi-workgroup panel pairs of one factorization, and the
+// grid barrier's ticket count so far (the host adds each launch's arrivals).
+struct PairState {
+  leaf::PairSync* dev = nullptr;
+  unsigned base = 0;
+};
+
 struct Ctx {
   hipStream_t s;
   GemmImpl impl;
+  PairState* pair = nullptr;  // set: two-column nodes of at least pair_mc_min() rows use G workgroups
 };
 
 static hipError_t trsm(const Ctx& c, bool left, bool upper, bool trans, bool nounit, int m, int n,
@@ -993,6 +1017,17 @@ static hipError_t laswp(const Ctx& c, int n, uint32_t* a, int lda, int k1, int k
     if (_e != hipSuccess) return _e;       \
   } while (0)
 
+// Rows from which a two-column node runs on several workgroups (PGA_LU_PAIR_MC_MIN;
+// 0 turns that off)
+static int pair_mc_min() {
+  static const int v = [] {
+    const char* e = getenv("PGA_LU_PAIR_MC_MIN");
+    const int x = e ? atoi(e) : 2048;
+    return x > 0 ? std::max(x, 1024) : INT32_MAX;
+  }();
+  return v;
+}
+
 // Rgetrf2 (Rgetrf2.cpp:32-185) on the m x n block at `a` (rows row_off.., columns
 // col_off.. of the matrix that ipiv / info describe; ipiv points at the block's first
 // entry).  The recursion's nodes cover columns [s, e) and rows [s, m) of the block; its
@@ -1019,6 +1054,28 @@ static hipError_t getrf2_node(const Ctx& c, int m, int n, uint32_t* a, int lda, 
     return hipGetLastError();
   }
   if (cols == 2) {  // rows >= 2 here: the node splits 1 + 1
+    if (c.pair && rows >= pair_mc_min()) {
+      // G blocks of at least 512 rows, at most 8 rows per lane
+      const int G = std::min(leaf::PAIR_MC_MAX_G, rows / 512), R = (rows + G - 1) / G, rpl = (R + 255) / 256;
+      const unsigned base = c.pair->base;
+      const dim3 g((unsigned)G), b(256);
+      bool launched = true;
+      if (rpl <= 2)
+        hipLaunchKernelGGL(k_getrf2_pair_mc<2>, g, b, 0, c.s, m, n, a, lda, s, ipiv, info, row_off, col_off, ps, pm,
+                           pe, c.pair->dev, base);
+      else if (rpl <= 4)
+        hipLaunchKernelGGL(k_getrf2_pair_mc<4>, g, b, 0, c.s, m, n, a, lda, s, ipiv, info, row_off, col_off, ps, pm,
+                           pe, c.pair->dev, base);
+      else if (rpl <= 8)
+        hipLaunchKernelGGL(k_getrf2_pair_mc<8>, g, b, 0, c.s, m, n, a, lda, s, ipiv, info, row_off, col_off, ps, pm,
+                           pe, c.pair->dev, base);
+      else
+        launched = false;
+      if (launched) {
+        c.pair->base = base + (unsigned)G * (pm > ps ? 3u : 2u);
+        return hipGetLastError();
+      }
+    }
     hipLaunchKernelGGL(k_getrf2_pair, dim3(1), dim3(LEAF_NT), 0, c.s, m, n, a, lda, s, ipiv, info, row_off, col_off,
                        ps, pm, pe);
     return hipGetLastError();
@@ -1105,7 +1162,7 @@ constexpr int GETRF_LOOKAHEAD_MIN = 1024;
 static hipError_t getrf_lookahead_on(const Ctx& c, hipStream_t wide, hipEvent_t join, hipEvent_t done, int m, int n,
                                      uint32_t* a, int lda, int* ipiv, int* info, int nb) {
   const int mn = m < n ? m : n;
-  const Ctx cx{wide, c.impl};
+  const Ctx cx{wide, c.impl, nullptr};
   hipError_t e = getrf2(c, m, std::min(nb, mn), a, lda, ipiv, info, 0, 0);
   bool recorded = false;  // the wide stream has recorded `done` at least once
   for (int j = 0; j < mn && e == hipSuccess; j += nb) {
@@ -1138,7 +1195,7 @@ static hipError_t getrf_lookahead(const Ctx& c, int m, int n, uint32_t* a, int l
 
 // Rgetrf_opencl (Rgetrf_opencl.cpp:32-154): blocked right-looking LU, nb columns per
 // panel; ipiv holds global 1-based rows, info the first zero pivot column.
-static hipError_t getrf(const Ctx& c, int m, int n, uint32_t* a, int lda, int* ipiv, int* info, int nb) {
+static hipError_t getrf_blocked(const Ctx& c, int m, int n, uint32_t* a, int lda, int* ipiv, int* info, int nb) {
   const int mn = m < n ? m : n;
   if (nb <= 1 || nb >= mn) return getrf2(c, m, n, a, lda, ipiv, info, 0, 0);
   if (mn >= GETRF_LOOKAHEAD_MIN) return getrf_lookahead(c, m, n, a, lda, ipiv, info, nb);
@@ -1158,6 +1215,29 @@ static hipError_t getrf(const Ctx& c, int m, int n, uint32_t* a, int lda, int* i
     }
   }
   return hipSuccess;
+}
+
+// The blocked LU with the device state of its multi-workgroup panel pairs, taken from the
+// library's memory pool on the caller's stream for the call (without a pool, or for
+// short panels, every pair runs on one workgroup).
+static hipError_t getrf(const Ctx& c, int m, int n, uint32_t* a, int lda, int* ipiv, int* info, int nb) {
+  hipMemPool_t pool = runtime().pool;
+  if (!pool || m < pair_mc_min()) return getrf_blocked(c, m, n, a, lda, ipiv, info, nb);
+  void* p = nullptr;
+  if (hipMallocFromPoolAsync(&p, sizeof(leaf::PairSync), pool, c.s) != hipSuccess) {
+    (void)hipGetLastError();
+    return getrf_blocked(c, m, n, a, lda, ipiv, info, nb);
+  }
+  hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(256), 0, c.s, (unsigned*)p,
+                     (int)(sizeof(leaf::PairSync) / sizeof(unsigned)));
+  PairState st;
+  st.dev = (leaf::PairSync*)p;
+  Ctx cp = c;
+  cp.pair = &st;
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = getrf_blocked(cp, m, n, a, lda, ipiv, info, nb);
+  const hipError_t f = hipFreeAsync(p, c.s);
+  return e != hipSuccess ? e : f;
 }
 
 // Rgetrs (Rgetrs.cpp:32-118)

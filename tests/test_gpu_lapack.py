@@ -417,6 +417,40 @@ def test_lu_lookahead_device(pga, oracle, m, n, nb, kind):
     _same(dA.to_host(), want)
 
 
+@pytest.mark.parametrize("m,n,nb,kind", [(2600, 520, 128, None), (4100, 300, 256, "zero_column"),
+                                         (3000, 200, 64, "nar"), (2100, 130, 64, "ties"),
+                                         (2200, 90, 32, "nar_pivot"), (9000, 260, 256, None)])
+def test_lu_pair_multi_workgroup(pga, oracle, m, n, nb, kind):
+    """Two-column nodes with at least 2048 rows run on several workgroups that meet at a
+    grid barrier per pivot search (2 to 16 blocks of rows): pivots found in any block,
+    ties between blocks (the first row wins), zero columns (no interchange of the pivot
+    column, info), NaR entries, a NaR on the diagonal over zeros (pivot below it with value
+    zero), and pending node solves after a pair."""
+    A = posit_matrix(oracle, 91 + m + n, m, n)
+    if kind == "zero_column":
+        A[:, 130] = 0
+    elif kind == "nar":
+        A[5, 3] = NAR
+        A[2900, 100] = NAR
+    elif kind == "ties":
+        A[:, 40] = oracle.from_double(0.5)  # equal magnitudes in every block
+        A[1500:, 41] = oracle.from_double(-0.75)
+    elif kind == "nar_pivot":
+        A[:, 20] = 0
+        A[20, 20] = NAR
+    mn = min(m, n)
+    dA = pga.DeviceArray.from_host(A)
+    dp = pga.DeviceArray(mn, np.int32, "C")
+    di = pga.DeviceArray(1, np.int32, "C")
+    pga.getrf_device(m, n, dA, m, dp, di, nb)
+    pga.synchronize()
+    want = A.copy(order="F")
+    _, wpiv, winfo = oracle.rgetrf(want, nb=nb, nthreads=THREADS)
+    assert int(di.to_host()[0]) == winfo
+    _same(dp.to_host(), wpiv)
+    _same(dA.to_host(), want)
+
+
 def test_cholesky_1024_device(pga, oracle):
     n = 1024
     A = _spd(oracle, 61, n)
