@@ -597,6 +597,7 @@ __global__ __launch_bounds__(256) void k_getrf2_pair_mc(int m_root, int w_root, 
 }
 
 __global__ void k_zero_words(unsigned* p, int n) {
+  PGA_KERNEL_VGPR_FLOOR();
   for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0u;
 }
 
