@@ -18,6 +18,7 @@
 #include "rgemm_kernels.hpp"
 #include "rgemm_tile.hpp"
 #include "lu_leaf.hpp"
+#include "lu_panel.hpp"
 #include "wg_factor.hpp"
 
 namespace pga {
@@ -596,6 +597,17 @@ __global__ __launch_bounds__(256) void k_getrf2_pair_mc(int m_root, int w_root, 
                                                          ts, tm, te, sy, base);
 }
 
+// A W-column node of the recursion on G workgroups (lu_panel.hpp: panel_mc), RPL rows per
+// lane.  At most 256 VGPRs, so that a workgroup fits beside the trailing update's waves.
+template <int W, int RPL>
+__global__ __launch_bounds__(256, 2) void k_getrf2_panel_mc(int m_root, int w_root, uint32_t* root, int lda, int s,
+                                                            int* ipiv, int* info, int row_off, int col_off,
+                                                            leaf::PairSync* sy, int launch) {
+  PGA_KERNEL_VGPR_FLOOR();
+  PGA_URGENT();
+  leaf::panel_mc<W, RPL, 4>(m_root, w_root, root, lda, s, ipiv, info, row_off, col_off, sy, launch);
+}
+
 __global__ void k_zero_words(unsigned* p, int n) {
   PGA_KERNEL_VGPR_FLOOR();
   for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0u;
@@ -806,6 +818,7 @@ static unsigned blocks_for(long n) { return (unsigned)((n + WG - 1) / WG); }
 struct PairState {
   leaf::PairSync* dev = nullptr;
   unsigned base = 0;
+  int panel_launches = 0;  // W-column node launches so far (each has its own ticket counter)
 };
 
 struct Ctx {
@@ -1029,6 +1042,62 @@ static int pair_mc_min() {
   return v;
 }
 
+// Width of the recursion nodes that run as one multi-workgroup launch (PGA_LU_PANEL_W:
+// 32 or 64; 0 turns that off), and the rows per workgroup it aims for (PGA_LU_PANEL_ROWS)
+static int panel_w() {
+  static const int v = [] {
+    const char* e = getenv("PGA_LU_PANEL_W");
+    const int x = e ? atoi(e) : 32;
+    return (x == 32 || x == 64) ? x : 0;
+  }();
+  return v;
+}
+static int panel_rows() {
+  static const int v = [] {
+    const char* e = getenv("PGA_LU_PANEL_ROWS");
+    const int x = e ? atoi(e) : 256;
+    return std::max(256, std::min(x, 4096));
+  }();
+  return v;
+}
+
+// Workgroups per CU of the trailing update that runs beside the panel nodes
+// (PGA_LU_GEMM_CAP; 0: no cap): three operand-panel GEMM workgroups leave a CU the
+// registers and LDS of one panel-node workgroup.
+static int panel_gemm_cap() {
+  static const int v = [] {
+    const char* e = getenv("PGA_LU_GEMM_CAP");
+    return e ? std::max(0, atoi(e)) : 3;
+  }();
+  return v;
+}
+
+// Launches the W-column node [s, s + W) of the block on G workgroups; false when the node
+// does not fit the kernel's shapes (the caller then recurses as usual).
+static bool panel_mc_launch(const Ctx& c, int m, int n, uint32_t* a, int lda, int s, int* ipiv, int* info,
+                            int row_off, int col_off, int W) {
+  if (c.pair->panel_launches >= leaf::PANEL_MC_MAX_LAUNCHES) return false;
+  const int rows = m - s;
+  const int G = std::max(1, std::min(leaf::PANEL_MC_MAX_G, (rows + panel_rows() - 1) / panel_rows()));
+  const int R = (rows + G - 1) / G, rpl = (R + 255) / 256;
+  const int launch = c.pair->panel_launches;
+  const dim3 g((unsigned)G), b(256);
+  leaf::PairSync* sy = c.pair->dev;
+  if (W == 32 && rpl <= 1)
+    hipLaunchKernelGGL((k_getrf2_panel_mc<32, 1>), g, b, 0, c.s, m, n, a, lda, s, ipiv, info, row_off, col_off, sy,
+                       launch);
+  else if (W == 32 && rpl <= 2)
+    hipLaunchKernelGGL((k_getrf2_panel_mc<32, 2>), g, b, 0, c.s, m, n, a, lda, s, ipiv, info, row_off, col_off, sy,
+                       launch);
+  else if (W == 64 && rpl <= 1)
+    hipLaunchKernelGGL((k_getrf2_panel_mc<64, 1>), g, b, 0, c.s, m, n, a, lda, s, ipiv, info, row_off, col_off, sy,
+                       launch);
+  else
+    return false;
+  c.pair->panel_launches = launch + 1;
+  return true;
+}
+
 // Rgetrf2 (Rgetrf2.cpp:32-185) on the m x n block at `a` (rows row_off.., columns
 // col_off.. of the matrix that ipiv / info describe; ipiv points at the block's first
 // entry).  The recursion's nodes cover columns [s, e) and rows [s, m) of the block; its
@@ -1054,6 +1123,9 @@ static hipError_t getrf2_node(const Ctx& c, int m, int n, uint32_t* a, int lda, 
                        ps, pm, pe);
     return hipGetLastError();
   }
+  if (c.pair && pm <= ps && cols == panel_w() && rows >= cols &&
+      panel_mc_launch(c, m, n, a, lda, s, ipiv, info, row_off, col_off, cols))
+    return hipGetLastError();
   if (cols == 2) {  // rows >= 2 here: the node splits 1 + 1
     if (c.pair && rows >= pair_mc_min()) {
       // G blocks of at least 512 rows, at most 8 rows per lane
@@ -1136,7 +1208,7 @@ static void aux_give(const Aux& x) {
 // rows below.  Columns are independent here, so any split of the right-hand columns
 // gives the same bits.
 static hipError_t getrf_update(const Ctx& c, int m, uint32_t* a, int lda, const int* ipiv, int j, int jb, int c0,
-                               int w, bool urgent) {
+                               int w, bool urgent, int cap = 0) {
   if (w <= 0) return hipSuccess;
   uint32_t* ajj = a + j + (size_t)j * lda;
   uint32_t* a12 = a + j + (size_t)c0 * lda;
@@ -1144,6 +1216,7 @@ static hipError_t getrf_update(const Ctx& c, int m, uint32_t* a, int lda, const 
   PGA_TRY(trsm(c, true, false, false, false, jb, w, P_ONE, ajj, lda, a12, lda, nullptr));
   GemmOpts o;
   o.urgent = urgent;
+  o.cap_per_cu = cap;
   if (j + jb < m)
     PGA_TRY(rgemm_launch(false, false, m - j - jb, w, jb, P_MONE, ajj + jb, lda, a12, lda, P_ONE, a12 + jb, lda,
                          c.s, c.impl, o));
@@ -1164,6 +1237,8 @@ static hipError_t getrf_lookahead_on(const Ctx& c, hipStream_t wide, hipEvent_t 
                                      uint32_t* a, int lda, int* ipiv, int* info, int nb) {
   const int mn = m < n ? m : n;
   const Ctx cx{wide, c.impl, nullptr};
+  // with multi-workgroup panel nodes, the bulk update leaves room on every CU for them
+  const int wide_cap = (c.pair && panel_w()) ? panel_gemm_cap() : 0;
   hipError_t e = getrf2(c, m, std::min(nb, mn), a, lda, ipiv, info, 0, 0);
   bool recorded = false;  // the wide stream has recorded `done` at least once
   for (int j = 0; j < mn && e == hipSuccess; j += nb) {
@@ -1171,7 +1246,7 @@ static hipError_t getrf_lookahead_on(const Ctx& c, hipStream_t wide, hipEvent_t 
     if ((e = hipEventRecord(join, c.s)) != hipSuccess) break;
     if ((e = hipStreamWaitEvent(wide, join, 0)) != hipSuccess) break;
     if (j > 0 && (e = laswp(cx, j, a, lda, j + 1, j + jb, ipiv, 1, 0, nullptr)) != hipSuccess) break;
-    if ((e = getrf_update(cx, m, a, lda, ipiv, j, jb, c1, n - c1, false)) != hipSuccess) break;
+    if ((e = getrf_update(cx, m, a, lda, ipiv, j, jb, c1, n - c1, false, wide_cap)) != hipSuccess) break;
     if (recorded && (e = hipStreamWaitEvent(c.s, done, 0)) != hipSuccess) break;
     if ((e = hipEventRecord(done, wide)) != hipSuccess) break;
     recorded = true;
@@ -1223,7 +1298,7 @@ static hipError_t getrf_blocked(const Ctx& c, int m, int n, uint32_t* a, int lda
 // short panels, every pair runs on one workgroup).
 static hipError_t getrf(const Ctx& c, int m, int n, uint32_t* a, int lda, int* ipiv, int* info, int nb) {
   hipMemPool_t pool = runtime().pool;
-  if (!pool || m < pair_mc_min()) return getrf_blocked(c, m, n, a, lda, ipiv, info, nb);
+  if (!pool || (m < pair_mc_min() && panel_w() == 0)) return getrf_blocked(c, m, n, a, lda, ipiv, info, nb);
   void* p = nullptr;
   if (hipMallocFromPoolAsync(&p, sizeof(leaf::PairSync), pool, c.s) != hipSuccess) {
     (void)hipGetLastError();

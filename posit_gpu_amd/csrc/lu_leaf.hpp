@@ -156,12 +156,28 @@ struct PairRec {
 constexpr int PAIR_MC_MAX_G = 16;
 // info after a launch whose grid barrier gave up waiting (its results are not valid)
 constexpr int PAIR_SYNC_FAILED_INFO = -1000;
+// One workgroup's candidate for a pivot search of the W-column node kernel (lu_panel.hpp):
+// the magnitude of its first maximum (-1 for NaR), that row (INT32_MAX: none), and the
+// row's values in the node's W columns, as FP64 numbers.
+constexpr int PANEL_MC_MAX_G = 32;
+constexpr int PANEL_MC_MAX_W = 64;
+constexpr int PANEL_MC_MAX_LAUNCHES = 1024;  // W-column node launches per factorization
+struct PanelRec {
+  double key;
+  int idx;
+  int nan;  // an FP64 result of this workgroup's previous step was NaN
+  double v[PANEL_MC_MAX_W];
+};
 struct PairSync {
   unsigned ctr;
   unsigned fail;
   unsigned pad[14];
   PairRec cand[2][PAIR_MC_MAX_G];
   PairRec own[2];  // row j before the first search, row j+1 before the second
+  // the W-column node kernel: per step parity, the workgroups' candidates and row j
+  PanelRec pcand[2][PANEL_MC_MAX_G];
+  double pown[2][PANEL_MC_MAX_W];
+  unsigned pctr[PANEL_MC_MAX_LAUNCHES];  // one ticket counter per W-column node launch
 };
 
 __device__ __forceinline__ void rec_store(PairRec* r, const PairRec& v) {

@@ -19,6 +19,7 @@ struct GemmOpts {
   const int* stop = nullptr; // device flag: when nonzero the launch does nothing
   int tile = 0;              // 0 automatic, else 64 / 32 / 16
   bool urgent = false;       // a step on a factorization's critical path (PGA_URGENT)
+  int cap_per_cu = 0;        // at most this many workgroups per CU (0: no cap), through unused LDS
 };
 
 // Device-pointer GEMM launch (rgemm.hip); arguments already validated.
@@ -77,6 +78,7 @@ struct Runtime {
   hipStream_t stream = nullptr;
   GemmImpl gemm_impl = GemmImpl::Tiled;
   int cus = 256;
+  int lds_per_cu = 160 * 1024;  // LDS bytes of one CU (the device's maxSharedMemoryPerMultiProcessor)
   int gemm_tile = 0;  // PGA_GEMM_TILE: 0 per shape, else the tile edge of the plain GEMM
   bool gemm_panels = true;         // PGA_GEMM_PANELS=0: large GEMMs decode in every tile
   long long panel_min_work = 1LL << 30;  // m*n*k from which the 64 x 32 tile uses panels

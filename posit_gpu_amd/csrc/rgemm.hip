@@ -186,17 +186,32 @@ __global__ __launch_bounds__(256) void scale_c(int m, int n, uint32_t beta, uint
 // Tile shapes: 64x64 with 4x4 elements per lane for large C; 32x32 (2x2) and 16x16
 // (1x1) keep enough waves in flight for skinny C (e.g. Cholesky's N = 64 updates),
 // since K is never split.
+// Dynamic LDS that keeps a kernel at `cap` workgroups per CU or fewer (0: none): each
+// workgroup then holds more than 1/(cap + 1) of the CU's LDS.  A factorization caps its
+// trailing update so that the waves of its concurrent panel kernel find room on a CU.
+static unsigned lds_for_cap(const void* kernel, int cap) {
+  if (cap <= 0) return 0;
+  hipFuncAttributes fa;
+  if (hipFuncGetAttributes(&fa, kernel) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  const long need = (long)runtime().lds_per_cu / (cap + 1) + 1 - (long)fa.sharedSizeBytes;
+  return need > 0 ? (unsigned)need : 0u;
+}
+
 template <int BM, int TM, bool TA, bool TB, bool SKIP0>
 static hipError_t launch_tiled_cfg(int m, int n, int k, uint32_t alpha, const uint32_t* A, int lda,
                                    const uint32_t* B, int ldb, uint32_t beta, uint32_t* C, int ldc,
-                                   hipStream_t s, int tri, int urgent, const int* stop) {
+                                   hipStream_t s, int tri, int urgent, const int* stop, int cap) {
   constexpr int BN = BM == 64 ? 32 : BM, TN = BM == 64 ? 2 : TM, BK = 16;
   constexpr int NT = (BM / TM) * (BN / TN);
   int tm = (m + BM - 1) / BM, tn = (n + BN - 1) / BN;
   long long blocks = (long long)tm * tn;
   if (blocks > 0x7FFFFFFF) return hipErrorInvalidValue;
+  const unsigned dyn = lds_for_cap((const void*)rgemm_tiled<BM, BN, BK, TM, TN, TA, TB, SKIP0>, cap);
   hipLaunchKernelGGL((rgemm_tiled<BM, BN, BK, TM, TN, TA, TB, SKIP0>), dim3((unsigned)blocks), dim3(NT),
-                     0, s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tm, tri, urgent, stop);
+                     dyn, s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tm, tri, urgent, stop);
   return hipGetLastError();
 }
 
@@ -212,7 +227,7 @@ static int pick_tile(int m, int n, int cus) {
 template <bool TA, bool TB, bool SKIP0>
 static hipError_t launch_panel(int m, int n, int k, uint32_t alpha, const uint32_t* A, int lda, const uint32_t* B,
                                int ldb, uint32_t beta, uint32_t* C, int ldc, hipStream_t s, int tri, int urgent,
-                               const int* stop, bool& done) {
+                               const int* stop, int cap, bool& done) {
   done = false;
   hipMemPool_t pool = runtime().pool;
   if (!pool) return hipSuccess;
@@ -230,8 +245,9 @@ static hipError_t launch_panel(int m, int n, int k, uint32_t alpha, const uint32
   hipLaunchKernelGGL((k_panel_a<TA>), dim3(tm * ks), dim3(256), 0, s, m, k, A, lda, const_cast<double*>(pn.a), ks);
   hipLaunchKernelGGL((k_panel_t<TB, TA, SKIP0>), dim3(tn * ks), dim3(256), 0, s, n, k, alpha, B, ldb,
                      const_cast<double*>(pn.t), ks);
-  hipLaunchKernelGGL((rgemm_tiled_panel<TA, TB, SKIP0>), dim3((unsigned)blocks), dim3(256), 0, s, m, n, k, alpha, A,
-                     lda, B, ldb, beta, C, ldc, tm, tri, urgent, stop, pn);
+  const unsigned dyn = lds_for_cap((const void*)rgemm_tiled_panel<TA, TB, SKIP0>, cap);
+  hipLaunchKernelGGL((rgemm_tiled_panel<TA, TB, SKIP0>), dim3((unsigned)blocks), dim3(256), dyn, s, m, n, k, alpha,
+                     A, lda, B, ldb, beta, C, ldc, tm, tri, urgent, stop, pn);
   hipError_t e = hipGetLastError();
   hipError_t f = hipFreeAsync(ws, s);
   return e != hipSuccess ? e : f;
@@ -240,20 +256,23 @@ static hipError_t launch_panel(int m, int n, int k, uint32_t alpha, const uint32
 template <bool TA, bool TB, bool SKIP0>
 static hipError_t launch_tiled(int m, int n, int k, uint32_t alpha, const uint32_t* A, int lda,
                                const uint32_t* B, int ldb, uint32_t beta, uint32_t* C, int ldc,
-                               hipStream_t s, int tri, int urgent, const int* stop, int tile) {
+                               hipStream_t s, int tri, int urgent, const int* stop, int tile, int cap) {
   // panels from m*n*k = 2^30 on (runtime().panel_min_work): below it the two extra launches
   // and the workspace cost more than the staging they save
   if (tile == 64 && runtime().gemm_panels && (long long)m * n * k >= runtime().panel_min_work) {
     bool done;
     hipError_t e = launch_panel<TA, TB, SKIP0>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, tri, urgent, stop,
-                                               done);
+                                               cap, done);
     if (done || e != hipSuccess) return e;
   }
   if (tile == 64)
-    return launch_tiled_cfg<64, 4, TA, TB, SKIP0>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, tri, urgent, stop);
+    return launch_tiled_cfg<64, 4, TA, TB, SKIP0>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, tri, urgent, stop,
+                                                  cap);
   if (tile == 32)
-    return launch_tiled_cfg<32, 2, TA, TB, SKIP0>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, tri, urgent, stop);
-  return launch_tiled_cfg<16, 1, TA, TB, SKIP0>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, tri, urgent, stop);
+    return launch_tiled_cfg<32, 2, TA, TB, SKIP0>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, tri, urgent, stop,
+                                                  cap);
+  return launch_tiled_cfg<16, 1, TA, TB, SKIP0>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, tri, urgent, stop,
+                                                  cap);
 }
 
 template <bool TA, bool TB, bool SKIP0>
@@ -270,17 +289,18 @@ static hipError_t launch_exact(int m, int n, int k, uint32_t alpha, const uint32
 template <bool SKIP0>
 static hipError_t dispatch(bool ta, bool tb, int m, int n, int k, uint32_t alpha, const uint32_t* A,
                            int lda, const uint32_t* B, int ldb, uint32_t beta, uint32_t* C, int ldc,
-                           hipStream_t s, GemmImpl impl, int tri, int urgent, const int* stop, int tile) {
+                           hipStream_t s, GemmImpl impl, int tri, int urgent, const int* stop, int tile,
+                           int cap) {
   if (impl == GemmImpl::Exact) {
     if (!ta && !tb) return launch_exact<false, false, SKIP0>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, tri, stop);
     if (!ta && tb) return launch_exact<false, true, SKIP0>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, tri, stop);
     if (ta && !tb) return launch_exact<true, false, SKIP0>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, tri, stop);
     return launch_exact<true, true, SKIP0>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, tri, stop);
   }
-  if (!ta && !tb) return launch_tiled<false, false, SKIP0>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, tri, urgent, stop, tile);
-  if (!ta && tb) return launch_tiled<false, true, SKIP0>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, tri, urgent, stop, tile);
-  if (ta && !tb) return launch_tiled<true, false, SKIP0>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, tri, urgent, stop, tile);
-  return launch_tiled<true, true, SKIP0>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, tri, urgent, stop, tile);
+  if (!ta && !tb) return launch_tiled<false, false, SKIP0>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, tri, urgent, stop, tile, cap);
+  if (!ta && tb) return launch_tiled<false, true, SKIP0>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, tri, urgent, stop, tile, cap);
+  if (ta && !tb) return launch_tiled<true, false, SKIP0>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, tri, urgent, stop, tile, cap);
+  return launch_tiled<true, true, SKIP0>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, tri, urgent, stop, tile, cap);
 }
 
 template <int BM, int TM, bool TA, bool TB>
@@ -337,8 +357,10 @@ hipError_t rgemm_launch(bool ta, bool tb, int m, int n, int k, uint32_t alpha, c
   }
   int tile = o.tile ? o.tile : (runtime().gemm_tile ? runtime().gemm_tile : pick_tile(m, n, runtime().cus));
   if (o.skip_zero_b)
-    return dispatch<true>(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, impl, o.tri, o.urgent, o.stop, tile);
-  return dispatch<false>(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, impl, o.tri, o.urgent, o.stop, tile);
+    return dispatch<true>(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, impl, o.tri, o.urgent, o.stop, tile,
+                          o.cap_per_cu);
+  return dispatch<false>(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, impl, o.tri, o.urgent, o.stop, tile,
+                         o.cap_per_cu);
 }
 
 }  // namespace pga

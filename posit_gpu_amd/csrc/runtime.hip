@@ -215,7 +215,10 @@ pga_status pga_init(int device) {
   g_rt.device = device;
   {
     hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, device) == hipSuccess) g_rt.cus = prop.multiProcessorCount;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
+      g_rt.cus = prop.multiProcessorCount;
+      if (prop.maxSharedMemoryPerMultiProcessor > 0) g_rt.lds_per_cu = (int)prop.maxSharedMemoryPerMultiProcessor;
+    }
   }
   // the fixed device array of the no-allocation batched GEMM, for this device
   PGA_HIP(batched_noalloc_setup(device));
