@@ -605,7 +605,7 @@ __global__ __launch_bounds__(256, 2) void k_getrf2_panel_mc(int m_root, int w_ro
                                                             leaf::PairSync* sy, int launch) {
   PGA_KERNEL_VGPR_FLOOR();
   PGA_URGENT();
-  leaf::panel_mc<W, RPL, 4>(m_root, w_root, root, lda, s, ipiv, info, row_off, col_off, sy, launch);
+  leaf::panel_mc<W, RPL, 4, 4>(m_root, w_root, root, lda, s, ipiv, info, row_off, col_off, sy, launch);
 }
 
 __global__ void k_zero_words(unsigned* p, int n) {

@@ -178,6 +178,12 @@ struct PairSync {
   PanelRec pcand[2][PANEL_MC_MAX_G];
   double pown[2][PANEL_MC_MAX_W];
   unsigned pctr[PANEL_MC_MAX_LAUNCHES];  // one ticket counter per W-column node launch
+  // the fast form's exchange: 64-bit words that each carry the tag of the step they
+  // belong to, so that a reader needs no barrier: per step parity, each workgroup's
+  // header (key, row, NaN report) and candidate row, and row j + 1
+  unsigned long long thead[2][PANEL_MC_MAX_G];
+  unsigned long long tval[2][PANEL_MC_MAX_G][PANEL_MC_MAX_W];
+  unsigned long long town[2][PANEL_MC_MAX_W];
 };
 
 __device__ __forceinline__ void rec_store(PairRec* r, const PairRec& v) {

@@ -372,6 +372,38 @@ PGA_HD uint32_t fp64_grid_chi(uint32_t hi) {
   return ((uint32_t)(eb + 25 + ak) << 20) | 0x00080000u;
 }
 
+// The constant of fp64_grid_chi computed from x's binary exponent instead of read from a
+// table: C = 1.5 * 2^(e + 25 + |k|') for scales e in -108 .. 107 (the same value),
+// 1.5 * 2^24 for zero (a zero stays zero), +infinity for every other exponent (then
+// (x + C) - C is NaN).  Costs a few VALU instructions and no LDS.
+PGA_HD double fp64_grid_c_arith(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const int e = __builtin_amdgcn_frexp_exp(x) - 1;
+#else
+  int fe = 0;
+  (void)__builtin_frexp(x, &fe);
+  const int e = fe - 1;
+#endif
+  const int ak = (e >> 2) ^ (e >> 31);
+  const bool in = (unsigned)(e + 108) <= 215u;
+  return __builtin_ldexp(1.5, in ? e + 25 + ak : 2000);
+}
+
+// round(1 / a) on the posit grid for a nonzero posit value a held in FP64: the correctly
+// rounded FP64 quotient, rounded to the grid of its binade ((q + C) - C).  Rounding twice
+// differs from rounding once only where the FP64 quotient is exactly a posit midpoint
+// (rounding to FP64 cannot carry a value across one, as midpoints are FP64 numbers);
+// there the pattern-level division decides.  NaN where 1/a lies outside scales
+// -108 .. 107.
+PGA_HD double fp64_posit_recip(double a) {
+  const double q = 1.0 / a;
+  const double c = fp64_grid_c_arith(q);
+  const double r = (q + c) - c;
+  const double half = (c / 1.5) * 0x1p-53;  // C = 1.5 * 2^E: the grid's quantum is 2^(E - 52)
+  if (__builtin_fabs(q - r) == half) return to_double(div(P_ONE, from_double(a)));
+  return r;
+}
+
 // Serial sums of posit values on the grid of one binade, for the long addition chains of
 // the triangular solves.  For t in binade e (2^e <= |t| < 2^(e+1)) at scales -108 .. 107
 // the posits of the binade are the multiples of the quantum q = 2^(e - nf), nf = 27 - |k|'

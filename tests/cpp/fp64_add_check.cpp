@@ -15,6 +15,7 @@
 
 extern "C" uint32_t orc_p32_add(uint32_t, uint32_t);
 extern "C" uint32_t orc_p32_mul(uint32_t, uint32_t);
+extern "C" uint32_t orc_p32_div(uint32_t, uint32_t);
 
 static uint64_t splitmix(uint64_t& s) {
   uint64_t z = (s += 0x9E3779B97F4A7C15ull);
@@ -257,7 +258,67 @@ int main(int argc, char** argv) {
       check_grid_chain(fam == 4 ? (uint32_t)splitmix(s) : t, p, 64);
     }
   }
+  // the arithmetic constant (pga::fp64_grid_c_arith) against the table's for every
+  // exponent field, both signs and several fractions: the same constant where the table
+  // has a finite one, infinity where it has infinity; zero stays zero
+  long arith_checked = 0;
+  for (uint32_t eb = 0; eb < 2048; ++eb)
+    for (int sg = 0; sg < 2; ++sg)
+      for (int f = 0; f < 4; ++f) {
+        const uint64_t bits = ((uint64_t)sg << 63) | ((uint64_t)eb << 52) | (splitmix(s) >> 12) * (f != 0);
+        double x;
+        __builtin_memcpy(&x, &bits, sizeof x);
+        if (eb == 2047) continue;  // inf / NaN: not a posit value
+        const double ct = grid_c_host(x), ca = pga::fp64_grid_c_arith(x);
+        ++arith_checked;
+        bool ok;
+        if (x == 0.0)
+          ok = (x + ca) - ca == 0.0;
+        else if (eb == 0)
+          ok = std::isinf(ca);  // subnormal (no posit value or product is one): redo
+        else
+          ok = std::isinf(ct) ? std::isinf(ca) : ca == ct;
+        if (!ok) {
+          ++bad;
+          if (bad < 8) printf("arith constant differs for exponent field %u: %.17g vs %.17g\n", eb, ca, ct);
+        }
+      }
   printf("checked %ld redone %ld mismatches %ld products %ld redone %ld gemm-sums %ld redone %ld grid-terms %ld exits %ld ties %ld\n",
          checked, redone, bad, mul_checked, mul_redone, gsum_checked, gsum_redone, grid_terms, grid_exits, grid_ties);
+  // the reciprocal of the LU panel (pga::fp64_posit_recip) against the oracle's division:
+  // random patterns, patterns with short fractions (quotients near midpoints), powers of
+  // two and their neighbours; NaN only where 1/a is outside scales -108 .. 107
+  long recip_checked = 0, recip_nan = 0;
+  auto check_recip = [&](uint32_t a) {
+    if (a == 0 || a == pga::P_NAR) return;
+    const double r = pga::fp64_posit_recip(pga::to_double(a));
+    const uint32_t want = orc_p32_div(pga::P_ONE, a);
+    ++recip_checked;
+    if (r != r) {
+      ++recip_nan;
+      const double w = std::fabs(pga::to_double(want));
+      if (w >= 0x1p-108 && w < 0x1p108) {
+        ++bad;
+        if (bad < 8) printf("recip of %08x: NaN inside the range\n", a);
+      }
+      return;
+    }
+    if (pga::from_double(r) != want) {
+      ++bad;
+      if (bad < 8) printf("recip of %08x: got %08x want %08x\n", a, pga::from_double(r), want);
+    }
+  };
+  for (long i = 0; i < 4 * n; ++i) check_recip((uint32_t)splitmix(s));
+  for (uint32_t hi = 0; hi < (1u << 16); ++hi) {  // short fractions at every regime
+    check_recip(hi << 16);
+    check_recip((hi << 16) | 1u);
+    check_recip((hi << 16) - 1u);
+  }
+  for (int e = -120; e <= 120; ++e) {
+    const uint32_t p2 = pga::from_double(std::ldexp(1.0, e));
+    for (int d = -3; d <= 3; ++d) check_recip(p2 + (uint32_t)d);
+  }
+  printf("arithmetic constants checked %ld, reciprocals checked %ld (NaN %ld)\n", arith_checked, recip_checked,
+         recip_nan);
   return bad ? 1 : 0;
 }

@@ -451,6 +451,45 @@ def test_lu_pair_multi_workgroup(pga, oracle, m, n, nb, kind):
     _same(dA.to_host(), want)
 
 
+@pytest.mark.parametrize("m,n,nb,kind", [(1100, 200, 64, "huge"), (1100, 200, 64, "tiny"),
+                                         (3000, 128, 64, "nar_late"), (1500, 96, 96, "zero_u_nar"),
+                                         (600, 64, 64, "ties"), (8400, 64, 64, None), (300, 40, 40, None)])
+def test_lu_panel_nodes(pga, oracle, m, n, nb, kind):
+    """32-column recursion nodes as one multi-workgroup launch (1 to 32 workgroups, FP64
+    multiply-adds in registers): scales beyond 2^+-108 in the middle of a node (the node
+    starts again with the pattern-level operations), a NaR that first meets the FP64 form
+    at the 20th column of a node, a zero multiplier above a NaR (the recursion's Rtrsm
+    rows skip it, its Rgemm rows do not), ties between workgroups, two workgroups per lane
+    slot (8400 rows), and a node width that is not a power of two (40 columns: the usual
+    recursion)."""
+    A = posit_matrix(oracle, 7 + m + n, m, n)
+    if kind == "huge":
+        A[700:720, 40:60] = oracle.from_double(2.0 ** 100)
+    elif kind == "tiny":
+        A[900:905, 70] = oracle.from_double(2.0 ** -100)
+    elif kind == "nar_late":
+        A[2500, 84] = NAR  # column 20 of the third node
+    elif kind == "zero_u_nar":
+        A[:, 2] = 0
+        A[2, 2] = oracle.from_double(1.0)
+        A[3:, 2] = 0
+        A[1400, 1] = NAR  # below the diagonal of column 1: in L
+        A[1, 5:30] = 0    # row 1 is in U: zero multipliers for columns 5 .. 29
+    elif kind == "ties":
+        A[:, 10] = oracle.from_double(-0.5)
+    mn = min(m, n)
+    dA = pga.DeviceArray.from_host(A)
+    dp = pga.DeviceArray(mn, np.int32, "C")
+    di = pga.DeviceArray(1, np.int32, "C")
+    pga.getrf_device(m, n, dA, m, dp, di, nb)
+    pga.synchronize()
+    want = A.copy(order="F")
+    _, wpiv, winfo = oracle.rgetrf(want, nb=nb, nthreads=THREADS)
+    assert int(di.to_host()[0]) == winfo
+    _same(dp.to_host(), wpiv)
+    _same(dA.to_host(), want)
+
+
 def test_cholesky_1024_device(pga, oracle):
     n = 1024
     A = _spd(oracle, 61, n)
