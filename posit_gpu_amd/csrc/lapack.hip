@@ -711,16 +711,16 @@ static __device__ int potrf2_lower_regs(int n, uint32_t* a, int lda) {
 // terms in the same order, each rounded as the pattern-level operations round it as long
 // as every product and sum stays in the table's scale range.  A result outside it (NaN),
 // a NaR, or a failing diagonal (whose value may then be meaningless) makes the call return
-// -1 with the block unchanged, and the caller redoes it with the pattern-level form.  sqrt
-// and the reciprocal stay pattern-level.
+// -1 with the block unchanged, and the caller redoes it with the pattern-level form.  The
+// root and the reciprocal come from FP64 rounded to the posit grid (fp64_posit_sqrt,
+// fp64_posit_recip: the pattern-level operation decides where that could round wrongly).
 template <int T>
 static __device__ int potrf2_lower_decoded(int n, uint32_t* a, int lda) {
   constexpr int SLOTS = (POTRF2_ONE_WG_MAX * (POTRF2_ONE_WG_MAX + 1) / 2 + T - 1) / T;
   __shared__ double col[POTRF2_ONE_WG_MAX];
-  __shared__ uint32_t dg;
+  __shared__ double dg;
   double c[SLOTS];  // posit values, exact in FP64 (NaR and out-of-range results: NaN)
   int ri[SLOTS], ci[SLOTS];
-  gemm::round_table_fill(threadIdx.x, T);
   {
     int j = 0, off = 0;
 #pragma unroll
@@ -735,20 +735,20 @@ static __device__ int potrf2_lower_decoded(int n, uint32_t* a, int lda) {
       c[q] = to_double(j < n ? a[ri[q] + (size_t)j * lda] : 0u);
     }
   }
-  if (threadIdx.x == 0) dg = a[0];
+  if (threadIdx.x == 0) dg = to_double(a[0]);
   __syncthreads();
   for (int j = 0; j < n; ++j) {
-    const uint32_t v = dg;
-    if (le(v, 0)) return -1;  // uniform; the pattern-level form decides
-    const uint32_t d = pga::sqrt(v);
-    const double r = to_double(j + 1 < n ? div(P_ONE, d) : P_ONE);
+    const double v = dg;
+    if (!(v > 0.0)) return -1;  // uniform (zero, negative or NaR); the pattern-level form decides
+    const double d = fp64_posit_sqrt(v);
+    const double r = j + 1 < n ? fp64_posit_recip(d) : 1.0;
 #pragma unroll
     for (int q = 0; q < SLOTS; ++q) {
       if (ci[q] == j) {
         if (ri[q] == j) {
-          c[q] = to_double(d);
+          c[q] = d;
         } else {
-          c[q] = gemm::fp64_posit_mul(r, c[q]);
+          c[q] = fp64_posit_mul_arith(r, c[q]);
           col[ri[q]] = c[q];
         }
       }
@@ -760,8 +760,8 @@ static __device__ int potrf2_lower_decoded(int n, uint32_t* a, int lda) {
         const double m = col[ci[q]];
         // a zero multiplier skips the term (Rsyrk.cpp:169, Rtrsm.cpp:272); a zero factor
         // makes it zero, which leaves the sum unchanged
-        if (m != 0.0) c[q] = gemm::fp64_mac(c[q], -m, col[ri[q]]);
-        if (ri[q] == j + 1 && ci[q] == j + 1) dg = from_double(c[q]);  // NaN -> NaR: redo
+        if (m != 0.0) c[q] = fp64_mac_arith(c[q], -m, col[ri[q]]);
+        if (ri[q] == j + 1 && ci[q] == j + 1) dg = c[q];  // NaN: redo
       }
     }
     __syncthreads();
@@ -793,6 +793,42 @@ __global__ __launch_bounds__(WG, 2) void k_potrf2_wg(int n, uint32_t* a, int lda
   const int f = wg::potrf2<UPPER>(n, blk, n, sh);
   __syncthreads();
   for (int e = threadIdx.x; e < n * n; e += WG) a[e % n + (size_t)(e / n) * lda] = blk[e];
+  if (f && threadIdx.x == 0) *info = col_off + f;
+}
+
+// The lower Rpotrf2 recursion (Rpotrf2.cpp:84-145) on a block in memory, as
+// wg::potrf2<false> walks it, with its products one element per lane (wg::gemm_direct):
+// neither LDS tiles nor many registers.  A failing step returns its 1-based column and
+// leaves the block as the reference leaves it.
+static __device__ int potrf2_lower_plain(int n, uint32_t* a, int lda) {
+  for (int j = 0; j < n; ++j) {
+    const uint32_t v = a[j + (size_t)j * lda];
+    if (le(v, 0)) return j + 1;  // Rpotrf2.cpp:90 (NaR is below zero in pattern order)
+    __syncthreads();
+    if (threadIdx.x == 0) a[j + (size_t)j * lda] = pga::sqrt(v);
+    __syncthreads();
+    int s, mid, e;
+    if (wg::node_after(j, n, s, mid, e)) {  // Rpotrf2.cpp:132-137
+      const int n1 = mid - s, n2 = e - mid;
+      uint32_t* a21 = wg::at(a, lda, mid, s);
+      wg::trsm_rltn(n2, n1, wg::at(a, lda, s, s), lda, a21, lda);
+      wg::gemm_direct<false, true, true>(n2, n2, n1, a21, lda, a21, lda, wg::at(a, lda, mid, mid), lda, 1);
+    }
+  }
+  return 0;
+}
+
+// The lower block on the critical path of the lower Cholesky: the FP64 form in registers
+// (potrf2_lower_decoded), and where that form gives up, the recursion itself on the block
+// in memory (potrf2_lower_plain).  80 VGPRs and under 1 KB of LDS, so that it finds room
+// on a CU beside the concurrent trailing update.
+__global__ __launch_bounds__(WG, 2) void k_potrf2_lower_fast(int n, uint32_t* a, int lda, int* info, int col_off) {
+  PGA_KERNEL_VGPR_FLOOR();
+  PGA_URGENT();
+  if (*info) return;  // every lane reads the same flag
+  if (potrf2_lower_decoded<WG>(n, a, lda) == 0) return;
+  __syncthreads();
+  const int f = potrf2_lower_plain(n, a, lda);
   if (f && threadIdx.x == 0) *info = col_off + f;
 }
 
@@ -1353,7 +1389,7 @@ static hipError_t potrf2(const Ctx& c, bool upper, int n, uint32_t* a, int lda, 
   if (n == 0) return hipSuccess;
   if (n <= POTRF2_ONE_WG_MAX) {
     if (upper) hipLaunchKernelGGL(k_potrf2_wg<true>, dim3(1), dim3(WG), 0, c.s, n, a, lda, info, col_off);
-    else hipLaunchKernelGGL(k_potrf2_wg<false>, dim3(1), dim3(WG), 0, c.s, n, a, lda, info, col_off);
+    else hipLaunchKernelGGL(k_potrf2_lower_fast, dim3(1), dim3(WG), 0, c.s, n, a, lda, info, col_off);
     return hipGetLastError();
   }
   if (n == 1) {
@@ -1521,17 +1557,31 @@ static int trail_tile(long m, long ncols, int nb) {
 }
 
 static hipError_t potrf_trail(hipStream_t s, int m, int ncols, int kb, const uint32_t* L, int lda, uint32_t* C,
-                              int nb, bool urgent, const int* stop) {
+                              int nb, bool urgent, const int* stop, int cap = 0) {
   if (m <= 0 || ncols <= 0 || kb <= 0) return hipSuccess;
   const int T = trail_tile(m, ncols, nb);
   const long tm = (m + T - 1) / T, tc = (ncols + T - 1) / T;
   const unsigned grid = (unsigned)(tc * tm - tc * (tc - 1) / 2);
   if (T == 64)
-    hipLaunchKernelGGL(k_potrf_trail<64>, dim3(grid), dim3(WG), 0, s, m, ncols, kb, L, lda, C, nb, (int)urgent, stop);
+    hipLaunchKernelGGL(k_potrf_trail<64>, dim3(grid), dim3(WG), lds_for_cap((const void*)k_potrf_trail<64>, cap), s, m,
+                       ncols, kb, L, lda, C, nb, (int)urgent, stop);
   else if (T == 32)
-    hipLaunchKernelGGL(k_potrf_trail<32>, dim3(grid), dim3(WG), 0, s, m, ncols, kb, L, lda, C, nb, (int)urgent, stop);
-  else hipLaunchKernelGGL(k_potrf_trail<16>, dim3(grid), dim3(WG), 0, s, m, ncols, kb, L, lda, C, nb, (int)urgent, stop);
+    hipLaunchKernelGGL(k_potrf_trail<32>, dim3(grid), dim3(WG), lds_for_cap((const void*)k_potrf_trail<32>, cap), s, m,
+                       ncols, kb, L, lda, C, nb, (int)urgent, stop);
+  else
+    hipLaunchKernelGGL(k_potrf_trail<16>, dim3(grid), dim3(WG), lds_for_cap((const void*)k_potrf_trail<16>, cap), s, m,
+                       ncols, kb, L, lda, C, nb, (int)urgent, stop);
   return hipGetLastError();
+}
+
+// Workgroups per CU of the lower Cholesky's bulk trailing update (PGA_CHOL_TRAIL_CAP; 0:
+// no cap), so that the diagonal block and panel kernels of the next block find room
+static int chol_trail_cap() {
+  static const int v = [] {
+    const char* e = getenv("PGA_CHOL_TRAIL_CAP");
+    return e ? std::max(0, atoi(e)) : 0;
+  }();
+  return v;
 }
 
 // Lower Rpotrf_opencl in right-looking order with one block of lookahead.  Each element
@@ -1564,7 +1614,9 @@ static hipError_t potrf_right_lower(const Ctx& c, int n, uint32_t* a, int lda, i
     if (c1 < n) {
       if ((e = hipEventRecord(x.join, c.s)) != hipSuccess) break;
       if ((e = hipStreamWaitEvent(x.s, x.join, 0)) != hipSuccess) break;
-      if ((e = potrf_trail(x.s, n - c1, n - c1, nb, A(c1, j), lda, A(c1, c1), nb, false, info)) != hipSuccess) break;
+      if ((e = potrf_trail(x.s, n - c1, n - c1, nb, A(c1, j), lda, A(c1, c1), nb, false, info, chol_trail_cap())) !=
+          hipSuccess)
+        break;
     }
     // block column j+1 has had every earlier block's terms once the previous aux launch ends
     if (recorded && (e = hipStreamWaitEvent(c.s, x.done, 0)) != hipSuccess) break;

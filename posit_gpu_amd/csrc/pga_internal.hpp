@@ -22,6 +22,11 @@ struct GemmOpts {
   int cap_per_cu = 0;        // at most this many workgroups per CU (0: no cap), through unused LDS
 };
 
+// Dynamic LDS that keeps a kernel at `cap` workgroups per CU or fewer (0: none): each
+// workgroup then holds more than 1/(cap + 1) of the CU's LDS.  A factorization caps its
+// trailing update so that the waves of its concurrent panel kernels find room on a CU.
+unsigned lds_for_cap(const void* kernel, int cap);
+
 // Device-pointer GEMM launch (rgemm.hip); arguments already validated.
 hipError_t rgemm_launch(bool ta, bool tb, int m, int n, int k, uint32_t alpha, const uint32_t* A,
                         int lda, const uint32_t* B, int ldb, uint32_t beta, uint32_t* C, int ldc,

@@ -389,6 +389,20 @@ PGA_HD double fp64_grid_c_arith(double x) {
   return __builtin_ldexp(1.5, in ? e + 25 + ak : 2000);
 }
 
+// The GEMM's FP64 product and multiply-add roundings (rgemm_tile.hpp: fp64_posit_mul,
+// fp64_mac) with the constants computed (fp64_grid_c_arith) instead of read from a table:
+// round(t * a) as fma(t, a, C) - C, and round(c + round(t * a)) as (x + C) - C.  NaN where
+// a scale leaves -108 .. 107 or an operand is NaR.
+PGA_HD double fp64_posit_mul_arith(double t, double a) {
+  const double c = fp64_grid_c_arith(t * a);
+  return __builtin_fma(t, a, c) - c;
+}
+PGA_HD double fp64_mac_arith(double c, double t, double a) {
+  const double x = c + fp64_posit_mul_arith(t, a);
+  const double k = fp64_grid_c_arith(x);
+  return (x + k) - k;
+}
+
 // round(1 / a) on the posit grid for a nonzero posit value a held in FP64: the correctly
 // rounded FP64 quotient, rounded to the grid of its binade ((q + C) - C).  Rounding twice
 // differs from rounding once only where the FP64 quotient is exactly a posit midpoint
@@ -401,6 +415,20 @@ PGA_HD double fp64_posit_recip(double a) {
   const double r = (q + c) - c;
   const double half = (c / 1.5) * 0x1p-53;  // C = 1.5 * 2^E: the grid's quantum is 2^(E - 52)
   if (__builtin_fabs(q - r) == half) return to_double(div(P_ONE, from_double(a)));
+  return r;
+}
+
+// sqrt(a) on the posit grid for a positive posit value a held in FP64: the FP64 root
+// (within an ulp of the true one) rounded to the grid of its binade.  That gives the posit
+// root unless the true root lies within a few FP64 ulps of a posit midpoint; there the
+// pattern-level root decides.
+PGA_HD double fp64_posit_sqrt(double a) {
+  const double q = __builtin_sqrt(a);
+  const double c = fp64_grid_c_arith(q);
+  const double r = (q + c) - c;
+  const double half = (c / 1.5) * 0x1p-53;
+  const double slack = __builtin_ldexp(q, -49);  // more than 4 ulps of q
+  if (__builtin_fabs(__builtin_fabs(q - r) - half) <= slack) return to_double(pga::sqrt(from_double(a)));
   return r;
 }
 

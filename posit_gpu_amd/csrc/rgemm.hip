@@ -186,19 +186,6 @@ __global__ __launch_bounds__(256) void scale_c(int m, int n, uint32_t beta, uint
 // Tile shapes: 64x64 with 4x4 elements per lane for large C; 32x32 (2x2) and 16x16
 // (1x1) keep enough waves in flight for skinny C (e.g. Cholesky's N = 64 updates),
 // since K is never split.
-// Dynamic LDS that keeps a kernel at `cap` workgroups per CU or fewer (0: none): each
-// workgroup then holds more than 1/(cap + 1) of the CU's LDS.  A factorization caps its
-// trailing update so that the waves of its concurrent panel kernel find room on a CU.
-static unsigned lds_for_cap(const void* kernel, int cap) {
-  if (cap <= 0) return 0;
-  hipFuncAttributes fa;
-  if (hipFuncGetAttributes(&fa, kernel) != hipSuccess) {
-    (void)hipGetLastError();
-    return 0;
-  }
-  const long need = (long)runtime().lds_per_cu / (cap + 1) + 1 - (long)fa.sharedSizeBytes;
-  return need > 0 ? (unsigned)need : 0u;
-}
 
 template <int BM, int TM, bool TA, bool TB, bool SKIP0>
 static hipError_t launch_tiled_cfg(int m, int n, int k, uint32_t alpha, const uint32_t* A, int lda,
@@ -209,7 +196,7 @@ static hipError_t launch_tiled_cfg(int m, int n, int k, uint32_t alpha, const ui
   int tm = (m + BM - 1) / BM, tn = (n + BN - 1) / BN;
   long long blocks = (long long)tm * tn;
   if (blocks > 0x7FFFFFFF) return hipErrorInvalidValue;
-  const unsigned dyn = lds_for_cap((const void*)rgemm_tiled<BM, BN, BK, TM, TN, TA, TB, SKIP0>, cap);
+  const unsigned dyn = pga::lds_for_cap((const void*)rgemm_tiled<BM, BN, BK, TM, TN, TA, TB, SKIP0>, cap);
   hipLaunchKernelGGL((rgemm_tiled<BM, BN, BK, TM, TN, TA, TB, SKIP0>), dim3((unsigned)blocks), dim3(NT),
                      dyn, s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tm, tri, urgent, stop);
   return hipGetLastError();
@@ -245,7 +232,7 @@ static hipError_t launch_panel(int m, int n, int k, uint32_t alpha, const uint32
   hipLaunchKernelGGL((k_panel_a<TA>), dim3(tm * ks), dim3(256), 0, s, m, k, A, lda, const_cast<double*>(pn.a), ks);
   hipLaunchKernelGGL((k_panel_t<TB, TA, SKIP0>), dim3(tn * ks), dim3(256), 0, s, n, k, alpha, B, ldb,
                      const_cast<double*>(pn.t), ks);
-  const unsigned dyn = lds_for_cap((const void*)rgemm_tiled_panel<TA, TB, SKIP0>, cap);
+  const unsigned dyn = pga::lds_for_cap((const void*)rgemm_tiled_panel<TA, TB, SKIP0>, cap);
   hipLaunchKernelGGL((rgemm_tiled_panel<TA, TB, SKIP0>), dim3((unsigned)blocks), dim3(256), dyn, s, m, n, k, alpha,
                      A, lda, B, ldb, beta, C, ldc, tm, tri, urgent, stop, pn);
   hipError_t e = hipGetLastError();
@@ -324,6 +311,17 @@ static hipError_t launch_persistent_tile(int tile, int grid, const BatchProblem*
 }
 
 }  // namespace gemm
+
+unsigned lds_for_cap(const void* kernel, int cap) {
+  if (cap <= 0) return 0;
+  hipFuncAttributes fa;
+  if (hipFuncGetAttributes(&fa, kernel) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  const long need = (long)runtime().lds_per_cu / (cap + 1) + 1 - (long)fa.sharedSizeBytes;
+  return need > 0 ? (unsigned)need : 0u;
+}
 
 // tiles of one problem for the persistent kernel's tile size `tile` (64 means 64 x 32)
 int batched_tiles(int m, int n, int tile) {

@@ -16,6 +16,7 @@
 extern "C" uint32_t orc_p32_add(uint32_t, uint32_t);
 extern "C" uint32_t orc_p32_mul(uint32_t, uint32_t);
 extern "C" uint32_t orc_p32_div(uint32_t, uint32_t);
+extern "C" uint32_t orc_p32_sqrt(uint32_t);
 
 static uint64_t splitmix(uint64_t& s) {
   uint64_t z = (s += 0x9E3779B97F4A7C15ull);
@@ -318,7 +319,33 @@ int main(int argc, char** argv) {
     const uint32_t p2 = pga::from_double(std::ldexp(1.0, e));
     for (int d = -3; d <= 3; ++d) check_recip(p2 + (uint32_t)d);
   }
-  printf("arithmetic constants checked %ld, reciprocals checked %ld (NaN %ld)\n", arith_checked, recip_checked,
-         recip_nan);
+  // the square root of the Cholesky diagonal blocks (pga::fp64_posit_sqrt) against the
+  // oracle's: random positive patterns, short fractions at every regime, perfect squares
+  // and their neighbours
+  long sqrt_checked = 0;
+  auto check_sqrt = [&](uint32_t a) {
+    if (a == 0 || (a >> 31)) return;
+    const double r = pga::fp64_posit_sqrt(pga::to_double(a));
+    const uint32_t want = orc_p32_sqrt(a);
+    ++sqrt_checked;
+    if (pga::from_double(r) != want) {
+      ++bad;
+      if (bad < 8) printf("sqrt of %08x: got %08x want %08x\n", a, pga::from_double(r), want);
+    }
+  };
+  for (long i = 0; i < 4 * n; ++i) check_sqrt((uint32_t)splitmix(s) >> 1);
+  for (uint32_t hi = 0; hi < (1u << 15); ++hi) {
+    check_sqrt(hi << 16);
+    check_sqrt((hi << 16) | 1u);
+    check_sqrt((hi << 16) - 1u);
+  }
+  for (uint32_t k = 1; k < 200000; ++k) {  // perfect squares of short values
+    const uint32_t sq = pga::from_double((double)k * (double)k * 0x1p-20);
+    check_sqrt(sq);
+    check_sqrt(sq + 1u);
+    check_sqrt(sq - 1u);
+  }
+  printf("arithmetic constants checked %ld, reciprocals checked %ld (NaN %ld), roots checked %ld\n", arith_checked,
+         recip_checked, recip_nan, sqrt_checked);
   return bad ? 1 : 0;
 }
