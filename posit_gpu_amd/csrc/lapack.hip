@@ -412,8 +412,10 @@ __global__ __launch_bounds__(WG) void k_trsm_right_sweep(int m, int n, uint32_t 
 // (Rtrsm.cpp:263-285), on RSW_ROWS rows of B per workgroup with B's rows and L held in
 // LDS as FP64 values: step k scales column k by 1/l_kk (the reciprocals formed once, in
 // parallel), then subtracts l_jk * b_k from every later column j (skipped when l_jk is
-// zero), each product and sum rounded to the posit grid as in the GEMM (gemm::fp64_mac).
-// A scale outside the table or a NaR leaves the block untouched in memory and the rows are
+// zero), each product and sum rounded to the posit grid as in the GEMM (fp64_mac_arith:
+// the GEMM's roundings with computed constants); L's strictly lower triangle is packed, so
+// that the kernel's 25 KB of LDS leave it room beside the concurrent trailing update.  A
+// scale outside -108 .. 107 or a NaR leaves the block untouched in memory and the rows are
 // solved again with the pattern-level sweep.
 constexpr int RSW_ROWS = 16, RSW_NMAX = 64;
 
@@ -421,7 +423,8 @@ __global__ __launch_bounds__(WG) void k_trsm_rltn_fp64(int m, int n, const uint3
                                                        uint32_t* __restrict__ B, int ldb, const int* stop) {
   PGA_KERNEL_VGPR_FLOOR();
   PGA_URGENT();
-  __shared__ double sL[RSW_NMAX][RSW_NMAX];  // sL[k][j] = l_jk
+  // the strictly lower triangle of L, packed by columns: l_jk (j > k) at tri(k) + j - k - 1
+  __shared__ double sL[RSW_NMAX * (RSW_NMAX - 1) / 2];
   __shared__ double sB[RSW_NMAX][RSW_ROWS];  // sB[column][row]
   __shared__ double sR[RSW_NMAX];            // 1 / l_kk
   const int r0 = blockIdx.x * RSW_ROWS;
@@ -429,22 +432,23 @@ __global__ __launch_bounds__(WG) void k_trsm_rltn_fp64(int m, int n, const uint3
   const int R = min(RSW_ROWS, m - r0);
   const int tid = threadIdx.x;
   uint32_t* b = B + r0;
-  gemm::round_table_fill(tid, WG);
-  for (int e = tid; e < n * n; e += WG) {
-    const int k = e / n, j = e % n;
-    sL[k][j] = j >= k ? to_double(A[j + (size_t)k * lda]) : 0.0;
-  }
+  auto tri = [n](int k) { return k * n - k * (k + 1) / 2; };
+  for (int k = 0; k + 1 < n; ++k)
+    for (int j = k + 1 + tid; j < n; j += WG) sL[tri(k) + j - k - 1] = to_double(A[j + (size_t)k * lda]);
   for (int e = tid; e < R * n; e += WG) sB[e / R][e % R] = to_double(b[e % R + (size_t)(e / R) * ldb]);
-  if (tid < n) sR[tid] = to_double(div(P_ONE, A[tid + (size_t)tid * lda]));
+  if (tid < n) {
+    const double d = to_double(A[tid + (size_t)tid * lda]);
+    sR[tid] = (d != 0.0 && d == d) ? fp64_posit_recip(d) : __builtin_nan("");  // NaR or zero: redo
+  }
   __syncthreads();
   for (int k = 0; k < n; ++k) {
-    if (tid < R) sB[k][tid] = gemm::fp64_posit_mul(sR[k], sB[k][tid]);
+    if (tid < R) sB[k][tid] = fp64_posit_mul_arith(sR[k], sB[k][tid]);
     __syncthreads();
     const int cnt = n - 1 - k;
     for (int e = tid; e < R * cnt; e += WG) {
       const int i = e % R, j = k + 1 + e / R;
-      const double t = sL[k][j];
-      if (t != 0.0) sB[j][i] = gemm::fp64_mac(sB[j][i], -t, sB[k][i]);
+      const double t = sL[tri(k) + j - k - 1];
+      if (t != 0.0) sB[j][i] = fp64_mac_arith(sB[j][i], -t, sB[k][i]);
     }
     __syncthreads();
   }
