@@ -1532,6 +1532,36 @@ __global__ __launch_bounds__(WG, 3) void k_potrf_trail(int m, int ncols, int kb,
   }
 }
 
+// k_potrf_trail<64> with its slices copied from FP64 operand panels of L (A side) and
+// -L^T (B side) instead of decoded in every tile: 28 KB of LDS, so five workgroups share a
+// CU as in the plain GEMM.  The same tiles, forms and order.
+__global__ __launch_bounds__(WG, 3) void k_potrf_trail_panel(int m, int ncols, int kb, const uint32_t* __restrict__ L,
+                                                          int lda, uint32_t* __restrict__ C, int nb, int urgent,
+                                                          const int* stop, gemm::Panels pn) {
+  PGA_KERNEL_VGPR_FLOOR();
+  if (urgent) PGA_URGENT();
+  __shared__ __align__(16) double sA[16][64];
+  __shared__ __align__(16) double sT[16][32];
+  if (stopped(stop)) return;
+  const int tm = (m + 63) / 64, tc = (ncols + 63) / 64, t = blockIdx.x;
+  int lo = 0, hi = tc - 1;  // column q of the trapezoid holds tiles q .. tm-1
+  while (lo < hi) {
+    const int q = (lo + hi + 1) >> 1;
+    if (q * tm - q * (q - 1) / 2 <= t) lo = q;
+    else hi = q - 1;
+  }
+  const int tj = lo, ti = tj + (t - (tj * tm - tj * (tj - 1) / 2));
+  const int i0 = ti * 64, j0 = tj * 64;
+  for (int jh = j0; jh < min(j0 + 64, ncols); jh += 32) {
+    if (i0 / nb == jh / nb)
+      gemm::gemm_tile_impl_fp64<64, 32, 16, 4, 2, false, true, true, true>(m, ncols, kb, P_MONE, L, lda, L, lda, P_ONE,
+                                                                          C, lda, i0, jh, 1, sA, sT, pn);
+    else
+      gemm::gemm_tile_impl_fp64<64, 32, 16, 4, 2, false, true, false, true>(m, ncols, kb, P_MONE, L, lda, L, lda,
+                                                                           P_ONE, C, lda, i0, jh, 0, sA, sT, pn);
+  }
+}
+
 // Rpotrf_opencl returns at the first failing diagonal block (Rpotrf_opencl.cpp:137-139):
 // the panel below that block and every later column keep their input values.  Once a
 // right-looking factorization has failed, put the saved inputs back there: columns c from
@@ -1560,12 +1590,46 @@ static int trail_tile(long m, long ncols, int nb) {
   return 16;
 }
 
+// Large lower-Cholesky trailing updates read FP64 operand panels (PGA_CHOL_TRAIL_PANELS=0
+// turns that off)
+static bool chol_trail_panels() {
+  static const bool v = [] {
+    const char* e = getenv("PGA_CHOL_TRAIL_PANELS");
+    return !e || atoi(e) != 0;
+  }();
+  return v;
+}
+
+// Workgroups per CU of the lower Cholesky's bulk trailing update (PGA_CHOL_TRAIL_CAP; 0:
+// no cap), so that the diagonal block and panel kernels of the next block find room
+static int chol_trail_cap() {
+  static const int v = [] {
+    const char* e = getenv("PGA_CHOL_TRAIL_CAP");
+    return e ? std::max(0, atoi(e)) : 0;
+  }();
+  return v;
+}
+
 static hipError_t potrf_trail(hipStream_t s, int m, int ncols, int kb, const uint32_t* L, int lda, uint32_t* C,
                               int nb, bool urgent, const int* stop, int cap = 0) {
   if (m <= 0 || ncols <= 0 || kb <= 0) return hipSuccess;
   const int T = trail_tile(m, ncols, nb);
   const long tm = (m + T - 1) / T, tc = (ncols + T - 1) / T;
   const unsigned grid = (unsigned)(tc * tm - tc * (tc - 1) / 2);
+  if (T == 64 && chol_trail_panels() && (long long)m * ncols * kb >= (1LL << 28)) {
+    // the B-side panel is -L^T; the Rsyrk tiles' zero skip is the tile's own (a zero stays zero)
+    gemm::Panels pn;
+    void* ws = nullptr;
+    hipError_t e = gemm::panels_build(false, true, true, m, ncols, kb, P_MONE, L, lda, L, lda, s, &pn, &ws);
+    if (e != hipSuccess) return e;
+    if (ws) {
+      hipLaunchKernelGGL(k_potrf_trail_panel, dim3(grid), dim3(WG), lds_for_cap((const void*)k_potrf_trail_panel, cap),
+                         s, m, ncols, kb, L, lda, C, nb, (int)urgent, stop, pn);
+      e = hipGetLastError();
+      const hipError_t f = hipFreeAsync(ws, s);
+      return e != hipSuccess ? e : f;
+    }
+  }
   if (T == 64)
     hipLaunchKernelGGL(k_potrf_trail<64>, dim3(grid), dim3(WG), lds_for_cap((const void*)k_potrf_trail<64>, cap), s, m,
                        ncols, kb, L, lda, C, nb, (int)urgent, stop);
@@ -1578,15 +1642,6 @@ static hipError_t potrf_trail(hipStream_t s, int m, int ncols, int kb, const uin
   return hipGetLastError();
 }
 
-// Workgroups per CU of the lower Cholesky's bulk trailing update (PGA_CHOL_TRAIL_CAP; 0:
-// no cap), so that the diagonal block and panel kernels of the next block find room
-static int chol_trail_cap() {
-  static const int v = [] {
-    const char* e = getenv("PGA_CHOL_TRAIL_CAP");
-    return e ? std::max(0, atoi(e)) : 0;
-  }();
-  return v;
-}
 
 // Lower Rpotrf_opencl in right-looking order with one block of lookahead.  Each element
 // still receives its terms in ascending l with the Rsyrk or Rgemm form the left-looking

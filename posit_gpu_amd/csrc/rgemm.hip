@@ -129,6 +129,35 @@ __global__ __launch_bounds__(256) void k_panel_t(int n, int k, uint32_t alpha, c
   }
 }
 
+// FP64 operand panels (declared in rgemm_tile.hpp)
+hipError_t panels_build(bool ta, bool tb, bool skip0, int m, int n, int k, uint32_t alpha, const uint32_t* A, int lda,
+                        const uint32_t* B, int ldb, hipStream_t s, Panels* pn, void** ws) {
+  *ws = nullptr;
+  hipMemPool_t pool = runtime().pool;
+  if (!pool || m <= 0 || n <= 0 || k <= 0) return hipSuccess;
+  const int tm = (m + 63) / 64, tn = (n + 31) / 32, ks = (k + 15) / 16;
+  if ((long long)tm * ks > 0x7FFFFFFF || (long long)tn * ks > 0x7FFFFFFF) return hipSuccess;
+  const size_t a_elems = (size_t)tm * ks * 16 * 64, t_elems = (size_t)tn * ks * 16 * 32;
+  if (hipMallocFromPoolAsync(ws, (a_elems + t_elems) * sizeof(double), pool, s) != hipSuccess) {
+    (void)hipGetLastError();
+    *ws = nullptr;
+    return hipSuccess;
+  }
+  double* pa = static_cast<double*>(*ws);
+  double* pt = pa + a_elems;
+  *pn = Panels{pa, pt, ks};
+  if (ta) hipLaunchKernelGGL((k_panel_a<true>), dim3(tm * ks), dim3(256), 0, s, m, k, A, lda, pa, ks);
+  else hipLaunchKernelGGL((k_panel_a<false>), dim3(tm * ks), dim3(256), 0, s, m, k, A, lda, pa, ks);
+  // the B side as the tile reads it: TN / TT sum first (no alpha), NN / NT carry alpha
+  if (tb && ta) hipLaunchKernelGGL((k_panel_t<true, true, false>), dim3(tn * ks), dim3(256), 0, s, n, k, alpha, B, ldb, pt, ks);
+  else if (!tb && ta) hipLaunchKernelGGL((k_panel_t<false, true, false>), dim3(tn * ks), dim3(256), 0, s, n, k, alpha, B, ldb, pt, ks);
+  else if (tb && skip0) hipLaunchKernelGGL((k_panel_t<true, false, true>), dim3(tn * ks), dim3(256), 0, s, n, k, alpha, B, ldb, pt, ks);
+  else if (tb) hipLaunchKernelGGL((k_panel_t<true, false, false>), dim3(tn * ks), dim3(256), 0, s, n, k, alpha, B, ldb, pt, ks);
+  else if (skip0) hipLaunchKernelGGL((k_panel_t<false, false, true>), dim3(tn * ks), dim3(256), 0, s, n, k, alpha, B, ldb, pt, ks);
+  else hipLaunchKernelGGL((k_panel_t<false, false, false>), dim3(tn * ks), dim3(256), 0, s, n, k, alpha, B, ldb, pt, ks);
+  return hipGetLastError();
+}
+
 __global__ void reset_counter(int* counter) {
   PGA_KERNEL_VGPR_FLOOR();
   if (threadIdx.x == 0) *counter = 0;

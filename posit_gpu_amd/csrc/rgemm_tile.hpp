@@ -584,5 +584,12 @@ __device__ __forceinline__ void gemm_tile_in(int m, int n, int k, uint32_t alpha
   gemm_tile_impl<BM, BN, BK, TM, TN, TA, TB, SKIP0>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, i0, j0, tri, lds);
 }
 
+// FP64 operand panels of op(A) and alpha * op(B) for the 64 x 32 tile, from the library's
+// memory pool on stream s (rgemm.hip); *ws receives the workspace to hand back with
+// hipFreeAsync on s once the kernels that read the panels are enqueued.  hipSuccess with
+// *ws == nullptr: no pool or no room (the caller decodes its slices itself).
+hipError_t panels_build(bool ta, bool tb, bool skip0, int m, int n, int k, uint32_t alpha, const uint32_t* A, int lda,
+                        const uint32_t* B, int ldb, hipStream_t s, Panels* pn, void** ws);
+
 }  // namespace gemm
 }  // namespace pga
