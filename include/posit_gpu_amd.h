@@ -92,6 +92,13 @@ pga_status pga_event_elapsed_ms(float *ms, void *start, void *stop);
  * 199-250) and Rgemm_internal (:72-197); copies only the logical sub-matrices.  transa /
  * transb follow Rgemm's argument check ('N','T','C' in either case; 'C' means 'T').
  * pga_rgemm_device: the same on device pointers, asynchronous on `stream`. */
+/* Autotuner of the plain GEMM: times the tile edges {64 (with FP64 operand panels where
+ * m*n*k >= 2^30), 32, 16} -- the workgroup's block of C, which takes the place of
+ * OPENCL_GEMM_BLOCKSIZE -- on scratch operands of this exact shape and records the fastest
+ * for later calls of that shape (pga_rgemm, pga_rgemm_device, the drop-in RgemmG).  Every
+ * tile gives the same bits.  *tile receives it (0 for an empty shape). */
+pga_status pga_rgemm_autotune(char transa, char transb, int m, int n, int k, uint32_t alpha, uint32_t beta,
+                              int *tile);
 pga_status pga_rgemm(char transa, char transb, int m, int n, int k, uint32_t alpha,
                      const uint32_t *a, int lda, const uint32_t *b, int ldb, uint32_t beta,
                      uint32_t *c, int ldc);
@@ -166,11 +173,13 @@ pga_status pga_rpotrs_device(char uplo, int n, int nrhs, const uint32_t *a, int 
  * A plan describes `count` independent problems C_i = alpha_i*op(A_i)*op(B_i) +
  * beta_i*C_i (device pointers, one transa/transb for the batch) with the semantics of
  * pga_rgemm_device for each.  Executing it launches a one-lane counter reset and one
- * persistent kernel in which workgroups take tiles of all problems from a queue (largest
- * problems first).  tile = 0 lets the autotuner time tile shapes {64,32,16} x grid sizes
- * {1,2,4,8} x CUs on scratch outputs and remember the winner per batch signature (also
- * in the file named by PGA_TUNE_CACHE); otherwise tile and grid (0 = 2 per CU) are used as
- * given.  Problems with alpha == 0 or k == 0 and beta != 1 are rejected
+ * persistent kernel in which workgroups take tiles of all problems from a queue.  tile = 0
+ * lets the autotuner time tile shapes {64,32,16} (the workgroup's block of C, which takes
+ * the place of OPENCL_GEMM_BLOCKSIZE) x grid sizes {1,2,4,8} x CUs, then at the best of
+ * them the problem order (largest first, the caller's, largest and smallest alternating),
+ * on scratch outputs, and remember the winner per batch signature (also in the file named
+ * by PGA_TUNE_CACHE); otherwise tile and grid (0 = 2 per CU) are used as given, largest
+ * problems first.  Problems with alpha == 0 or k == 0 and beta != 1 are rejected
  * (PGA_ERROR_UNSUPPORTED).  The reference has no batched form: this extends RgemmG
  * (libRpack/Rgemm_opencl.cpp:199) to many problems per launch. */
 typedef struct pga_batch_plan pga_batch_plan;
@@ -184,6 +193,8 @@ pga_status pga_batch_plan_capture_graph(pga_batch_plan *plan, void *stream);
 pga_status pga_batch_plan_launch_graph(pga_batch_plan *plan, void *stream);
 pga_status pga_batch_plan_info(const pga_batch_plan *plan, int *tile, int *grid, int *total_tiles,
                                double *tune_ms);
+/* The plan's problem order: 0 largest first, 1 the caller's, 2 largest and smallest alternating */
+pga_status pga_batch_plan_schedule(const pga_batch_plan *plan, int *sched);
 pga_status pga_batch_plan_destroy(pga_batch_plan *plan);
 
 /* No-allocation batched GEMM.  No counterpart in the reference: it is for callers that
@@ -254,6 +265,12 @@ pga_status pga_factor_plan_execute(pga_factor_plan *plan, void *stream);
 pga_status pga_factor_plan_capture_graph(pga_factor_plan *plan, void *stream);
 pga_status pga_factor_plan_launch_graph(pga_factor_plan *plan, void *stream);
 pga_status pga_factor_plan_info(const pga_factor_plan *plan, int *count, int *grid);
+/* The autotuner of a plan's persistent launch: times it on scratch copies of every task's
+ * operands for each workgroup count per CU that fits (1 .. the occupancy limit) and each
+ * task order (0 longest first, 1 the caller's, 2 long and short alternating), and keeps
+ * the fastest.  Results are the same bits in every configuration.  A captured graph is
+ * released (capture again to launch the tuned form).  grid, sched and tune_ms may be NULL. */
+pga_status pga_factor_plan_autotune(pga_factor_plan *plan, int *grid, int *sched, double *tune_ms);
 pga_status pga_factor_plan_destroy(pga_factor_plan *plan);
 
 /* ---- serving: posit32 solve requests with continuous batching -----------------------

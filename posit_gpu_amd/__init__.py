@@ -96,6 +96,7 @@ _sig("pga_event_synchronize", [_P])
 _sig("pga_event_elapsed_ms", [ctypes.POINTER(ctypes.c_float), _P, _P])
 _sig("pga_rgemm", [_C, _C, _I, _I, _I, _U32, _P, _I, _P, _I, _U32, _P, _I])
 _sig("pga_rgemm_device", [_C, _C, _I, _I, _I, _U32, _P, _I, _P, _I, _U32, _P, _I, _P])
+_sig("pga_rgemm_autotune", [_C, _C, _I, _I, _I, _U32, _U32, ctypes.POINTER(ctypes.c_int)])
 _sig("pga_p32_from_f64_device", [_P, _P, _SZ, _P])
 _sig("pga_p32_to_f64_device", [_P, _P, _SZ, _P])
 _sig("pga_fill_uniform_p32_device", [_P, _SZ, ctypes.c_uint64, ctypes.c_double, ctypes.c_double, _P])
@@ -285,6 +286,15 @@ def rgemm_device(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, s
     pc = c.ptr if isinstance(c, DeviceArray) else c
     _check(_lib.pga_rgemm_device(_ch(transa), _ch(transb), m, n, k, alpha, pa, lda, pb, ldb, beta, pc, ldc,
                                  stream), "pga_rgemm_device")
+
+
+def rgemm_autotune(transa, transb, m, n, k, alpha=0xC0000000, beta=0x40000000):
+    """Times the tile edges 64 / 32 / 16 for this exact shape on scratch operands and
+    records the fastest for later GEMMs of that shape; returns it."""
+    t = ctypes.c_int()
+    _check(_lib.pga_rgemm_autotune(_ch(transa), _ch(transb), m, n, k, alpha, beta, ctypes.byref(t)),
+           "pga_rgemm_autotune")
+    return t.value
 
 
 def RgemmG(transa, transb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc):  # noqa: N802
@@ -487,6 +497,7 @@ _sig("pga_batch_plan_execute", [_P, _P])
 _sig("pga_batch_plan_capture_graph", [_P, _P])
 _sig("pga_batch_plan_launch_graph", [_P, _P])
 _sig("pga_batch_plan_info", [_P, _IP, _IP, _IP, ctypes.POINTER(ctypes.c_double)])
+_sig("pga_batch_plan_schedule", [_P, _IP])
 _sig("pga_batch_plan_destroy", [_P])
 
 
@@ -524,7 +535,9 @@ class BatchPlan:
         ms = ctypes.c_double()
         _check(_lib.pga_batch_plan_info(self.h, ctypes.byref(t), ctypes.byref(g), ctypes.byref(n), ctypes.byref(ms)),
                "pga_batch_plan_info")
-        return {"tile": t.value, "grid": g.value, "total_tiles": n.value, "tune_ms": ms.value}
+        sc = ctypes.c_int()
+        _check(_lib.pga_batch_plan_schedule(self.h, ctypes.byref(sc)), "pga_batch_plan_schedule")
+        return {"tile": t.value, "grid": g.value, "total_tiles": n.value, "tune_ms": ms.value, "schedule": sc.value}
 
     def __del__(self):
         try:
@@ -575,6 +588,7 @@ _sig("pga_factor_plan_execute", [_P, _P])
 _sig("pga_factor_plan_capture_graph", [_P, _P])
 _sig("pga_factor_plan_launch_graph", [_P, _P])
 _sig("pga_factor_plan_info", [_P, _IP, _IP])
+_sig("pga_factor_plan_autotune", [_P, _IP, _IP, ctypes.POINTER(ctypes.c_double)])
 _sig("pga_factor_plan_destroy", [_P])
 
 
@@ -621,6 +635,13 @@ class FactorPlan:
         c, g = ctypes.c_int(), ctypes.c_int()
         _check(_lib.pga_factor_plan_info(self.h, ctypes.byref(c), ctypes.byref(g)), "pga_factor_plan_info")
         return {"count": c.value, "grid": g.value}
+
+    def autotune(self):
+        """Tunes the persistent launch (workgroups per CU, task order) on scratch copies."""
+        g, sc, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        _check(_lib.pga_factor_plan_autotune(self.h, ctypes.byref(g), ctypes.byref(sc), ctypes.byref(ms)),
+               "pga_factor_plan_autotune")
+        return {"grid": g.value, "schedule": sc.value, "tune_ms": ms.value}
 
     def __del__(self):
         try:

@@ -49,6 +49,7 @@ struct pga_batch_plan {
   int count = 0;
   int tile = 64;
   int grid = 0;
+  int sched = 0;  // problem order (pga::schedule_positions): 0 largest first, 1 the caller's, 2 large and small alternating
   int total_tiles = 0;
   double tune_ms = 0.0;
   std::vector<pga::BatchProblem> host;  // in scheduling order
@@ -64,7 +65,10 @@ namespace pga {
 namespace {
 
 std::mutex g_tune_mutex;
-std::map<std::string, std::pair<int, int>> g_tune_cache;  // signature -> (tile, grid)
+struct TuneEntry {
+  int tile, grid, sched;
+};
+std::map<std::string, TuneEntry> g_tune_cache;  // signature -> tuned configuration
 bool g_tune_loaded = false;
 
 const char* tune_file() { return getenv("PGA_TUNE_CACHE"); }
@@ -76,10 +80,13 @@ void load_tune_cache() {
   if (!f) return;
   FILE* fp = fopen(f, "r");
   if (!fp) return;
-  char sig[512];
-  int tile, grid;
-  // one entry per line: <signature> <tile> <grid>
-  while (fscanf(fp, "%511s %d %d", sig, &tile, &grid) == 3) g_tune_cache[sig] = {tile, grid};
+  char line[640], sig[512];
+  int tile, grid, sched;
+  // one entry per line: <signature> <tile> <grid> [<schedule>]
+  while (fgets(line, sizeof line, fp)) {
+    sched = 0;
+    if (sscanf(line, "%511s %d %d %d", sig, &tile, &grid, &sched) >= 3) g_tune_cache[sig] = {tile, grid, sched};
+  }
   fclose(fp);
 }
 
@@ -88,7 +95,8 @@ void save_tune_cache() {
   if (!f) return;
   FILE* fp = fopen(f, "w");
   if (!fp) return;
-  for (auto& kv : g_tune_cache) fprintf(fp, "%s %d %d\n", kv.first.c_str(), kv.second.first, kv.second.second);
+  for (auto& kv : g_tune_cache)
+    fprintf(fp, "%s %d %d %d\n", kv.first.c_str(), kv.second.tile, kv.second.grid, kv.second.sched);
   fclose(fp);
 }
 
@@ -169,7 +177,35 @@ hipError_t batched_noalloc_setup(int device) {
 
 namespace {
 
-// Time every (tile, grid) candidate on scratch copies of the output blocks, keep the best.
+// Positions in the largest-first list for schedule `sched`: 0 that list, 1 the caller's
+// order (order[] holds each position's caller index), 2 largest and smallest alternating
+// (long tiles and short ones mixed through the persistent launch).
+std::vector<int> schedule_positions(const std::vector<int>& order, int sched) {
+  const int n = (int)order.size();
+  std::vector<int> pos(n);
+  for (int i = 0; i < n; ++i) pos[i] = i;
+  if (sched == 1) {
+    std::sort(pos.begin(), pos.end(), [&](int x, int y) { return order[x] < order[y]; });
+  } else if (sched == 2) {
+    for (int i = 0, lo = 0, hi = n - 1; i < n; ++i) pos[i] = (i % 2 == 0) ? lo++ : hi--;
+  }
+  return pos;
+}
+
+// Puts the plan's problems (kept in largest-first order in lpt / lpt_order) into the order
+// of schedule `sched`.
+void apply_schedule(pga_batch_plan& p, const std::vector<BatchProblem>& lpt, const std::vector<int>& lpt_order,
+                    int sched) {
+  const std::vector<int> pos = schedule_positions(lpt_order, sched);
+  for (size_t i = 0; i < pos.size(); ++i) {
+    p.host[i] = lpt[pos[i]];
+    p.order[i] = lpt_order[pos[i]];
+  }
+  p.sched = sched;
+}
+
+// Time every (tile, grid) candidate on scratch copies of the output blocks, then the
+// other problem orders at the best of them; keep the best.
 hipError_t autotune(pga_batch_plan& p) {
   const int cus = runtime().cus;
   const int tiles[3] = {64, 32, 16};
@@ -219,6 +255,29 @@ hipError_t autotune(pga_batch_plan& p) {
     }
     if (e != hipSuccess) break;
   }
+  // the problem order, at the best tile and grid
+  const std::vector<BatchProblem> lpt_scratch = p.host;
+  const std::vector<int> lpt_order = p.order;
+  int best_sched = 0;
+  p.tile = best_tile;
+  p.grid = best_grid;
+  for (int sched = 1; sched <= 2 && e == hipSuccess; ++sched) {
+    apply_schedule(p, lpt_scratch, lpt_order, sched);
+    if ((e = build_device_tables(p)) != hipSuccess) break;
+    if ((e = run(p, s)) != hipSuccess) break;  // warm-up
+    (void)hipEventRecord(e0, s);
+    for (int r = 0; r < 2; ++r) (void)run(p, s);
+    (void)hipEventRecord(e1, s);
+    (void)hipEventSynchronize(e1);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    if (ms < best) {
+      best = ms;
+      best_sched = sched;
+    }
+  }
+  p.host = lpt_scratch;
+  p.order = lpt_order;
   p.tune_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
@@ -228,6 +287,7 @@ hipError_t autotune(pga_batch_plan& p) {
   p.tile = best_tile;
   p.grid = best_grid;
   if (e != hipSuccess) return e;
+  apply_schedule(p, saved, lpt_order, best_sched);
   return build_device_tables(p);
 }
 
@@ -300,13 +360,16 @@ pga_status pga_batch_plan_create(pga_batch_plan** out, char transa, char transb,
     std::string sig = signature(*p);
     auto it = g_tune_cache.find(sig);
     if (it != g_tune_cache.end()) {
-      p->tile = it->second.first;
-      p->grid = it->second.second;
+      p->tile = it->second.tile;
+      p->grid = it->second.grid;
+      const std::vector<BatchProblem> lpt = p->host;
+      const std::vector<int> lpt_order = p->order;
+      apply_schedule(*p, lpt, lpt_order, it->second.sched);
       e = build_device_tables(*p);
     } else {
       e = p->count ? autotune(*p) : hipSuccess;
       if (e == hipSuccess && p->count) {
-        g_tune_cache[sig] = {p->tile, p->grid};
+        g_tune_cache[sig] = {p->tile, p->grid, p->sched};
         save_tune_cache();
       }
     }
@@ -354,6 +417,12 @@ pga_status pga_batch_plan_launch_graph(pga_batch_plan* p, void* stream) {
   PGA_CHECK_INIT();
   if (!p || !p->exec) return PGA_ERROR_ILLEGAL_ARGUMENT;
   PGA_HIP(hipGraphLaunch(p->exec, stream ? (hipStream_t)stream : runtime().stream));
+  return PGA_SUCCESS;
+}
+
+pga_status pga_batch_plan_schedule(const pga_batch_plan* p, int* sched) {
+  if (!p) return PGA_ERROR_ILLEGAL_ARGUMENT;
+  if (sched) *sched = p->sched;
   return PGA_SUCCESS;
 }
 

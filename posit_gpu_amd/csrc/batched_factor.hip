@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <vector>
 
 #include "pga_internal.hpp"
@@ -62,6 +63,21 @@ __global__ void k_reset_counter(int* counter) {
   if (threadIdx.x == 0) *counter = 0;
 }
 
+// The autotuner's restore: task t's operands (a, b, and ipiv of a re-solve) from src[t]
+// into dst[t], one workgroup per task.
+__global__ __launch_bounds__(NT) void k_copy_task_operands(const FactorTask* __restrict__ src,
+                                                           const FactorTask* __restrict__ dst, int count) {
+  PGA_KERNEL_VGPR_FLOOR();
+  const int t = blockIdx.x;
+  if (t >= count) return;
+  const FactorTask s = src[t], d = dst[t];
+  for (long e = threadIdx.x; e < (long)s.lda * s.n; e += NT) d.a[e] = s.a[e];
+  if (s.b)
+    for (long e = threadIdx.x; e < (long)s.ldb * s.nrhs; e += NT) d.b[e] = s.b[e];
+  if (s.ipiv && s.kind == PGA_TASK_GETRS)
+    for (int e = threadIdx.x; e < s.n; e += NT) d.ipiv[e] = s.ipiv[e];
+}
+
 }  // namespace wg
 }  // namespace pga
 
@@ -71,7 +87,9 @@ __global__ void k_reset_counter(int* counter) {
 struct pga_factor_plan {
   int count = 0;                       // tasks run one workgroup each
   int grid = 0;
+  int sched = 0;                       // task order: 0 longest first, 1 the caller's, 2 long and short alternating
   std::vector<pga::FactorTask> host;  // scheduling order
+  std::vector<int> order;             // scheduling order -> the caller's index
   pga::FactorTask* d_tasks = nullptr;
   int* d_counter = nullptr;
   pga::BlockedPlan* blocked = nullptr;  // tasks larger than one workgroup handles
@@ -143,6 +161,7 @@ pga_status pga_factor_plan_create(pga_factor_plan** out, int count, const pga_fa
   }
   auto* p = new pga_factor_plan();
   std::vector<FactorTask> v, big;
+  std::vector<int> v_idx;
   for (int i = 0; i < count; ++i) {
     const pga_factor_task& q = tasks[i];
     FactorTask t{};
@@ -170,12 +189,20 @@ pga_status pga_factor_plan_create(pga_factor_plan** out, int count, const pga_fa
       return PGA_ERROR_ILLEGAL_ARGUMENT;
     }
     if (!solve) t.nrhs = 0;
-    (t.n <= PGA_FACTOR_TASK_MAX_N ? v : big).push_back(t);
+    if (t.n <= PGA_FACTOR_TASK_MAX_N) {
+      v.push_back(t);
+      v_idx.push_back(i);
+    } else {
+      big.push_back(t);
+    }
   }
   std::vector<int> idx(v.size());
   for (size_t i = 0; i < idx.size(); ++i) idx[i] = (int)i;
   std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return task_cost(v[x]) > task_cost(v[y]); });
-  for (int i : idx) p->host.push_back(v[i]);
+  for (int i : idx) {
+    p->host.push_back(v[i]);
+    p->order.push_back(v_idx[i]);
+  }
   p->count = (int)p->host.size();
   p->grid = grid > 0 ? std::max(1, std::min(p->count, grid)) : factor_tasks_grid(p->count);
   hipError_t e = hipMalloc(&p->d_counter, sizeof(int));
@@ -232,6 +259,151 @@ pga_status pga_factor_plan_launch_graph(pga_factor_plan* p, void* stream) {
   PGA_CHECK_INIT();
   if (!p || !p->exec) return PGA_ERROR_ILLEGAL_ARGUMENT;
   PGA_HIP(hipGraphLaunch(p->exec, stream ? (hipStream_t)stream : runtime().stream));
+  return PGA_SUCCESS;
+}
+
+// Tunes the persistent launch of a plan: times it on scratch copies of every task's
+// operands for each workgroup count per CU that fits (1 .. the occupancy limit) and each
+// task order (longest first, the caller's, long and short alternating), and keeps the
+// fastest.  Every configuration gives the same bits (each task runs the same code).
+pga_status pga_factor_plan_autotune(pga_factor_plan* p, int* grid, int* sched, double* tune_ms) {
+  PGA_CHECK_INIT();
+  if (!p) return PGA_ERROR_ILLEGAL_ARGUMENT;
+  const auto t_start = std::chrono::steady_clock::now();
+  if (p->count > 0) {
+    hipStream_t s = runtime().stream;
+    // back to the longest-first list (ties in the caller's order), which every order below
+    // is taken from, so that tuning a plan again starts from the same list
+    {
+      std::vector<int> k(p->count);
+      for (int i = 0; i < p->count; ++i) k[i] = i;
+      std::sort(k.begin(), k.end(), [&](int x, int y) {
+        const double cx = task_cost(p->host[x]), cy = task_cost(p->host[y]);
+        return cx != cy ? cx > cy : p->order[x] < p->order[y];
+      });
+      std::vector<FactorTask> h(p->count);
+      std::vector<int> o(p->count);
+      for (int i = 0; i < p->count; ++i) {
+        h[i] = p->host[k[i]];
+        o[i] = p->order[k[i]];
+      }
+      p->host = h;
+      p->order = o;
+    }
+    // scratch operands, one block per task
+    std::vector<FactorTask> scratch = p->host;
+    size_t words = 0;
+    for (const FactorTask& t : p->host) {
+      words += (size_t)t.lda * t.n + (t.b ? (size_t)t.ldb * t.nrhs : 0) + (t.ipiv ? (size_t)t.n : 0) + 1;
+      words = (words + 63) & ~(size_t)63;
+    }
+    uint32_t* buf = nullptr;
+    FactorTask *d_src = nullptr, *d_dst = nullptr, *d_try = nullptr;
+    hipError_t e = hipMalloc((void**)&buf, words * 4);
+    if (e == hipSuccess) e = hipMalloc((void**)&d_src, sizeof(FactorTask) * p->count);
+    if (e == hipSuccess) e = hipMalloc((void**)&d_dst, sizeof(FactorTask) * p->count);
+    if (e == hipSuccess) e = hipMalloc((void**)&d_try, sizeof(FactorTask) * p->count);
+    if (e == hipSuccess) {
+      size_t off = 0;
+      for (FactorTask& t : scratch) {
+        t.a = buf + off;
+        off += (size_t)t.lda * t.n;
+        if (t.b) {
+          t.b = buf + off;
+          off += (size_t)t.ldb * t.nrhs;
+        }
+        if (t.ipiv) {
+          t.ipiv = reinterpret_cast<int*>(buf + off);
+          off += t.n;
+        }
+        t.info = reinterpret_cast<int*>(buf + off);
+        off = (off + 1 + 63) & ~(size_t)63;
+      }
+      e = hipMemcpyAsync(d_src, p->host.data(), sizeof(FactorTask) * p->count, hipMemcpyHostToDevice, s);
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(d_dst, scratch.data(), sizeof(FactorTask) * p->count, hipMemcpyHostToDevice, s);
+    }
+    int per_cu = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, wg::k_factor_tasks, wg::NT, 0) != hipSuccess ||
+        per_cu < 1)
+      per_cu = 1;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    float best = 1e30f;
+    int best_grid = p->grid, best_sched = 0;
+    std::vector<FactorTask> ordered(p->count);
+    for (int sc = 0; sc < 3 && e == hipSuccess; ++sc) {
+      // positions in the longest-first list for this order
+      std::vector<int> pos(p->count);
+      for (int i = 0; i < p->count; ++i) pos[i] = i;
+      if (sc == 1) std::sort(pos.begin(), pos.end(), [&](int x, int y) { return p->order[x] < p->order[y]; });
+      if (sc == 2)
+        for (int i = 0, lo = 0, hi = p->count - 1; i < p->count; ++i) pos[i] = (i % 2 == 0) ? lo++ : hi--;
+      for (int i = 0; i < p->count; ++i) ordered[i] = scratch[pos[i]];
+      if ((e = hipMemcpyAsync(d_try, ordered.data(), sizeof(FactorTask) * p->count, hipMemcpyHostToDevice, s)) !=
+          hipSuccess)
+        break;
+      for (int g = 1; g <= per_cu && e == hipSuccess; ++g) {
+        const int gr = std::max(1, std::min(p->count, g * runtime().cus));
+        float total = 0.f;
+        for (int r = 0; r < 2 && e == hipSuccess; ++r) {
+          hipLaunchKernelGGL(wg::k_copy_task_operands, dim3((unsigned)p->count), dim3(wg::NT), 0, s, d_src, d_dst,
+                             p->count);
+          (void)hipEventRecord(e0, s);
+          e = factor_tasks_launch(d_try, p->count, p->d_counter, gr, s);
+          (void)hipEventRecord(e1, s);
+          if (e == hipSuccess) e = hipEventSynchronize(e1);
+          float ms = 0.f;
+          (void)hipEventElapsedTime(&ms, e0, e1);
+          if (r > 0) total += ms;  // the first run warms up
+        }
+        if (e == hipSuccess && total < best) {
+          best = total;
+          best_grid = gr;
+          best_sched = sc;
+        }
+      }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipStreamSynchronize(s);
+    if (buf) (void)hipFree(buf);
+    if (d_src) (void)hipFree(d_src);
+    if (d_dst) (void)hipFree(d_dst);
+    if (d_try) (void)hipFree(d_try);
+    if (e != hipSuccess) {  // the plan keeps its grid, in longest-first order
+      p->sched = 0;
+      (void)hipMemcpy(p->d_tasks, p->host.data(), sizeof(FactorTask) * p->count, hipMemcpyHostToDevice);
+      return from_hip(e);
+    }
+    // the chosen order on the plan's own tasks
+    std::vector<int> pos(p->count);
+    for (int i = 0; i < p->count; ++i) pos[i] = i;
+    if (best_sched == 1) std::sort(pos.begin(), pos.end(), [&](int x, int y) { return p->order[x] < p->order[y]; });
+    if (best_sched == 2)
+      for (int i = 0, lo = 0, hi = p->count - 1; i < p->count; ++i) pos[i] = (i % 2 == 0) ? lo++ : hi--;
+    std::vector<FactorTask> host(p->count);
+    std::vector<int> order(p->count);
+    for (int i = 0; i < p->count; ++i) {
+      host[i] = p->host[pos[i]];
+      order[i] = p->order[pos[i]];
+    }
+    p->host = host;
+    p->order = order;
+    p->grid = best_grid;
+    p->sched = best_sched;
+    PGA_HIP(hipMemcpy(p->d_tasks, p->host.data(), sizeof(FactorTask) * p->count, hipMemcpyHostToDevice));
+    if (p->exec) {  // a captured graph holds the old launch: capture again to use the new one
+      (void)hipGraphExecDestroy(p->exec);
+      (void)hipGraphDestroy(p->graph);
+      p->exec = nullptr;
+      p->graph = nullptr;
+    }
+  }
+  if (grid) *grid = p->grid;
+  if (sched) *sched = p->sched;
+  if (tune_ms) *tune_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
   return PGA_SUCCESS;
 }
 

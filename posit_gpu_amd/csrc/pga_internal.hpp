@@ -27,6 +27,10 @@ struct GemmOpts {
 // trailing update so that the waves of its concurrent panel kernels find room on a CU.
 unsigned lds_for_cap(const void* kernel, int cap);
 
+// The tile edge pga_rgemm_autotune chose for this exact shape (0: none), and its record
+int tuned_tile(bool ta, bool tb, int m, int n, int k);
+void set_tuned_tile(bool ta, bool tb, int m, int n, int k, int tile);
+
 // Device-pointer GEMM launch (rgemm.hip); arguments already validated.
 hipError_t rgemm_launch(bool ta, bool tb, int m, int n, int k, uint32_t alpha, const uint32_t* A,
                         int lda, const uint32_t* B, int ldb, uint32_t beta, uint32_t* C, int ldc,

@@ -5,6 +5,10 @@
 // which is what every reference caller of RgemmG expects bit for bit.
 #include <hip/hip_runtime.h>
 
+#include <map>
+#include <mutex>
+#include <tuple>
+
 #include "pga_internal.hpp"
 #include "rgemm_tile.hpp"
 
@@ -341,6 +345,23 @@ static hipError_t launch_persistent_tile(int tile, int grid, const BatchProblem*
 
 }  // namespace gemm
 
+// Tile edges chosen by pga_rgemm_autotune, per exact GEMM shape
+namespace {
+std::mutex g_shape_mu;
+std::map<std::tuple<bool, bool, int, int, int>, int> g_shape_tile;
+}  // namespace
+
+int tuned_tile(bool ta, bool tb, int m, int n, int k) {
+  std::lock_guard<std::mutex> lk(g_shape_mu);
+  if (g_shape_tile.empty()) return 0;
+  auto it = g_shape_tile.find(std::make_tuple(ta, tb, m, n, k));
+  return it == g_shape_tile.end() ? 0 : it->second;
+}
+void set_tuned_tile(bool ta, bool tb, int m, int n, int k, int tile) {
+  std::lock_guard<std::mutex> lk(g_shape_mu);
+  g_shape_tile[std::make_tuple(ta, tb, m, n, k)] = tile;
+}
+
 unsigned lds_for_cap(const void* kernel, int cap) {
   if (cap <= 0) return 0;
   hipFuncAttributes fa;
@@ -382,7 +403,9 @@ hipError_t rgemm_launch(bool ta, bool tb, int m, int n, int k, uint32_t alpha, c
     }
     return hipGetLastError();
   }
-  int tile = o.tile ? o.tile : (runtime().gemm_tile ? runtime().gemm_tile : pick_tile(m, n, runtime().cus));
+  int tile = o.tile ? o.tile : runtime().gemm_tile;
+  if (!tile) tile = tuned_tile(ta, tb, m, n, k);
+  if (!tile) tile = pick_tile(m, n, runtime().cus);
   if (o.skip_zero_b)
     return dispatch<true>(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, impl, o.tri, o.urgent, o.stop, tile,
                           o.cap_per_cu);

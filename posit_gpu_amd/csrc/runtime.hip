@@ -420,6 +420,66 @@ pga_status pga_rgemm_device(char transa, char transb, int m, int n, int k, uint3
   return PGA_SUCCESS;
 }
 
+pga_status pga_rgemm_autotune(char transa, char transb, int m, int n, int k, uint32_t alpha, uint32_t beta,
+                              int* tile) {
+  PGA_CHECK_INIT();
+  bool ta, tb;
+  const bool ta0 = !(transa == 'N' || transa == 'n'), tb0 = !(transb == 'N' || transb == 'n');
+  const int lda = std::max(1, ta0 ? k : m), ldb = std::max(1, tb0 ? n : k), ldc = std::max(1, m);
+  int bad = check_gemm_args(transa, transb, m, n, k, lda, ldb, ldc, ta, tb);
+  if (bad) {
+    set_illegal_argument(bad);
+    return PGA_ERROR_ILLEGAL_ARGUMENT;
+  }
+  int best_tile = 0;
+  if (m > 0 && n > 0 && k > 0) {
+    hipStream_t s = g_rt.stream;
+    uint32_t *A = nullptr, *B = nullptr, *C = nullptr;
+    const size_t na = (size_t)lda * (ta ? m : k), nb = (size_t)ldb * (tb ? k : n), nc = (size_t)ldc * n;
+    hipError_t e = hipMalloc((void**)&A, na * 4);
+    if (e == hipSuccess) e = hipMalloc((void**)&B, nb * 4);
+    if (e == hipSuccess) e = hipMalloc((void**)&C, nc * 4);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(k_fill_uniform, dim3(grid_for(na)), dim3(256), 0, s, A, na, 11ull, -1.0, 1.0);
+      hipLaunchKernelGGL(k_fill_uniform, dim3(grid_for(nb)), dim3(256), 0, s, B, nb, 12ull, -1.0, 1.0);
+      hipLaunchKernelGGL(k_fill_uniform, dim3(grid_for(nc)), dim3(256), 0, s, C, nc, 13ull, -1.0, 1.0);
+      e = hipGetLastError();
+    }
+    float best = 1e30f;
+    for (int t : {64, 32, 16}) {
+      if (e != hipSuccess) break;
+      GemmOpts o;
+      o.tile = t;
+      e = rgemm_launch(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, g_rt.gemm_impl, o);  // warm-up
+      if (e != hipSuccess) break;
+      (void)hipEventRecord(e0, s);
+      for (int r = 0; r < 3 && e == hipSuccess; ++r)
+        e = rgemm_launch(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s, g_rt.gemm_impl, o);
+      (void)hipEventRecord(e1, s);
+      if (e == hipSuccess) e = hipEventSynchronize(e1);
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      if (e == hipSuccess && ms < best) {
+        best = ms;
+        best_tile = t;
+      }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipStreamSynchronize(s);
+    if (A) (void)hipFree(A);
+    if (B) (void)hipFree(B);
+    if (C) (void)hipFree(C);
+    PGA_HIP(e);
+    set_tuned_tile(ta, tb, m, n, k, best_tile);
+  }
+  if (tile) *tile = best_tile;
+  return PGA_SUCCESS;
+}
+
 pga_status pga_rgemm(char transa, char transb, int m, int n, int k, uint32_t alpha,
                      const uint32_t* a, int lda, const uint32_t* b, int ldb, uint32_t beta,
                      uint32_t* c, int ldc) {

@@ -283,6 +283,64 @@ def test_blocked_graph_replays(pga, oracle):
                 _same(got[key], want[key], f"replay {rep} task {i} {key}")
 
 
+def test_autotuned_plan_same_bits(pga, oracle):
+    """The plan's tuner (workgroups per CU, task order) times scratch copies and leaves
+    the caller's operands alone; the tuned plan, eager and captured, and a plan tuned twice
+    give the per-call bits."""
+    rng = np.random.default_rng(31)
+    cases = []
+    for i in range(40):
+        n = int(rng.integers(1, 200))
+        kind = ["getrf", "potrf", "gesv", "posv", "getrs", "potrs"][i % 6]
+        uplo = "LU"[(i // 6) % 2]
+        A = posit_matrix(oracle, 1400 + i, n, n) if kind in ("getrf", "gesv", "getrs") else _spd(oracle, 1400 + i, n)
+        B = posit_matrix(oracle, 1500 + i, n, int(rng.integers(1, 4))) if kind not in ("getrf", "potrf") else None
+        factored = None
+        if kind == "getrs":
+            F = A.copy(order="F")
+            _, piv, _ = oracle.rgetrf(F, nb=0, nthreads=THREADS)
+            factored = (F, piv)
+        elif kind == "potrs":
+            G = A.copy(order="F")
+            assert oracle.rpotrf(uplo, G, nb=0, nthreads=THREADS)[1] == 0
+            factored = (G, None)
+        cases.append(Case(pga, kind, A, uplo, [0, 16, 1][i % 3], B, factored))
+    cases.append(Case(pga, "gesv", posit_matrix(oracle, 1600, 600, 600), B=posit_matrix(oracle, 1601, 600, 2)))
+    bufs = [c.buffers() for c in cases]
+    plan = pga.FactorPlan([c.task(d) for c, d in zip(cases, bufs)])
+    plan.capture_graph()
+    untouched = [c.read(d) for c, d in zip(cases, bufs)]
+    first = plan.autotune()
+    second = plan.autotune()
+    for t in (first, second):
+        assert t["grid"] >= 1 and t["schedule"] in (0, 1, 2) and t["tune_ms"] >= 0, t
+    for i, (c, d) in enumerate(zip(cases, bufs)):  # tuning ran on copies
+        for key, v in untouched[i].items():
+            assert np.array_equal(np.asarray(c.read(d)[key]), np.asarray(v)), (i, key)
+    assert plan.info()["grid"] == second["grid"]
+    for mode in ("eager", "graph"):
+        for c, d in zip(cases, bufs):
+            src = c.A if c.factored is None else c.factored[0]
+            d["a"].copy_from_host(src.copy(order="F"))
+            if "b" in d:
+                d["b"].copy_from_host(c.B.copy(order="F"))
+            d["info"].copy_from_host(np.array([-7], np.int32))
+        pga.synchronize()
+        if mode == "eager":
+            plan.execute()
+        else:
+            plan.capture_graph()
+            plan.launch_graph()
+        pga.synchronize()
+        for i, (c, d) in enumerate(zip(cases, bufs)):
+            got, want = c.read(d), c.per_call()
+            what = f"{mode} task {i} {c.kind} n={c.n} uplo={c.uplo} nb={c.nb}"
+            assert got["info"] == want["info"], (what, got["info"], want["info"])
+            for key in want:
+                if key != "info":
+                    _same(got[key], want[key], f"{what} {key}")
+
+
 def test_rejects_bad_tasks(pga):
     a = pga.DeviceArray((4, 4))
     with pytest.raises(pga.PositError):

@@ -215,6 +215,23 @@ def test_tile_override_same_bits(pga, tmp_path):
         assert np.array_equal(outs[tile], outs[""]), tile
 
 
+@pytest.mark.parametrize("ta,tb,m,n,k", [("N", "N", 300, 270, 130), ("T", "N", 64, 1000, 33), ("N", "T", 777, 5, 260)])
+def test_autotuned_tile_same_bits(pga, oracle, ta, tb, m, n, k):
+    """pga_rgemm_autotune times the tile edges on one shape and keeps the fastest for it;
+    calls of that shape then use it and give the oracle's bits."""
+    rng = np.random.default_rng(m + n + k)
+    A = posit_matrix(oracle, 141, *((k, m) if ta == "T" else (m, k)))
+    B = posit_matrix(oracle, 142, *((n, k) if tb == "T" else (k, n)))
+    C = posit_matrix(oracle, 143, m, n)
+    C[rng.integers(0, m, 3), rng.integers(0, n, 3)] = NAR
+    before = pga.rgemm(ta, tb, MONE, A, B, ONE, C)
+    tile = pga.rgemm_autotune(ta, tb, m, n, k)
+    assert tile in (16, 32, 64)
+    got, want = _both(oracle, pga, ta, tb, MONE, ONE, A, B, C)
+    _assert_same(got, want)
+    _assert_same(before, want)
+
+
 _TILE_SCRIPT = """
 import sys, numpy as np
 sys.path.insert(0, {root!r})
