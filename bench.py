@@ -575,6 +575,10 @@ def run_batched_factor(pga, reps):
                   nb=64) for it in items]
     t0 = time.perf_counter()
     plan = pga.FactorPlan(tasks)
+    default = plan.info()
+    tune = plan.autotune()
+    log(f"plan tuned: grid {default['grid']} -> {tune['grid']}, task order {tune['schedule']}, "
+        f"{tune['tune_ms'] / 1e3:.2f} s")
     plan.capture_graph()
     t_build = time.perf_counter() - t0
     times = []
@@ -613,7 +617,8 @@ def run_batched_factor(pga, reps):
             "value": round(len(items) / t_graph, 1), "unit": "factorizations/s",
             "graph_s": round(t_graph, 4), "per_call_s": round(t_plain, 4),
             "speedup_vs_per_call": round(t_plain / t_graph, 2), "gops": round(ops / t_graph / 1e9, 2),
-            "plan": pinfo, "plan_build_s": round(t_build, 3), "bitwise_mismatches_vs_per_call": mismatches}
+            "plan": dict(pinfo, schedule=tune["schedule"], untuned_grid=default["grid"]),
+            "plan_build_s": round(t_build, 3), "bitwise_mismatches_vs_per_call": mismatches}
 
 
 def bench_batched_factor(pga, info, args):
