@@ -209,7 +209,9 @@ void apply_schedule(pga_batch_plan& p, const std::vector<BatchProblem>& lpt, con
 hipError_t autotune(pga_batch_plan& p) {
   const int cus = runtime().cus;
   const int tiles[3] = {64, 32, 16};
-  const int grids_per_cu[4] = {1, 2, 4, 8};
+  // largest grid first: workgroups beyond the queue's tiles end at once, so a larger grid
+  // is rarely much slower, while a too-small one leaves CUs idle
+  const int grids_per_cu[4] = {8, 4, 2, 1};
   // scratch outputs so that tuning does not touch the caller's C
   size_t elems = 0;
   for (auto& q : p.host) elems += (size_t)q.m * q.n;
@@ -232,6 +234,21 @@ hipError_t autotune(pga_batch_plan& p) {
   hipEvent_t e0, e1;
   (void)hipEventCreate(&e0);
   (void)hipEventCreate(&e1);
+  // ms per run of `reps` runs back to back
+  auto timed = [&](int reps, float& ms) -> hipError_t {
+    (void)hipEventRecord(e0, s);
+    for (int r = 0; r < reps; ++r)
+      if (hipError_t er = run(p, s)) return er;
+    (void)hipEventRecord(e1, s);
+    if (hipError_t er = hipEventSynchronize(e1)) return er;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    ms /= reps;
+    return hipSuccess;
+  };
+  // a candidate whose first (warm-up) run is already well behind the best so far is
+  // dropped without the timed runs, and so are the tile's smaller grids after it (a
+  // tile's first run may include loading its kernel: a wider margin there)
+  constexpr float PRUNE = 1.15f, PRUNE_FIRST = 1.5f;
   float best = 1e30f;
   int best_tile = 64, best_grid = cus;
   auto t0 = std::chrono::steady_clock::now();
@@ -240,13 +257,10 @@ hipError_t autotune(pga_batch_plan& p) {
     if ((e = build_device_tables(p)) != hipSuccess) break;
     for (int g : grids_per_cu) {
       p.grid = g * cus;
-      if ((e = run(p, s)) != hipSuccess) break;  // warm-up
-      (void)hipEventRecord(e0, s);
-      for (int r = 0; r < 2; ++r) (void)run(p, s);
-      (void)hipEventRecord(e1, s);
-      (void)hipEventSynchronize(e1);
-      float ms = 0;
-      (void)hipEventElapsedTime(&ms, e0, e1);
+      float warm = 0.f, ms = 0.f;
+      if ((e = timed(1, warm)) != hipSuccess) break;
+      if (warm > (g == grids_per_cu[0] ? PRUNE_FIRST : PRUNE) * best) break;
+      if ((e = timed(2, ms)) != hipSuccess) break;
       if (ms < best) {
         best = ms;
         best_tile = tile;
@@ -264,13 +278,10 @@ hipError_t autotune(pga_batch_plan& p) {
   for (int sched = 1; sched <= 2 && e == hipSuccess; ++sched) {
     apply_schedule(p, lpt_scratch, lpt_order, sched);
     if ((e = build_device_tables(p)) != hipSuccess) break;
-    if ((e = run(p, s)) != hipSuccess) break;  // warm-up
-    (void)hipEventRecord(e0, s);
-    for (int r = 0; r < 2; ++r) (void)run(p, s);
-    (void)hipEventRecord(e1, s);
-    (void)hipEventSynchronize(e1);
-    float ms = 0;
-    (void)hipEventElapsedTime(&ms, e0, e1);
+    float warm = 0.f, ms = 0.f;
+    if ((e = timed(1, warm)) != hipSuccess) break;
+    if (warm > PRUNE * best) continue;
+    if ((e = timed(2, ms)) != hipSuccess) break;
     if (ms < best) {
       best = ms;
       best_sched = sched;

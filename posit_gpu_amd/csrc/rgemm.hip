@@ -35,8 +35,14 @@ __device__ __forceinline__ void xcd_tile(int bid, int tiles_m, int tiles_n, int&
   ti = g * GROUP_M + r % gm;
 }
 
+// Workgroups per CU the compiler must leave room for: five 64 x 32 tiles (their FP64
+// slices and table, 28 KB of LDS each, and at most 96 VGPRs) fill a CU; smaller tiles
+// fit more anyway.
+template <int TM, int TN>
+constexpr int tile_wgs_per_cu() { return TM * TN >= 8 ? 5 : 3; }
+
 template <int BM, int BN, int BK, int TM, int TN, bool TA, bool TB, bool SKIP0>
-__global__ __launch_bounds__((BM / TM) * (BN / TN), 3) void rgemm_tiled(
+__global__ __launch_bounds__((BM / TM) * (BN / TN), (tile_wgs_per_cu<TM, TN>())) void rgemm_tiled(
     int m, int n, int k, uint32_t alpha, const uint32_t* __restrict__ A, int lda,
     const uint32_t* __restrict__ B, int ldb, uint32_t beta, uint32_t* __restrict__ C, int ldc,
     int tiles_m, int tri, int urgent, const int* __restrict__ stop) {
@@ -57,7 +63,7 @@ __global__ __launch_bounds__((BM / TM) * (BN / TN), 3) void rgemm_tiled(
 // caller wants them scheduled, e.g. most expensive first).  Each tile is computed by the
 // same gemm_tile as the plain kernel, so every element gets the same bits as a plain call.
 template <int BM, int BN, int BK, int TM, int TN, bool TA, bool TB>
-__global__ __launch_bounds__((BM / TM) * (BN / TN), 3) void rgemm_batched_persistent(
+__global__ __launch_bounds__((BM / TM) * (BN / TN), (tile_wgs_per_cu<TM, TN>())) void rgemm_batched_persistent(
     const BatchProblem* __restrict__ probs, const int* __restrict__ tile_start, int nprob, int total_tiles,
     int* __restrict__ counter) {
   PGA_KERNEL_VGPR_FLOOR();

@@ -77,6 +77,15 @@ struct TileLds {
   static constexpr int words = 3 * BK * BM + 3 * BK * BN + 16 * 4;
 };
 
+// Tiles of several elements per lane always take the FP64 form (gemm_tile_impl), so a
+// kernel that runs only such a tile needs only the FP64 slices: 2 words per element.
+template <int TM, int TN>
+constexpr bool tile_fp64_only() { return TM * TN >= 4; }
+template <int BM, int BN, int BK, int TM, int TN>
+struct TileLdsFor {
+  static constexpr int words = tile_fp64_only<TM, TN>() ? 2 * BK * BM + 2 * BK * BN : TileLds<BM, BN, BK>::words;
+};
+
 // The tile body with the integer decoded multiply-add (gemm::mac): used for one element
 // per lane and short K, where the FP64 form would wait on two LDS table reads per step
 // with nothing else to issue and fill its table for little work.
@@ -550,12 +559,12 @@ __device__ __forceinline__ void gemm_tile_impl(int m, int n, int k, uint32_t alp
   // one element per lane: the FP64 form pays for its table and its LDS round trips only
   // over long K loops (a 512^3 GEMM: 1.45x the integer form; the LU panel's node updates,
   // K <= 128: 0.35x)
-  if (TM * TN >= 4 || k >= 256) {
+  if (tile_fp64_only<TM, TN>() || k >= 256) {
     double* d = reinterpret_cast<double*>(lds);
     gemm_tile_impl_fp64<BM, BN, BK, TM, TN, TA, TB, SKIP0>(
         m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, i0, j0, tri, reinterpret_cast<double(*)[BM]>(d),
         reinterpret_cast<double(*)[BN]>(d + BK * BM));
-  } else {
+  } else if constexpr (!tile_fp64_only<TM, TN>()) {  // (not even instantiated for such tiles)
     gemm_tile_impl_int<BM, BN, BK, TM, TN, TA, TB, SKIP0>(
         m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, i0, j0, tri, reinterpret_cast<uint32_t(*)[BM]>(lds),
         reinterpret_cast<int(*)[BM]>(lds + BK * BM), reinterpret_cast<uint32_t(*)[BM]>(lds + 2 * BK * BM),
@@ -570,7 +579,7 @@ template <int BM, int BN, int BK, int TM, int TN, bool TA, bool TB, bool SKIP0>
 __device__ __forceinline__ void gemm_tile(int m, int n, int k, uint32_t alpha, const uint32_t* __restrict__ A,
                                           int lda, const uint32_t* __restrict__ B, int ldb, uint32_t beta,
                                           uint32_t* __restrict__ C, int ldc, int i0, int j0, int tri) {
-  __shared__ __align__(16) uint32_t lds[TileLds<BM, BN, BK>::words];
+  __shared__ __align__(16) uint32_t lds[TileLdsFor<BM, BN, BK, TM, TN>::words];
   gemm_tile_impl<BM, BN, BK, TM, TN, TA, TB, SKIP0>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, i0, j0, tri, lds);
 }
 
