@@ -147,12 +147,12 @@ __global__ __launch_bounds__(PAIR_NT) void k_bb_pair(const FactorTask* __restric
 // The node's Rgemm (A22 -= A21 A12, Rgetrf2.cpp:155-159) of every problem's panel in one
 // launch of T x T tiles (workgroups take tiles from a counter).
 template <int T>
-__global__ __launch_bounds__(NT, 3) void k_bb_node_gemm(const FactorTask* __restrict__ tasks,
+__global__ __launch_bounds__(NT, (gemm::square_tile_wgs_per_cu<T>())) void k_bb_node_gemm(const FactorTask* __restrict__ tasks,
                                                      const Item* __restrict__ items,
                                                      const int* __restrict__ tile_start, int nitems, int total,
                                                      int* __restrict__ counter, int s, int mid, int e) {
   PGA_KERNEL_VGPR_FLOOR();
-  __shared__ __align__(16) uint32_t lds[gemm::TileLds<T, T, 16>::words];
+  __shared__ __align__(16) uint32_t lds[gemm::square_tile_lds_words<T>()];
   __shared__ int s_tile;
   for (;;) {
     if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1);
@@ -227,10 +227,10 @@ __device__ __forceinline__ void bb_tile(const Gemm& p, int i0, int j0, uint32_t*
 }
 
 template <int T>
-__global__ __launch_bounds__(NT, 3) void k_bb_gemm(const Gemm* __restrict__ probs, const int* __restrict__ tile_start,
+__global__ __launch_bounds__(NT, (gemm::square_tile_wgs_per_cu<T>())) void k_bb_gemm(const Gemm* __restrict__ probs, const int* __restrict__ tile_start,
                                                 int nprob, int total, int* __restrict__ counter) {
   PGA_KERNEL_VGPR_FLOOR();
-  __shared__ __align__(16) uint32_t lds[gemm::TileLds<T, T, 16>::words];
+  __shared__ __align__(16) uint32_t lds[gemm::square_tile_lds_words<T>()];
   __shared__ int s_tile;
   for (;;) {
     if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1);

@@ -1495,12 +1495,13 @@ static hipError_t potrf_overlap(const Ctx& c, bool upper, int n, uint32_t* a, in
 // T divides nb, so each T x T tile is wholly one or the other.  One workgroup per tile of
 // the lower trapezoid, tiles numbered column by column.
 template <int T>
-__global__ __launch_bounds__(WG, 3) void k_potrf_trail(int m, int ncols, int kb, const uint32_t* __restrict__ L,
-                                                    int lda, uint32_t* __restrict__ C, int nb, int urgent,
-                                                    const int* stop) {
+__global__ __launch_bounds__(WG, (gemm::square_tile_wgs_per_cu<T>())) void k_potrf_trail(int m, int ncols, int kb,
+                                                                     const uint32_t* __restrict__ L, int lda,
+                                                                     uint32_t* __restrict__ C, int nb, int urgent,
+                                                                     const int* stop) {
   PGA_KERNEL_VGPR_FLOOR();
   if (urgent) PGA_URGENT();
-  __shared__ __align__(16) uint32_t lds[gemm::TileLds<T, T, 16>::words];
+  __shared__ __align__(16) uint32_t lds[gemm::square_tile_lds_words<T>()];
   if (stopped(stop)) return;
   const int tm = (m + T - 1) / T, tc = (ncols + T - 1) / T, t = blockIdx.x;
   // column q holds tiles q .. tm-1: it starts at q*tm - q*(q-1)/2

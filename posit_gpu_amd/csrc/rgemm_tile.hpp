@@ -86,6 +86,17 @@ struct TileLdsFor {
   static constexpr int words = tile_fp64_only<TM, TN>() ? 2 * BK * BM + 2 * BK * BN : TileLds<BM, BN, BK>::words;
 };
 
+// LDS of the kernels that cover a T x T region per workgroup with gemm_tile_in: T = 64 as
+// two 64 x 32 tiles, T = 32 and 16 as one T x T tile (T / 16 elements per lane each way).
+template <int T>
+constexpr int square_tile_lds_words() {
+  return T == 64 ? TileLdsFor<64, 32, 16, 4, 2>::words : TileLdsFor<T, T, 16, T / 16, T / 16>::words;
+}
+// Workgroups per CU such kernels leave room for: five with 64 x 32 tiles (28 KB of LDS
+// with the rounding table, at most 96 VGPRs), smaller tiles fit more anyway.
+template <int T>
+constexpr int square_tile_wgs_per_cu() { return T == 64 ? 5 : 3; }
+
 // The tile body with the integer decoded multiply-add (gemm::mac): used for one element
 // per lane and short K, where the FP64 form would wait on two LDS table reads per step
 // with nothing else to issue and fill its table for little work.
