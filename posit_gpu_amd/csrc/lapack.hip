@@ -548,6 +548,41 @@ __global__ __launch_bounds__(WG) void k_laswp(int n, uint32_t* a, int lda, int k
   }
 }
 
+// LASWP (Rlaswp.cpp:70-92, incx = 1) of one column per workgroup, for interchanges that
+// stay within the `rows` rows from row k1 (1-based) down, as an LU's do: the lanes stage
+// those rows in LDS, one lane applies the interchanges in order there, the lanes write
+// the rows back.  The same moves in the same order as k_laswp, without a global-memory
+// round trip per interchange on the one lane's path.
+__global__ __launch_bounds__(WG) void k_laswp_lds(uint32_t* a, int lda, int k1, int k2, const int* ipiv, int rows,
+                                                  const int* stop) {
+  PGA_KERNEL_VGPR_FLOOR();
+  PGA_URGENT();
+  extern __shared__ uint32_t seg[];  // rows k1 .. k1 + rows - 1 of this column
+  if (stopped(stop)) return;
+  uint32_t* col = a + (size_t)blockIdx.x * lda + (k1 - 1);
+  for (int r = threadIdx.x; r < rows; r += WG) seg[r] = col[r];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    constexpr int U = 16;  // the pivots of U interchanges read ahead
+    for (int i0 = k1; i0 <= k2; i0 += U) {
+      int ipc[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) ipc[u] = i0 + u <= k2 ? ipiv[i0 + u - 1] : i0 + u;
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int i = i0 + u, ip = ipc[u];
+        if (ip != i && ip >= k1 && ip - k1 < rows) {  // (an LU's pivots always are in range)
+          const uint32_t t = seg[i - k1];
+          seg[i - k1] = seg[ip - k1];
+          seg[ip - k1] = t;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int r = threadIdx.x; r < rows; r += WG) col[r] = seg[r];
+}
+
 // ---------------------------------------------------------------------------------
 // Rgetrf2 leaves (Rgetrf2.cpp:86-132); ipiv is written as a global 1-based row index
 // (row_off + local), info as a global column index (col_off + 1), first one only.
@@ -1057,9 +1092,21 @@ static hipError_t trsm(const Ctx& c, bool left, bool upper, bool trans, bool nou
   return hipGetLastError();
 }
 
+// Rows a column segment of k_laswp_lds may hold (64 KB of LDS)
+constexpr int LASWP_LDS_ROWS = 16384;
+// Columns up to which the interchanges of few columns go through LDS (beyond it the
+// one-lane-per-column kernel has parallelism enough)
+constexpr int LASWP_LDS_MAX_COLS = 1024;
+
+// `rows` (optional): the rows from row k1 to the end of the matrix, which bound every
+// interchange of an LU's ipiv (ip >= i).
 static hipError_t laswp(const Ctx& c, int n, uint32_t* a, int lda, int k1, int k2, const int* ipiv,
-                        int incx, int ipoff, const int* stop) {
+                        int incx, int ipoff, const int* stop, int rows = 0) {
   if (n <= 0 || incx == 0) return hipSuccess;
+  if (incx == 1 && ipoff == 0 && rows >= k2 - k1 + 1 && rows <= LASWP_LDS_ROWS && n <= LASWP_LDS_MAX_COLS) {
+    hipLaunchKernelGGL(k_laswp_lds, dim3(n), dim3(WG), (unsigned)rows * 4u, c.s, a, lda, k1, k2, ipiv, rows, stop);
+    return hipGetLastError();
+  }
   hipLaunchKernelGGL(k_laswp, dim3(blocks_for(n)), dim3(WG), 0, c.s, n, a, lda, k1, k2, ipiv, incx, ipoff,
                      stop);
   return hipGetLastError();
@@ -1252,7 +1299,7 @@ static hipError_t getrf_update(const Ctx& c, int m, uint32_t* a, int lda, const 
   if (w <= 0) return hipSuccess;
   uint32_t* ajj = a + j + (size_t)j * lda;
   uint32_t* a12 = a + j + (size_t)c0 * lda;
-  PGA_TRY(laswp(c, w, a + (size_t)c0 * lda, lda, j + 1, j + jb, ipiv, 1, 0, nullptr));
+  PGA_TRY(laswp(c, w, a + (size_t)c0 * lda, lda, j + 1, j + jb, ipiv, 1, 0, nullptr, m - j));
   PGA_TRY(trsm(c, true, false, false, false, jb, w, P_ONE, ajj, lda, a12, lda, nullptr));
   GemmOpts o;
   o.urgent = urgent;
