@@ -392,6 +392,68 @@ __device__ __forceinline__ void gemm_tile_impl_fp64(int m, int n, int k, uint32_
   const bool alpha_one = (alpha == P_ONE);
   const bool alpha_mone = (alpha == P_MONE);
 
+  // the slices' patterns: read from A and B (zero outside the matrices), then converted
+  // into LDS with alpha applied to the B side
+  // (measured: the 32 x 32 tile, one workgroup per CU on small GEMMs, gains; one element
+  // per lane, LDS-bound, does not; larger tiles hide the loads behind other workgroups)
+  constexpr bool PREFETCH = !PANEL && TM * TN == 4;
+  constexpr int QA = (BM * BK) / NT, QT = (BN * BK) / NT;
+  uint32_t ra[QA], rt[QT];
+  auto a_pos = [&](int q, int& ii, int& ll) {
+    const int e = tid + q * NT;
+    if (!TA) {  // A is m x k: walk rows fastest (coalesced)
+      ii = e % BM;
+      ll = e / BM;
+    } else {  // A is k x m: walk l fastest
+      ll = e % BK;
+      ii = e / BK;
+    }
+  };
+  auto t_pos = [&](int q, int& jj, int& ll) {
+    const int e = tid + q * NT;
+    if (!TB) {  // B is k x n: walk l fastest
+      ll = e % BK;
+      jj = e / BK;
+    } else {  // B is n x k: walk columns fastest
+      jj = e % BN;
+      ll = e / BN;
+    }
+  };
+  auto slice_load = [&](int kt0) {
+#pragma unroll
+    for (int q = 0; q < QA; ++q) {
+      int ii, ll;
+      a_pos(q, ii, ll);
+      const int i = i0 + ii, l = kt0 + ll;
+      ra[q] = (i < m && l < k) ? (TA ? A[l + (size_t)i * lda] : A[i + (size_t)l * lda]) : 0u;
+    }
+#pragma unroll
+    for (int q = 0; q < QT; ++q) {
+      int jj, ll;
+      t_pos(q, jj, ll);
+      const int j = j0 + jj, l = kt0 + ll;
+      rt[q] = (j < n && l < k) ? (TB ? B[j + (size_t)l * ldb] : B[l + (size_t)j * ldb]) : 0u;
+    }
+  };
+  auto slice_store = [&]() {
+#pragma unroll
+    for (int q = 0; q < QA; ++q) {
+      int ii, ll;
+      a_pos(q, ii, ll);
+      sA[ll][ii] = pga::to_double(ra[q]);  // exact; NaR -> NaN
+    }
+#pragma unroll
+    for (int q = 0; q < QT; ++q) {
+      int jj, ll;
+      t_pos(q, jj, ll);
+      uint32_t v = rt[q];
+      // alpha*b; with SKIP0 a zero b stays zero (it marks a skipped step even if alpha is NaR)
+      if (!SUM_FIRST && !(SKIP0 && v == 0))
+        v = alpha_one ? v : (alpha_mone ? pga::neg(v) : pga::mul(alpha, v));
+      sT[ll][jj] = pga::to_double(v);
+    }
+  };
+
   for (int kt = 0; kt < k; kt += BK) {
     if constexpr (PANEL) {
       // ---- copy the slices from the panels: 16-byte pieces, in the LDS order ----
@@ -411,43 +473,13 @@ __device__ __forceinline__ void gemm_tile_impl_fp64(int m, int n, int k, uint32_
       for (int q = 0; q < (BN * BK) / (2 * NT); ++q) lt[tid + q * NT] = vt[q];
       __syncthreads();
     } else {
-    // ---- stage the A slice (BM x BK) and the B-side slice (BK x BN) as FP64 values ----
-#pragma unroll
-    for (int q = 0; q < (BM * BK) / NT; ++q) {
-      int e = tid + q * NT;
-      int ii, ll;
-      if (!TA) {  // A is m x k: walk rows fastest (coalesced)
-        ii = e % BM;
-        ll = e / BM;
-      } else {  // A is k x m: walk l fastest
-        ll = e % BK;
-        ii = e / BK;
-      }
-      int i = i0 + ii, l = kt + ll;
-      uint32_t v = 0;
-      if (i < m && l < k) v = TA ? A[l + (size_t)i * lda] : A[i + (size_t)l * lda];
-      sA[ll][ii] = pga::to_double(v);  // exact; NaR -> NaN
-    }
-#pragma unroll
-    for (int q = 0; q < (BN * BK) / NT; ++q) {
-      int e = tid + q * NT;
-      int jj, ll;
-      if (!TB) {  // B is k x n: walk l fastest
-        ll = e % BK;
-        jj = e / BK;
-      } else {  // B is n x k: walk columns fastest
-        jj = e % BN;
-        ll = e / BN;
-      }
-      int j = j0 + jj, l = kt + ll;
-      uint32_t v = 0;
-      if (j < n && l < k) v = TB ? B[j + (size_t)l * ldb] : B[l + (size_t)j * ldb];
-      // alpha*b; with SKIP0 a zero b stays zero (it marks a skipped step even if alpha is NaR)
-      if (!SUM_FIRST && !(SKIP0 && v == 0))
-        v = alpha_one ? v : (alpha_mone ? pga::neg(v) : pga::mul(alpha, v));
-      sT[ll][jj] = pga::to_double(v);
-    }
-    __syncthreads();
+      // ---- stage the A slice (BM x BK) and the B-side slice (BK x BN) as FP64 values ----
+      // With PREFETCH the next slice's patterns are read before this one is computed, so
+      // that its loads are in flight meanwhile.
+      if (!PREFETCH || kt == 0) slice_load(kt);
+      slice_store();
+      __syncthreads();
+      if (PREFETCH && kt + BK < k) slice_load(kt + BK);
     }
 
     const int lmax = min(BK, k - kt);
