@@ -461,6 +461,75 @@ __global__ __launch_bounds__(WG) void k_trsm_rltn_fp64(int m, int n, const uint3
   for (int e = tid; e < R * n; e += WG) b[e % R + (size_t)(e / R) * ldb] = from_double(sB[e / R][e % R]);
 }
 
+// B := U^-T B with U non-unit upper m x m, alpha = 1, at the pattern level on C columns
+// of B, one after another (Rtrsm.cpp:161-173; the sweep of k_trsm_left<true, true>).
+static __device__ void lutn_cols_pattern(int m, int C, const uint32_t* __restrict__ A, int lda,
+                                         uint32_t* __restrict__ b, int ldb) {
+  const int tid = threadIdx.x;
+  for (int c = 0; c < C; ++c) {
+    uint32_t* bc = b + (size_t)c * ldb;
+    for (int k = 0; k < m; ++k) {
+      const uint32_t bk = div(bc[k], A[k + (size_t)k * lda]);
+      for (int i = k + 1 + tid; i < m; i += WG) bc[i] = sub(bc[i], mul(A[k + (size_t)i * lda], bk));
+      __syncthreads();
+      if (tid == 0) bc[k] = bk;
+      __syncthreads();
+    }
+  }
+}
+
+// The upper Cholesky's row-block solve, B := U^-T B with U non-unit upper m x m, alpha =
+// 1 (Rtrsm.cpp:161-173: temp = b_i - sum over k < i of u_ki * b_k in ascending k, then
+// temp / u_ii), on LUTN_COLS columns of B per workgroup with B's columns and U held in
+// LDS as FP64 values: step k divides b_k by u_kk (fp64_posit_div), then subtracts
+// u_ki * b_k from every later row i (fp64_mac_arith: the GEMM's roundings with computed
+// constants; this form skips no zero term).  A scale outside -108 .. 107, a NaR or a zero
+// pivot leaves the block untouched in memory and the columns are solved again with the
+// pattern-level sweep.
+constexpr int LUTN_COLS = 16, LUTN_MMAX = 64;
+
+__global__ __launch_bounds__(WG) void k_trsm_lutn_fp64(int m, int n, const uint32_t* __restrict__ A, int lda,
+                                                       uint32_t* __restrict__ B, int ldb, const int* stop) {
+  PGA_KERNEL_VGPR_FLOOR();
+  PGA_URGENT();
+  // U's strictly upper triangle, packed by columns: u_ki (k < i) at i * (i - 1) / 2 + k
+  __shared__ double sU[LUTN_MMAX * (LUTN_MMAX - 1) / 2];
+  __shared__ double sB[LUTN_MMAX][LUTN_COLS];  // sB[row][column]
+  __shared__ double sD[LUTN_MMAX];
+  const int c0 = blockIdx.x * LUTN_COLS;
+  if (c0 >= n || stopped(stop)) return;
+  const int C = min(LUTN_COLS, n - c0);
+  const int tid = threadIdx.x;
+  uint32_t* b = B + (size_t)c0 * ldb;
+  for (int e = tid; e < m * m; e += WG) {
+    const int k = e % m, i = e / m;
+    if (k < i) sU[i * (i - 1) / 2 + k] = to_double(A[k + (size_t)i * lda]);
+  }
+  for (int e = tid; e < m * C; e += WG) sB[e % m][e / m] = to_double(b[e % m + (size_t)(e / m) * ldb]);
+  if (tid < m) {
+    const double d = to_double(A[tid + (size_t)tid * lda]);
+    sD[tid] = (d != 0.0) ? d : __builtin_nan("");  // a zero pivot: redo (NaR is NaN already)
+  }
+  __syncthreads();
+  for (int k = 0; k < m; ++k) {
+    if (tid < C) sB[k][tid] = fp64_posit_div(sB[k][tid], sD[k]);
+    __syncthreads();
+    const int cnt = m - 1 - k;
+    for (int e = tid; e < cnt * C; e += WG) {
+      const int c = e % C, i = k + 1 + e / C;
+      sB[i][c] = fp64_mac_arith(sB[i][c], -sU[i * (i - 1) / 2 + k], sB[k][c]);
+    }
+    __syncthreads();
+  }
+  bool bad = false;
+  for (int e = tid; e < m * C; e += WG) bad |= sB[e % m][e / m] != sB[e % m][e / m];  // NaN
+  if (__syncthreads_or(bad)) {
+    lutn_cols_pattern(m, C, A, lda, b, ldb);
+    return;
+  }
+  for (int e = tid; e < m * C; e += WG) b[e % m + (size_t)(e / m) * ldb] = from_double(sB[e % m][e / m]);
+}
+
 // B = 0 (alpha == 0, Rtrsm.cpp:101-108)
 __global__ __launch_bounds__(WG) void k_zero(int m, int n, uint32_t* B, int ldb, const int* stop) {
   PGA_KERNEL_VGPR_FLOOR();
@@ -815,6 +884,69 @@ static __device__ int potrf2_lower_decoded(int n, uint32_t* a, int lda) {
   return 0;
 }
 
+// The upper Rpotrf2 recursion (Rpotrf2.cpp:102-125) on an n x n block (n <= 64) held in
+// LDS as FP64 values, S[column][row]: leaf j takes the root of its diagonal element
+// (fp64_posit_sqrt); the one node whose left half ends at j then solves its A12 (Rtrsm
+// 'L','U','T','N', Rtrsm.cpp:161-173: ascending k, then the quotient by u_ii,
+// fp64_posit_div) and updates the upper triangle of its A22 (Rsyrk 'U','T',
+// Rsyrk.cpp:182-195: t = sum over the left half's rows l of a_li * a_lj, formed from zero
+// in ascending l, then c - t).  Products and sums are rounded as in the GEMM
+// (fp64_mac_arith).  Returns 0 with the block written back, or -1 with the block untouched
+// where a value left scales -108 .. 107, a NaR appeared or a pivot is not positive: the
+// pattern-level recursion then decides (and sets info).
+template <int T>
+static __device__ int potrf2_upper_decoded(int n, uint32_t* a, int lda,
+                                           double (*S)[POTRF2_ONE_WG_MAX + 1]) {
+  const int tid = threadIdx.x;
+  for (int e = tid; e < n * n; e += T) {
+    const int r = e % n, c = e / n;
+    if (r <= c) S[c][r] = to_double(a[r + (size_t)c * lda]);
+  }
+  __syncthreads();
+  for (int j = 0; j < n; ++j) {
+    const double v = S[j][j];
+    if (!(v > 0.0)) return -1;  // uniform (zero, negative or NaR)
+    __syncthreads();
+    if (tid == 0) S[j][j] = fp64_posit_sqrt(v);
+    __syncthreads();
+    int s, mid, e;
+    if (wg::node_after(j, n, s, mid, e)) {
+      const int n2 = e - mid;
+      for (int k = s; k < mid; ++k) {  // A12 := U11^-T A12, rows s .. mid-1
+        const double d = S[k][k];
+        for (int c = mid + tid; c < e; c += T) S[c][k] = fp64_posit_div(S[c][k], d);
+        __syncthreads();
+        const int cnt = mid - 1 - k;
+        for (int q = tid; q < cnt * n2; q += T) {
+          const int i = k + 1 + q % cnt, c = mid + q / cnt;
+          S[c][i] = fp64_mac_arith(S[c][i], -S[i][k], S[c][k]);
+        }
+        __syncthreads();
+      }
+      for (int q = tid; q < n2 * n2; q += T) {  // A22 -= A12^T A12, upper triangle
+        const int i = mid + q % n2, jj = mid + q / n2;
+        if (i <= jj) {
+          double t = 0.0;
+          for (int l = s; l < mid; ++l) t = fp64_mac_arith(t, S[i][l], S[jj][l]);
+          S[jj][i] = fp64_mac_arith(S[jj][i], -1.0, t);
+        }
+      }
+      __syncthreads();
+    }
+  }
+  bool bad = false;
+  for (int e = tid; e < n * n; e += T) {
+    const int r = e % n, c = e / n;
+    if (r <= c) bad |= S[c][r] != S[c][r];  // NaN: a scale left the range, or NaR
+  }
+  if (__syncthreads_or(bad)) return -1;
+  for (int e = tid; e < n * n; e += T) {
+    const int r = e % n, c = e / n;
+    if (r <= c) a[r + (size_t)c * lda] = from_double(S[c][r]);
+  }
+  return 0;
+}
+
 template <bool UPPER>
 __global__ __launch_bounds__(WG, 2) void k_potrf2_wg(int n, uint32_t* a, int lda, int* info, int col_off) {
   PGA_KERNEL_VGPR_FLOOR();
@@ -822,10 +954,14 @@ __global__ __launch_bounds__(WG, 2) void k_potrf2_wg(int n, uint32_t* a, int lda
   __shared__ wg::Shared sh;
   __shared__ uint32_t blk[POTRF2_ONE_WG_MAX * POTRF2_ONE_WG_MAX];
   if (*info) return;  // every lane reads the same flag
-  if (!UPPER) {
+  if constexpr (!UPPER) {
     int f = potrf2_lower_decoded<WG>(n, a, lda);
     if (f < 0) f = potrf2_lower_regs<WG>(n, a, lda);
     if (f == 0) return;
+  } else {
+    __shared__ double S[POTRF2_ONE_WG_MAX][POTRF2_ONE_WG_MAX + 1];
+    if (potrf2_upper_decoded<WG>(n, a, lda, S) == 0) return;
+    __syncthreads();
   }
   for (int e = threadIdx.x; e < n * n; e += WG) blk[e] = a[e % n + (size_t)(e / n) * lda];
   __syncthreads();
@@ -1057,6 +1193,9 @@ static hipError_t trsm(const Ctx& c, bool left, bool upper, bool trans, bool nou
                      lda, B, ldb, stop)
     if (upper && !trans) PGA_TL(true, false);
     else if (!upper && !trans) PGA_TL(false, false);
+    else if (upper && trans && nounit && alpha == P_ONE && m <= LUTN_MMAX)
+      hipLaunchKernelGGL(k_trsm_lutn_fp64, dim3((unsigned)((n + LUTN_COLS - 1) / LUTN_COLS)), dim3(WG), 0, c.s, m, n,
+                         A, lda, B, ldb, stop);
     else if (upper && trans) PGA_TL(true, true);
     else if (m <= LLTN_PIPE_CAP + 1)
       hipLaunchKernelGGL(k_trsm_lltn_pipe, dim3(n), dim3(WG), 0, c.s, m, n, alpha, (int)nounit, A, lda, B, ldb,

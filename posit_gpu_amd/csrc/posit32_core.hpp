@@ -418,6 +418,20 @@ PGA_HD double fp64_posit_recip(double a) {
   return r;
 }
 
+// b / a on the posit grid for posit values held in FP64, as fp64_posit_recip forms 1 / a:
+// the FP64 quotient (correctly rounded) rounded to the grid of its binade, which is the
+// posit quotient unless it lands exactly on a posit midpoint (only there can the first
+// rounding have decided the second); there the pattern-level quotient decides.  NaN for a
+// zero or NaR divisor, a NaR dividend, and quotients outside scales -108 .. 107.
+PGA_HD double fp64_posit_div(double b, double a) {
+  const double q = b / a;
+  const double c = fp64_grid_c_arith(q);
+  const double r = (q + c) - c;
+  const double half = (c / 1.5) * 0x1p-53;
+  if (__builtin_fabs(q - r) == half) return to_double(div(from_double(b), from_double(a)));
+  return r;
+}
+
 // sqrt(a) on the posit grid for a positive posit value a held in FP64: the FP64 root
 // (within an ulp of the true one) rounded to the grid of its binade.  That gives the posit
 // root unless the true root lies within a few FP64 ulps of a posit midpoint; there the

@@ -345,7 +345,46 @@ int main(int argc, char** argv) {
     check_sqrt(sq + 1u);
     check_sqrt(sq - 1u);
   }
-  printf("arithmetic constants checked %ld, reciprocals checked %ld (NaN %ld), roots checked %ld\n", arith_checked,
-         recip_checked, recip_nan, sqrt_checked);
+  // the quotients of the upper Cholesky's solves (pga::fp64_posit_div) against the
+  // oracle's division: random pairs, short fractions, and dividends over powers of two
+  // (exact quotients whose dropped bits are often exactly half a quantum: ties, which the
+  // FP64 rounding settles to the even value by itself).  The pattern-level fallback is for
+  // an inexact quotient whose FP64 rounding lands exactly on a midpoint; posit midpoints
+  // are FP64 values and rounding is monotonic, so any other FP64 quotient lies on the same
+  // side of every midpoint as the exact one.  Such quotients are too rare to sample here.
+  long div_checked = 0, div_nan = 0, div_ties = 0;
+  auto check_div = [&](uint32_t b, uint32_t a) {
+    if (a == 0 || a == pga::P_NAR || b == pga::P_NAR) return;
+    const double r = pga::fp64_posit_div(pga::to_double(b), pga::to_double(a));
+    const uint32_t want = orc_p32_div(b, a);
+    ++div_checked;
+    if (r != r) {
+      ++div_nan;
+      const double w = std::fabs(pga::to_double(want));
+      if (w >= 0x1p-108 && w < 0x1p108) {
+        ++bad;
+        if (bad < 8) printf("%08x / %08x: NaN inside the range\n", b, a);
+      }
+      return;
+    }
+    if (pga::from_double(r) != want) {
+      ++bad;
+      if (bad < 8) printf("%08x / %08x: got %08x want %08x\n", b, a, pga::from_double(r), want);
+    }
+  };
+  for (long i = 0; i < 4 * n; ++i) check_div((uint32_t)splitmix(s), (uint32_t)splitmix(s));
+  for (long i = 0; i < 4 * n; ++i)
+    check_div((uint32_t)(splitmix(s) >> 32) & 0xFFFF0000u, (uint32_t)(splitmix(s) >> 32) & 0xFFF00000u);
+  for (long i = 0; i < 4 * n; ++i) {
+    const uint32_t b = (uint32_t)splitmix(s);
+    const int e = (int)(splitmix(s) % 81) - 40;
+    const uint32_t a = pga::from_double(std::ldexp((splitmix(s) & 1) ? -1.0 : 1.0, e));
+    const double q = pga::to_double(b) / pga::to_double(a);  // exact: a is a power of two
+    if (q == q && pga::to_double(orc_p32_div(b, a)) != q) ++div_ties;  // the posit quotient rounded
+    check_div(b, a);
+  }
+  printf("arithmetic constants checked %ld, reciprocals checked %ld (NaN %ld), roots checked %ld, quotients checked "
+         "%ld (NaN %ld, rounded exact quotients %ld)\n",
+         arith_checked, recip_checked, recip_nan, sqrt_checked, div_checked, div_nan, div_ties);
   return bad ? 1 : 0;
 }

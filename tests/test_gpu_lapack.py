@@ -323,11 +323,13 @@ def test_potrf_not_pd(pga, oracle, uplo):
         _same(got, want)  # the factorization stops at the same place
 
 
+@pytest.mark.parametrize("uplo", ["L", "U"])
 @pytest.mark.parametrize("kind", ["huge", "tiny", "sparse", "nar"])
-def test_potrf_lower_diagonal_block_ranges(pga, oracle, kind):
-    """The lower diagonal blocks run decoded while every product and sum stays in the
-    fast scale range and fall back to the pattern-level form otherwise: scaled far up or
-    down, with exact zeros (skipped multipliers), or with a NaR (a failing block)."""
+def test_potrf_diagonal_block_ranges(pga, oracle, kind, uplo):
+    """The diagonal blocks (and the upper factor's row-block solves) run in FP64 while
+    every product, quotient and sum stays in the fast scale range and fall back to the
+    pattern-level form otherwise: scaled far up or down, with exact zeros (skipped
+    multipliers in the lower form), or with a NaR (a failing block)."""
     n = 150
     if kind == "sparse":
         rng = np.random.default_rng(57)
@@ -340,12 +342,12 @@ def test_potrf_lower_diagonal_block_ranges(pga, oracle, kind):
         S *= {"huge": 2.0 ** 112, "tiny": 2.0 ** -116, "nar": 1.0}[kind]
     A = np.asfortranarray(oracle.from_double_v(S.ravel(order="F")).reshape(n, n, order="F"))
     if kind == "nar":
-        A[100, 70] = 0x80000000
-    for nb in (0, 16):
+        A[100, 70] = A[70, 100] = 0x80000000
+    for nb in (0, 16, 64):
         want = A.copy(order="F")
-        _, winfo = oracle.rpotrf("L", want, nb=nb, nthreads=THREADS)
+        _, winfo = oracle.rpotrf(uplo, want, nb=nb, nthreads=THREADS)
         got = A.copy(order="F")
-        info = pga.Rpotrf_opencl("L", n, got, n, nb)
+        info = pga.Rpotrf_opencl(uplo, n, got, n, nb)
         assert info == winfo and (info > 0) == (kind == "nar")
         _same(got, want)
 
