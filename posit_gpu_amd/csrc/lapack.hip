@@ -737,11 +737,11 @@ __global__ void k_potrf2_leaf(uint32_t* a, int* info, int col_off) {
 }
 
 // ---------------------------------------------------------------------------------
-// Rpotrf2 on a whole diagonal block by one workgroup (wg_factor.hpp: the same recursion
-// and operation order as the launches of lah::potrf2, in one launch).  info doubles as
-// the stop flag, as for every step of lah::potrf.
+// Rpotrf2 on a whole diagonal block (n <= 64) by one workgroup, lower and upper: the
+// recursion's operations in its order (as the launches of lah::potrf2 give them), in the
+// FP64 form first and at the pattern level where that gives up.  info doubles as the stop
+// flag, as for every step of lah::potrf.
 // ---------------------------------------------------------------------------------
-// The block is staged in LDS: each recursion step then waits on LDS, not on L2.
 constexpr int POTRF2_ONE_WG_MAX = 64;
 
 // Lower Rpotrf2 of an n x n block (n <= 64) with the lower triangle held in registers,
@@ -755,73 +755,15 @@ constexpr int POTRF2_ONE_WG_MAX = 64;
 // Rsyrk calls alone, then the test and the sqrt of the leaf (Rpotrf2.cpp:90-97).  An
 // Rsyrk term c + (-m)*x and an Rtrsm term c - m*x round alike (posit rounding is
 // symmetric in sign), so the right-looking order below -- column j final, then its terms
-// applied to every later column -- computes the same bits.  The state the recursion
-// leaves after a failing leaf is not this one: on a failure the caller redoes the block
-// from its unchanged input with the recursion itself.  Returns 0 or the failing column
-// (1-based); the block in memory is written only on success.
-template <int T>
-static __device__ int potrf2_lower_regs(int n, uint32_t* a, int lda) {
-  constexpr int SLOTS = (POTRF2_ONE_WG_MAX * (POTRF2_ONE_WG_MAX + 1) / 2 + T - 1) / T;
-  __shared__ uint32_t colv[POTRF2_ONE_WG_MAX];
-  __shared__ uint32_t dg;
-  uint32_t c[SLOTS];
-  int ri[SLOTS], ci[SLOTS];
-  {
-    int j = 0, off = 0;  // off: packed index of column j's diagonal element
-#pragma unroll
-    for (int q = 0; q < SLOTS; ++q) {
-      const int e = threadIdx.x + q * T;
-      while (j < n && off + (n - j) <= e) {
-        off += n - j;
-        ++j;
-      }
-      ri[q] = j + (e - off);
-      ci[q] = j < n ? j : -1;  // -1: no element (matches no step)
-      c[q] = j < n ? a[ri[q] + (size_t)j * lda] : 0u;
-    }
-  }
-  if (threadIdx.x == 0) dg = a[0];
-  __syncthreads();
-  for (int j = 0; j < n; ++j) {
-    const uint32_t v = dg;  // every lane reads the same value
-    if (le(v, 0)) return j + 1;
-    const uint32_t d = pga::sqrt(v);
-    const uint32_t r = j + 1 < n ? div(P_ONE, d) : 0u;
-#pragma unroll
-    for (int q = 0; q < SLOTS; ++q) {
-      if (ci[q] == j) {
-        if (ri[q] == j) {
-          c[q] = d;
-        } else {
-          c[q] = mul(r, c[q]);
-          colv[ri[q]] = c[q];
-        }
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < SLOTS; ++q) {
-      if (ci[q] > j) {
-        const uint32_t m = colv[ci[q]];
-        if (m != 0) c[q] = sub(c[q], mul(m, colv[ri[q]]));
-        if (ri[q] == j + 1 && ci[q] == j + 1) dg = c[q];
-      }
-    }
-    __syncthreads();
-  }
-#pragma unroll
-  for (int q = 0; q < SLOTS; ++q)
-    if (ci[q] >= 0) a[ri[q] + (size_t)ci[q] * lda] = c[q];
-  return 0;
-}
-
-// potrf2_lower_regs computed with the GEMM's FP64 multiply-add (gemm::fp64_mac): the same
-// terms in the same order, each rounded as the pattern-level operations round it as long
-// as every product and sum stays in the table's scale range.  A result outside it (NaN),
-// a NaR, or a failing diagonal (whose value may then be meaningless) makes the call return
-// -1 with the block unchanged, and the caller redoes it with the pattern-level form.  The
-// root and the reciprocal come from FP64 rounded to the posit grid (fp64_posit_sqrt,
-// fp64_posit_recip: the pattern-level operation decides where that could round wrongly).
+// applied to every later column -- computes the same bits.  The terms run in the GEMM's
+// FP64 multiply-add form with computed constants (fp64_mac_arith), each rounded as the
+// pattern-level operations round it as long as every product and sum stays in scales
+// -108 .. 107; the root and the reciprocal come from FP64 rounded to the posit grid
+// (fp64_posit_sqrt, fp64_posit_recip: the pattern-level operation decides where that could
+// round wrongly).  A result outside that range (NaN), a NaR, or a failing diagonal (the
+// state the recursion leaves after a failing leaf is not this one) makes the call return
+// -1 with the block unchanged; the caller then walks the recursion itself.  Returns 0
+// with the block written back otherwise.
 template <int T>
 static __device__ int potrf2_lower_decoded(int n, uint32_t* a, int lda) {
   constexpr int SLOTS = (POTRF2_ONE_WG_MAX * (POTRF2_ONE_WG_MAX + 1) / 2 + T - 1) / T;
@@ -885,41 +827,43 @@ static __device__ int potrf2_lower_decoded(int n, uint32_t* a, int lda) {
 }
 
 // The upper Rpotrf2 recursion (Rpotrf2.cpp:102-125) on an n x n block (n <= 64) held in
-// LDS as FP64 values, S[column][row]: leaf j takes the root of its diagonal element
-// (fp64_posit_sqrt); the one node whose left half ends at j then solves its A12 (Rtrsm
-// 'L','U','T','N', Rtrsm.cpp:161-173: ascending k, then the quotient by u_ii,
-// fp64_posit_div) and updates the upper triangle of its A22 (Rsyrk 'U','T',
-// Rsyrk.cpp:182-195: t = sum over the left half's rows l of a_li * a_lj, formed from zero
-// in ascending l, then c - t).  Products and sums are rounded as in the GEMM
-// (fp64_mac_arith).  Returns 0 with the block written back, or -1 with the block untouched
-// where a value left scales -108 .. 107, a NaR appeared or a pivot is not positive: the
-// pattern-level recursion then decides (and sets info).
+// LDS as FP64 values, its upper triangle packed by columns (element (r, c), r <= c, at
+// c * (c + 1) / 2 + r): leaf j takes the root of its diagonal element (fp64_posit_sqrt);
+// the one node whose left half ends at j then solves its A12 (Rtrsm 'L','U','T','N',
+// Rtrsm.cpp:161-173: ascending k, then the quotient by u_ii, fp64_posit_div) and updates
+// the upper triangle of its A22 (Rsyrk 'U','T', Rsyrk.cpp:182-195: t = sum over the left
+// half's rows l of a_li * a_lj, formed from zero in ascending l, then c - t).  Products and
+// sums are rounded as in the GEMM (fp64_mac_arith).  Returns 0 with the block written
+// back, or -1 with the block untouched where a value left scales -108 .. 107, a NaR
+// appeared or a pivot is not positive: the pattern-level recursion then decides.
+constexpr int POTRF2_UPPER_PACKED = POTRF2_ONE_WG_MAX * (POTRF2_ONE_WG_MAX + 1) / 2;
+
 template <int T>
-static __device__ int potrf2_upper_decoded(int n, uint32_t* a, int lda,
-                                           double (*S)[POTRF2_ONE_WG_MAX + 1]) {
+static __device__ int potrf2_upper_decoded(int n, uint32_t* a, int lda, double* S) {
   const int tid = threadIdx.x;
+  auto at = [&](int r, int c) -> double& { return S[c * (c + 1) / 2 + r]; };
   for (int e = tid; e < n * n; e += T) {
     const int r = e % n, c = e / n;
-    if (r <= c) S[c][r] = to_double(a[r + (size_t)c * lda]);
+    if (r <= c) at(r, c) = to_double(a[r + (size_t)c * lda]);
   }
   __syncthreads();
   for (int j = 0; j < n; ++j) {
-    const double v = S[j][j];
+    const double v = at(j, j);
     if (!(v > 0.0)) return -1;  // uniform (zero, negative or NaR)
     __syncthreads();
-    if (tid == 0) S[j][j] = fp64_posit_sqrt(v);
+    if (tid == 0) at(j, j) = fp64_posit_sqrt(v);
     __syncthreads();
     int s, mid, e;
     if (wg::node_after(j, n, s, mid, e)) {
       const int n2 = e - mid;
       for (int k = s; k < mid; ++k) {  // A12 := U11^-T A12, rows s .. mid-1
-        const double d = S[k][k];
-        for (int c = mid + tid; c < e; c += T) S[c][k] = fp64_posit_div(S[c][k], d);
+        const double d = at(k, k);
+        for (int c = mid + tid; c < e; c += T) at(k, c) = fp64_posit_div(at(k, c), d);
         __syncthreads();
         const int cnt = mid - 1 - k;
         for (int q = tid; q < cnt * n2; q += T) {
           const int i = k + 1 + q % cnt, c = mid + q / cnt;
-          S[c][i] = fp64_mac_arith(S[c][i], -S[i][k], S[c][k]);
+          at(i, c) = fp64_mac_arith(at(i, c), -at(k, i), at(k, c));
         }
         __syncthreads();
       }
@@ -927,8 +871,8 @@ static __device__ int potrf2_upper_decoded(int n, uint32_t* a, int lda,
         const int i = mid + q % n2, jj = mid + q / n2;
         if (i <= jj) {
           double t = 0.0;
-          for (int l = s; l < mid; ++l) t = fp64_mac_arith(t, S[i][l], S[jj][l]);
-          S[jj][i] = fp64_mac_arith(S[jj][i], -1.0, t);
+          for (int l = s; l < mid; ++l) t = fp64_mac_arith(t, at(l, i), at(l, jj));
+          at(i, jj) = fp64_mac_arith(at(i, jj), -1.0, t);
         }
       }
       __syncthreads();
@@ -937,39 +881,71 @@ static __device__ int potrf2_upper_decoded(int n, uint32_t* a, int lda,
   bool bad = false;
   for (int e = tid; e < n * n; e += T) {
     const int r = e % n, c = e / n;
-    if (r <= c) bad |= S[c][r] != S[c][r];  // NaN: a scale left the range, or NaR
+    if (r <= c) bad |= at(r, c) != at(r, c);  // NaN: a scale left the range, or NaR
   }
   if (__syncthreads_or(bad)) return -1;
   for (int e = tid; e < n * n; e += T) {
     const int r = e % n, c = e / n;
-    if (r <= c) a[r + (size_t)c * lda] = from_double(S[c][r]);
+    if (r <= c) a[r + (size_t)c * lda] = from_double(at(r, c));
   }
   return 0;
 }
 
-template <bool UPPER>
-__global__ __launch_bounds__(WG, 2) void k_potrf2_wg(int n, uint32_t* a, int lda, int* info, int col_off) {
+// The same recursion on the block in memory with the pattern-level operations, one
+// element per lane: neither LDS nor many registers.  A failing step returns its 1-based
+// column and leaves the block as the reference leaves it.
+template <int T>
+static __device__ int potrf2_upper_plain(int n, uint32_t* a, int lda) {
+  const int tid = threadIdx.x;
+  auto A = [&](int r, int c) -> uint32_t& { return a[r + (size_t)c * lda]; };
+  for (int j = 0; j < n; ++j) {
+    const uint32_t v = A(j, j);
+    if (le(v, 0)) return j + 1;  // Rpotrf2.cpp:90 (NaR is below zero in pattern order)
+    __syncthreads();
+    if (tid == 0) A(j, j) = pga::sqrt(v);
+    __syncthreads();
+    int s, mid, e;
+    if (wg::node_after(j, n, s, mid, e)) {
+      const int n2 = e - mid;
+      for (int k = s; k < mid; ++k) {
+        for (int c = mid + tid; c < e; c += T) A(k, c) = div(A(k, c), A(k, k));
+        __syncthreads();
+        const int cnt = mid - 1 - k;
+        for (int q = tid; q < cnt * n2; q += T) {
+          const int i = k + 1 + q % cnt, c = mid + q / cnt;
+          A(i, c) = sub(A(i, c), mul(A(k, i), A(k, c)));
+        }
+        __syncthreads();
+      }
+      for (int q = tid; q < n2 * n2; q += T) {
+        const int i = mid + q % n2, jj = mid + q / n2;
+        if (i <= jj) {
+          uint32_t t = 0;
+          for (int l = s; l < mid; ++l) t = add(t, mul(A(l, i), A(l, jj)));
+          A(i, jj) = add(mul(P_MONE, t), mul(P_ONE, A(i, jj)));
+        }
+      }
+      __syncthreads();
+    }
+  }
+  return 0;
+}
+
+// The upper diagonal block on the upper Cholesky's critical path: the FP64 form
+// (potrf2_upper_decoded, 17 KB of LDS), and where that gives up the recursion in memory
+// (potrf2_upper_plain), so that the block finds room beside the concurrent trailing
+// update.
+__global__ __launch_bounds__(WG, 2) void k_potrf2_upper_fast(int n, uint32_t* a, int lda, int* info, int col_off) {
   PGA_KERNEL_VGPR_FLOOR();
   PGA_URGENT();
-  __shared__ wg::Shared sh;
-  __shared__ uint32_t blk[POTRF2_ONE_WG_MAX * POTRF2_ONE_WG_MAX];
+  __shared__ double S[POTRF2_UPPER_PACKED];
   if (*info) return;  // every lane reads the same flag
-  if constexpr (!UPPER) {
-    int f = potrf2_lower_decoded<WG>(n, a, lda);
-    if (f < 0) f = potrf2_lower_regs<WG>(n, a, lda);
-    if (f == 0) return;
-  } else {
-    __shared__ double S[POTRF2_ONE_WG_MAX][POTRF2_ONE_WG_MAX + 1];
-    if (potrf2_upper_decoded<WG>(n, a, lda, S) == 0) return;
-    __syncthreads();
-  }
-  for (int e = threadIdx.x; e < n * n; e += WG) blk[e] = a[e % n + (size_t)(e / n) * lda];
+  if (potrf2_upper_decoded<WG>(n, a, lda, S) == 0) return;
   __syncthreads();
-  const int f = wg::potrf2<UPPER>(n, blk, n, sh);
-  __syncthreads();
-  for (int e = threadIdx.x; e < n * n; e += WG) a[e % n + (size_t)(e / n) * lda] = blk[e];
+  const int f = potrf2_upper_plain<WG>(n, a, lda);
   if (f && threadIdx.x == 0) *info = col_off + f;
 }
+
 
 // The lower Rpotrf2 recursion (Rpotrf2.cpp:84-145) on a block in memory, as
 // wg::potrf2<false> walks it, with its products one element per lane (wg::gemm_direct):
@@ -1578,7 +1554,7 @@ static hipError_t syrk(const Ctx& c, bool upper, bool trans, int n, int k, uint3
 static hipError_t potrf2(const Ctx& c, bool upper, int n, uint32_t* a, int lda, int* info, int col_off) {
   if (n == 0) return hipSuccess;
   if (n <= POTRF2_ONE_WG_MAX) {
-    if (upper) hipLaunchKernelGGL(k_potrf2_wg<true>, dim3(1), dim3(WG), 0, c.s, n, a, lda, info, col_off);
+    if (upper) hipLaunchKernelGGL(k_potrf2_upper_fast, dim3(1), dim3(WG), 0, c.s, n, a, lda, info, col_off);
     else hipLaunchKernelGGL(k_potrf2_lower_fast, dim3(1), dim3(WG), 0, c.s, n, a, lda, info, col_off);
     return hipGetLastError();
   }
@@ -1670,21 +1646,21 @@ static hipError_t potrf_overlap(const Ctx& c, bool upper, int n, uint32_t* a, in
   return e;
 }
 
-// The lower Cholesky's trailing update by one finished block of kb columns, L (rows
-// from the region's origin down):  C(i, c) -= L(i, l) * L(c, l) for l = 0 .. kb-1 in
-// order, on the lower trapezoid i >= c of an m x ncols region whose origin lies on the
-// nb-block grid.  An element whose row and column fall in one nb x nb diagonal block
-// receives the Rsyrk 'L','N' terms (zero multipliers L(c, l) skipped, Rsyrk.cpp:149-169),
-// any other element the RgemmG 'N','T' terms (no skip, Rpotrf_opencl.cpp:146) -- the same
-// per-element steps as the left-looking order's Rsyrk on the diagonal block and RgemmG on
-// the panel below it.
+// A Cholesky trailing update by one finished block of kb columns, L (rows from the
+// region's origin down): C(i, c) = beta*C(i, c) + sum of (alpha*L(c, l)) * L(i, l) for
+// l = 0 .. kb-1 in order, on the lower trapezoid i >= c of an m x ncols region whose
+// origin lies on the nb-block grid.  The lower factor (alpha = -1, beta = 1, SKIP): an
+// element whose row and column fall in one nb x nb diagonal block receives the Rsyrk
+// 'L','N' terms (zero multipliers L(c, l) skipped, Rsyrk.cpp:149-169), any other element
+// the RgemmG 'N','T' terms (no skip, Rpotrf_opencl.cpp:146) -- the same per-element steps
+// as the left-looking order's Rsyrk on the diagonal block and RgemmG on the panel below
+// it.  The upper factor's running sums (alpha = 1, no SKIP) take every term.
 // T divides nb, so each T x T tile is wholly one or the other.  One workgroup per tile of
 // the lower trapezoid, tiles numbered column by column.
-template <int T>
-__global__ __launch_bounds__(WG, (gemm::square_tile_wgs_per_cu<T>())) void k_potrf_trail(int m, int ncols, int kb,
-                                                                     const uint32_t* __restrict__ L, int lda,
-                                                                     uint32_t* __restrict__ C, int nb, int urgent,
-                                                                     const int* stop) {
+template <int T, bool SKIP>
+__global__ __launch_bounds__(WG, (gemm::square_tile_wgs_per_cu<T>())) void k_potrf_trail(
+    int m, int ncols, int kb, const uint32_t* __restrict__ L, int lda, uint32_t* __restrict__ C, int ldc,
+    uint32_t alpha, uint32_t beta, int nb, int urgent, const int* stop) {
   PGA_KERNEL_VGPR_FLOOR();
   if (urgent) PGA_URGENT();
   __shared__ __align__(16) uint32_t lds[gemm::square_tile_lds_words<T>()];
@@ -1700,31 +1676,33 @@ __global__ __launch_bounds__(WG, (gemm::square_tile_wgs_per_cu<T>())) void k_pot
   const int tj = lo, ti = tj + (t - (tj * tm - tj * (tj - 1) / 2));
   const int i0 = ti * T, j0 = tj * T;
   if constexpr (T == 64) {
-    // a 64 x 64 tile as two 64 x 32 halves (4 x 2 per lane: three waves per SIMD)
+    // a 64 x 64 tile as two 64 x 32 halves (4 x 2 per lane)
     for (int jh = j0; jh < min(j0 + 64, ncols); jh += 32) {
       if (i0 / nb == jh / nb)
-        gemm::gemm_tile_in<64, 32, 16, 4, 2, false, true, true>(m, ncols, kb, P_MONE, L, lda, L, lda, P_ONE, C, lda,
+        gemm::gemm_tile_in<64, 32, 16, 4, 2, false, true, SKIP>(m, ncols, kb, alpha, L, lda, L, lda, beta, C, ldc,
                                                                 i0, jh, 1, lds);
       else
-        gemm::gemm_tile_in<64, 32, 16, 4, 2, false, true, false>(m, ncols, kb, P_MONE, L, lda, L, lda, P_ONE, C, lda,
+        gemm::gemm_tile_in<64, 32, 16, 4, 2, false, true, false>(m, ncols, kb, alpha, L, lda, L, lda, beta, C, ldc,
                                                                  i0, jh, 0, lds);
     }
   } else {
     if (i0 / nb == j0 / nb)
-      gemm::gemm_tile_in<T, T, 16, T / 16, T / 16, false, true, true>(m, ncols, kb, P_MONE, L, lda, L, lda, P_ONE, C,
-                                                                     lda, i0, j0, 1, lds);
+      gemm::gemm_tile_in<T, T, 16, T / 16, T / 16, false, true, SKIP>(m, ncols, kb, alpha, L, lda, L, lda, beta, C,
+                                                                     ldc, i0, j0, 1, lds);
     else
-      gemm::gemm_tile_in<T, T, 16, T / 16, T / 16, false, true, false>(m, ncols, kb, P_MONE, L, lda, L, lda, P_ONE,
-                                                                      C, lda, i0, j0, 0, lds);
+      gemm::gemm_tile_in<T, T, 16, T / 16, T / 16, false, true, false>(m, ncols, kb, alpha, L, lda, L, lda, beta,
+                                                                      C, ldc, i0, j0, 0, lds);
   }
 }
 
 // k_potrf_trail<64> with its slices copied from FP64 operand panels of L (A side) and
 // -L^T (B side) instead of decoded in every tile: 28 KB of LDS, so five workgroups share a
 // CU as in the plain GEMM.  The same tiles, forms and order.
+template <bool SKIP>
 __global__ __launch_bounds__(WG, 3) void k_potrf_trail_panel(int m, int ncols, int kb, const uint32_t* __restrict__ L,
-                                                          int lda, uint32_t* __restrict__ C, int nb, int urgent,
-                                                          const int* stop, gemm::Panels pn) {
+                                                          int lda, uint32_t* __restrict__ C, int ldc, uint32_t alpha,
+                                                          uint32_t beta, int nb, int urgent, const int* stop,
+                                                          gemm::Panels pn) {
   PGA_KERNEL_VGPR_FLOOR();
   if (urgent) PGA_URGENT();
   __shared__ __align__(16) double sA[16][64];
@@ -1741,11 +1719,11 @@ __global__ __launch_bounds__(WG, 3) void k_potrf_trail_panel(int m, int ncols, i
   const int i0 = ti * 64, j0 = tj * 64;
   for (int jh = j0; jh < min(j0 + 64, ncols); jh += 32) {
     if (i0 / nb == jh / nb)
-      gemm::gemm_tile_impl_fp64<64, 32, 16, 4, 2, false, true, true, true>(m, ncols, kb, P_MONE, L, lda, L, lda, P_ONE,
-                                                                          C, lda, i0, jh, 1, sA, sT, pn);
+      gemm::gemm_tile_impl_fp64<64, 32, 16, 4, 2, false, true, SKIP, true>(m, ncols, kb, alpha, L, lda, L, lda, beta,
+                                                                          C, ldc, i0, jh, 1, sA, sT, pn);
     else
-      gemm::gemm_tile_impl_fp64<64, 32, 16, 4, 2, false, true, false, true>(m, ncols, kb, P_MONE, L, lda, L, lda,
-                                                                           P_ONE, C, lda, i0, jh, 0, sA, sT, pn);
+      gemm::gemm_tile_impl_fp64<64, 32, 16, 4, 2, false, true, false, true>(m, ncols, kb, alpha, L, lda, L, lda,
+                                                                           beta, C, ldc, i0, jh, 0, sA, sT, pn);
   }
 }
 
@@ -1797,36 +1775,51 @@ static int chol_trail_cap() {
   return v;
 }
 
-static hipError_t potrf_trail(hipStream_t s, int m, int ncols, int kb, const uint32_t* L, int lda, uint32_t* C,
-                              int nb, bool urgent, const int* stop, int cap = 0) {
-  if (m <= 0 || ncols <= 0 || kb <= 0) return hipSuccess;
+template <bool SKIP>
+static hipError_t potrf_trail_form(hipStream_t s, int m, int ncols, int kb, const uint32_t* L, int lda, uint32_t* C,
+                                   int ldc, uint32_t alpha, uint32_t beta, int nb, bool urgent, const int* stop,
+                                   int cap) {
   const int T = trail_tile(m, ncols, nb);
   const long tm = (m + T - 1) / T, tc = (ncols + T - 1) / T;
   const unsigned grid = (unsigned)(tc * tm - tc * (tc - 1) / 2);
   if (T == 64 && chol_trail_panels() && (long long)m * ncols * kb >= (1LL << 28)) {
-    // the B-side panel is -L^T; the Rsyrk tiles' zero skip is the tile's own (a zero stays zero)
+    // the B-side panel is alpha*L^T; the Rsyrk tiles' zero skip is the tile's own (a zero stays zero)
     gemm::Panels pn;
     void* ws = nullptr;
-    hipError_t e = gemm::panels_build(false, true, true, m, ncols, kb, P_MONE, L, lda, L, lda, s, &pn, &ws);
+    hipError_t e = gemm::panels_build(false, true, SKIP, m, ncols, kb, alpha, L, lda, L, lda, s, &pn, &ws);
     if (e != hipSuccess) return e;
     if (ws) {
-      hipLaunchKernelGGL(k_potrf_trail_panel, dim3(grid), dim3(WG), lds_for_cap((const void*)k_potrf_trail_panel, cap),
-                         s, m, ncols, kb, L, lda, C, nb, (int)urgent, stop, pn);
+      hipLaunchKernelGGL(k_potrf_trail_panel<SKIP>, dim3(grid), dim3(WG),
+                         lds_for_cap((const void*)k_potrf_trail_panel<SKIP>, cap), s, m, ncols, kb, L, lda, C, ldc,
+                         alpha, beta, nb, (int)urgent, stop, pn);
       e = hipGetLastError();
       const hipError_t f = hipFreeAsync(ws, s);
       return e != hipSuccess ? e : f;
     }
   }
-  if (T == 64)
-    hipLaunchKernelGGL(k_potrf_trail<64>, dim3(grid), dim3(WG), lds_for_cap((const void*)k_potrf_trail<64>, cap), s, m,
-                       ncols, kb, L, lda, C, nb, (int)urgent, stop);
-  else if (T == 32)
-    hipLaunchKernelGGL(k_potrf_trail<32>, dim3(grid), dim3(WG), lds_for_cap((const void*)k_potrf_trail<32>, cap), s, m,
-                       ncols, kb, L, lda, C, nb, (int)urgent, stop);
-  else
-    hipLaunchKernelGGL(k_potrf_trail<16>, dim3(grid), dim3(WG), lds_for_cap((const void*)k_potrf_trail<16>, cap), s, m,
-                       ncols, kb, L, lda, C, nb, (int)urgent, stop);
+#define PGA_TRAIL(TT)                                                                                               \
+  hipLaunchKernelGGL((k_potrf_trail<TT, SKIP>), dim3(grid), dim3(WG),                                               \
+                     lds_for_cap((const void*)k_potrf_trail<TT, SKIP>, cap), s, m, ncols, kb, L, lda, C, ldc, alpha, \
+                     beta, nb, (int)urgent, stop)
+  if (T == 64) PGA_TRAIL(64);
+  else if (T == 32) PGA_TRAIL(32);
+  else PGA_TRAIL(16);
+#undef PGA_TRAIL
   return hipGetLastError();
+}
+
+// The Cholesky trailing updates on the lower trapezoid of an m x ncols region of C whose
+// origin lies on the nb-block grid diagonal, L holding the region's rows:
+// C(i, c) = beta*C(i, c) + sum over l = 0 .. kb-1, in ascending order, of (alpha*L(c, l)) *
+// L(i, l), each step rounded (the GEMM's 'N','T' accumulation).  With skip (the lower
+// factor: alpha = -1, beta = 1) an element of an nb x nb diagonal block takes the Rsyrk
+// 'L','N' form instead, which skips zero multipliers L(c, l).
+static hipError_t potrf_trail(hipStream_t s, int m, int ncols, int kb, const uint32_t* L, int lda, uint32_t* C,
+                              int ldc, int nb, bool urgent, const int* stop, int cap = 0, uint32_t alpha = P_MONE,
+                              uint32_t beta = P_ONE, bool skip = true) {
+  if (m <= 0 || ncols <= 0 || kb <= 0) return hipSuccess;
+  return skip ? potrf_trail_form<true>(s, m, ncols, kb, L, lda, C, ldc, alpha, beta, nb, urgent, stop, cap)
+              : potrf_trail_form<false>(s, m, ncols, kb, L, lda, C, ldc, alpha, beta, nb, urgent, stop, cap);
 }
 
 
@@ -1860,7 +1853,7 @@ static hipError_t potrf_right_lower(const Ctx& c, int n, uint32_t* a, int lda, i
     if (c1 < n) {
       if ((e = hipEventRecord(x.join, c.s)) != hipSuccess) break;
       if ((e = hipStreamWaitEvent(x.s, x.join, 0)) != hipSuccess) break;
-      if ((e = potrf_trail(x.s, n - c1, n - c1, nb, A(c1, j), lda, A(c1, c1), nb, false, info, chol_trail_cap())) !=
+      if ((e = potrf_trail(x.s, n - c1, n - c1, nb, A(c1, j), lda, A(c1, c1), lda, nb, false, info, chol_trail_cap())) !=
           hipSuccess)
         break;
     }
@@ -1870,7 +1863,7 @@ static hipError_t potrf_right_lower(const Ctx& c, int n, uint32_t* a, int lda, i
       if ((e = hipEventRecord(x.done, x.s)) != hipSuccess) break;
       recorded = true;
     }
-    if ((e = potrf_trail(c.s, n - c0, nb1, nb, A(c0, j), lda, A(c0, c0), nb, true, info)) != hipSuccess) break;
+    if ((e = potrf_trail(c.s, n - c0, nb1, nb, A(c0, j), lda, A(c0, c0), lda, nb, true, info)) != hipSuccess) break;
     if ((e = potrf2(c, false, nb1, A(c0, c0), lda, info, c0)) != hipSuccess) break;
     if (c1 < n) e = trsm(c, false, false, true, true, n - c1, nb1, P_ONE, A(c0, c0), lda, A(c1, c0), lda, info);
   }
@@ -1889,10 +1882,132 @@ static hipError_t potrf_right_lower(const Ctx& c, int n, uint32_t* a, int lda, i
   return e;
 }
 
+// dst (R x K) := src^T for src K x R (rows l, columns r), through 32 x 32 LDS tiles so
+// that both sides stay coalesced; with a stop flag set, nothing.
+__global__ __launch_bounds__(WG) void k_transpose(int K, int R, const uint32_t* __restrict__ src, int lds,
+                                                  uint32_t* __restrict__ dst, int ldd, const int* stop) {
+  PGA_KERNEL_VGPR_FLOOR();
+  __shared__ uint32_t t[32][33];
+  if (stopped(stop)) return;
+  const int r0 = blockIdx.x * 32, l0 = blockIdx.y * 32, tx = threadIdx.x % 32, ty = threadIdx.x / 32;
+  for (int q = ty; q < 32; q += WG / 32) {
+    const int l = l0 + tx, r = r0 + q;
+    if (l < K && r < R) t[q][tx] = src[l + (size_t)r * lds];
+  }
+  __syncthreads();
+  for (int q = ty; q < 32; q += WG / 32) {
+    const int r = r0 + tx, l = l0 + q;
+    if (l < K && r < R) dst[r + (size_t)l * ldd] = t[tx][q];
+  }
+}
+
+// The upper Cholesky's block row J takes its sums (Rsyrk 'U','T' on the diagonal block,
+// RgemmG 'T','N' on the rest, Rpotrf_opencl.cpp:107,114-119; both end with
+// c = alpha*t + beta*c, Rsyrk.cpp:188-192, Rgemm.cpp:197-201): A := -1*t + 1*A on an
+// m x ncols region, its upper triangle only where `diag`, with the sum of element (i, c)
+// at St[c + i * lds] (the sums are kept transposed); with a stop flag set, nothing.
+__global__ __launch_bounds__(WG) void k_apply_sums(int m, int ncols, const uint32_t* __restrict__ St, int lds,
+                                                   uint32_t* __restrict__ A, int lda, int diag, const int* stop) {
+  PGA_KERNEL_VGPR_FLOOR();
+  if (stopped(stop)) return;
+  for (long e = blockIdx.x * (long)WG + threadIdx.x; e < (long)m * ncols; e += (long)gridDim.x * WG) {
+    const int i = (int)(e % m), c = (int)(e / m);
+    if (diag && i > c) continue;
+    uint32_t& v = A[i + (size_t)c * lda];
+    v = add(mul(P_MONE, St[c + (size_t)i * lds]), mul(P_ONE, v));
+  }
+}
+
+static hipError_t apply_sums(hipStream_t s, int m, int ncols, const uint32_t* St, int lds, uint32_t* A, int lda,
+                             bool diag, const int* stop) {
+  const long total = (long)m * ncols;
+  if (total <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_apply_sums, dim3((unsigned)std::min<long>((total + WG - 1) / WG, 4096)), dim3(WG), 0, s, m,
+                     ncols, St, lds, A, lda, (int)diag, stop);
+  return hipGetLastError();
+}
+
+// Upper Rpotrf_opencl (Rpotrf_opencl.cpp:100-124) in right-looking order.  The reference
+// forms block row J from all earlier ones at once: Rsyrk 'U','T' on the diagonal block
+// and RgemmG 'T','N' on the rest both compute t = sum over l < J of a_li * a_lc from zero
+// in ascending l (each step t = t + a_li * a_lc rounded), then c = -t + c.  Those running
+// sums live in a workspace, transposed (St(c, i) = t of element (i, c), a lower
+// trapezoid): once block row j of U is final, its nb terms are added to every later
+// element's sum in ascending l by the lower trailing update with alpha = 1 and no zero
+// skip (potrf_trail, over P = the block row's transpose), beta = 0 for the first block so
+// that every sum starts from zero; St keeps each sum as the posit the reference holds
+// after that term.  Block row J then takes c := -t + c on its diagonal block, Rpotrf2,
+// and, once that succeeded, c := -t + c and Rtrsm on the rest (the reference's RgemmG
+// comes after its info check).  The caller's matrix is written only in the block row in
+// hand, so a failing block leaves it as the reference does.  One block of lookahead: the
+// caller's stream adds block j to block row j+1's sums and factors that row, while a
+// second stream adds block j to every later row's sums.
+static hipError_t potrf_right_upper(const Ctx& c, int n, uint32_t* a, int lda, int* info, int nb) {
+  auto A = [&](int r, int col) { return a + r + (size_t)col * lda; };
+  uint32_t *S = nullptr, *P = nullptr;
+  PGA_TRY_E(hipMallocAsync((void**)&S, (size_t)n * n * sizeof(uint32_t), c.s));
+  hipError_t e = hipMallocAsync((void**)&P, (size_t)2 * n * nb * sizeof(uint32_t), c.s);
+  Aux x;
+  if (e == hipSuccess) e = aux_take(x);
+  if (e != hipSuccess) {
+    (void)hipFreeAsync(S, c.s);
+    if (P) (void)hipFreeAsync(P, c.s);
+    return e;
+  }
+  auto St = [&](int r, int col) { return S + r + (size_t)col * n; };
+  const int jb0 = std::min(nb, n);
+  e = potrf2(c, true, jb0, a, lda, info, 0);
+  if (e == hipSuccess && jb0 < n) e = trsm(c, true, true, true, true, jb0, n - jb0, P_ONE, a, lda, A(0, jb0), lda, info);
+  bool recorded = false;  // the aux stream has recorded x.done at least once
+  for (int j = 0; j + nb < n && e == hipSuccess; j += nb) {
+    const int c0 = j + nb, nb1 = std::min(nb, n - c0), c1 = c0 + nb1, ldp = n - c0;
+    const uint32_t beta = j == 0 ? 0u : P_ONE;  // block 0's terms start every sum
+    // P (n - c0 x nb) = (rows j .. j+nb-1 of U, columns c0 ..)^T; the two buffers alternate,
+    // and the aux launch that read this one two blocks ago has ended (x.done, waited below)
+    uint32_t* Pj = P + (size_t)((j / nb) & 1) * n * nb;
+    hipLaunchKernelGGL(k_transpose, dim3((unsigned)((n - c0 + 31) / 32), (unsigned)((nb + 31) / 32)), dim3(WG), 0,
+                       c.s, nb, n - c0, A(j, c0), lda, Pj, ldp, info);
+    if ((e = hipGetLastError()) != hipSuccess) break;
+    if (c1 < n) {  // the aux stream adds block j to the sums of rows c1 ..
+      if ((e = hipEventRecord(x.join, c.s)) != hipSuccess) break;
+      if ((e = hipStreamWaitEvent(x.s, x.join, 0)) != hipSuccess) break;
+      if ((e = potrf_trail(x.s, n - c1, n - c1, nb, Pj + (c1 - c0), ldp, St(c1, c1), n, nb, false, info,
+                           chol_trail_cap(), P_ONE, beta, false)) != hipSuccess)
+        break;
+    }
+    // block row c0's sums have every earlier block's terms once the previous aux launch ends
+    if (recorded && (e = hipStreamWaitEvent(c.s, x.done, 0)) != hipSuccess) break;
+    if (c1 < n) {
+      if ((e = hipEventRecord(x.done, x.s)) != hipSuccess) break;
+      recorded = true;
+    }
+    if ((e = potrf_trail(c.s, n - c0, nb1, nb, Pj, ldp, St(c0, c0), n, nb, true, info, 0, P_ONE, beta, false)) !=
+        hipSuccess)
+      break;
+    if ((e = apply_sums(c.s, nb1, nb1, St(c0, c0), n, A(c0, c0), lda, true, info)) != hipSuccess) break;
+    if ((e = potrf2(c, true, nb1, A(c0, c0), lda, info, c0)) != hipSuccess) break;
+    if (c1 < n) {
+      if ((e = apply_sums(c.s, nb1, n - c1, St(c1, c0), n, A(c0, c1), lda, false, info)) != hipSuccess) break;
+      e = trsm(c, true, true, true, true, nb1, n - c1, P_ONE, A(c0, c0), lda, A(c0, c1), lda, info);
+    }
+  }
+  if (recorded) {
+    hipError_t w = hipStreamWaitEvent(c.s, x.done, 0);
+    if (e == hipSuccess) e = w;
+  }
+  hipError_t f = hipFreeAsync(S, c.s);
+  if (e == hipSuccess) e = f;
+  f = hipFreeAsync(P, c.s);
+  if (e == hipSuccess) e = f;
+  aux_give(x);
+  return e;
+}
+
 // Rpotrf_opencl (Rpotrf_opencl.cpp:32-166): blocked left-looking Cholesky
 static hipError_t potrf(const Ctx& c, bool upper, int n, uint32_t* a, int lda, int* info, int nb) {
   if (nb <= 1 || nb >= n) return potrf2(c, upper, n, a, lda, info, 0);
   if (!upper && n >= POTRF_OVERLAP_MIN_N && nb % 16 == 0) return potrf_right_lower(c, n, a, lda, info, nb);
+  if (upper && n >= POTRF_OVERLAP_MIN_N && nb % 16 == 0) return potrf_right_upper(c, n, a, lda, info, nb);
   if (n >= POTRF_OVERLAP_MIN_N) return potrf_overlap(c, upper, n, a, lda, info, nb);
   auto A = [&](int r, int col) { return a + r + (size_t)col * lda; };
   for (int j = 0; j < n; j += nb) {

@@ -508,12 +508,15 @@ def test_cholesky_1024_device(pga, oracle):
 @pytest.mark.parametrize("uplo,n,nb,bad", [("L", 1100, 64, 0), ("L", 1100, 64, 700), ("U", 1100, 64, 0),
                                            ("U", 1100, 64, 700), ("L", 1300, 128, 0), ("L", 1300, 128, 5),
                                            ("L", 1300, 128, 1290), ("L", 1111, 48, 1063), ("L", 1040, 80, 0),
-                                           ("L", 1040, 80, 959), ("L", 1030, 50, 0), ("L", 1030, 50, 777)])
+                                           ("L", 1040, 80, 959), ("L", 1030, 50, 0), ("L", 1030, 50, 777),
+                                           ("U", 1300, 128, 5), ("U", 1300, 64, 1290), ("U", 1111, 48, 1063),
+                                           ("U", 1040, 80, 0), ("U", 1030, 50, 777), ("U", 1024, 64, 63)])
 def test_cholesky_panel_overlap(pga, oracle, uplo, n, nb, bad):
-    """Orders from 1024 run on two streams: the upper factor with each block's panel GEMM
-    beside the block's Rsyrk / Rpotrf2, the lower one right-looking with one block of
-    lookahead.  The factor, and after a failing block the whole matrix the reference
-    leaves behind, must stay the same bits."""
+    """Orders from 1024 run on two streams, right-looking with one block of lookahead: the
+    lower factor updates the trailing matrix in place, the upper one keeps each later
+    element's Rsyrk / RgemmG sum in a workspace until its block row comes up.  The factor,
+    and after a failing block the whole matrix the reference leaves behind, must stay the
+    same bits."""
     A = _spd(oracle, 62, n)
     if bad:
         A[bad, bad] = oracle.from_double(-2.0)
