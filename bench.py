@@ -438,6 +438,12 @@ def bench_cholesky(pga, info, args):
     pga.synchronize()
     same = bool(np.array_equal(L1, np.tril(dA.to_host())))
     del L1
+    # the upper factor of the same matrix (Rpotrf_opencl 'U', nb = 64): its own sums per
+    # block row, so no block-size invariance to check; its bits are pinned by the tests
+    dA.copy_from(dA0)
+    pga.synchronize()
+    t_fu = _timed(pga, lambda: pga.potrf_device("U", n, dA, n, di, 0))
+    info_u = int(di.to_host()[0])
     ops = n ** 3 / 3.0
     cpu = None
     if not args.no_cpu_baseline:
@@ -459,7 +465,8 @@ def bench_cholesky(pga, info, args):
         "data": "synthetic: M*M^T/n + I with M seeded uniform[-1,1), FP64 -> posit32",
         "config": {"workload": f"Rpotrf_opencl lower n={n} nb=64 + Rpotrs", "factor_s": round(t_f, 4),
                    "solve_s": round(t_s, 4), "info": info_v, "backward_error": berr,
-                   "lower_nb64_equals_nb128": same, "device": info["name"]},
+                   "lower_nb64_equals_nb128": same, "upper_factor_s": round(t_fu, 4), "upper_info": info_u,
+                   "device": info["name"]},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
