@@ -488,8 +488,12 @@ static __device__ void lutn_cols_pattern(int m, int C, const uint32_t* __restric
 // pattern-level sweep.
 constexpr int LUTN_COLS = 16, LUTN_MMAX = 64;
 
+// With St, B first takes the upper Cholesky's right-looking sums, b := -t + b with the
+// sum of element (i, c) at St[c + i * lds] (k_apply_sums on the block row), written back
+// before the solve.
 __global__ __launch_bounds__(WG) void k_trsm_lutn_fp64(int m, int n, const uint32_t* __restrict__ A, int lda,
-                                                       uint32_t* __restrict__ B, int ldb, const int* stop) {
+                                                       uint32_t* __restrict__ B, int ldb, const int* stop,
+                                                       const uint32_t* __restrict__ St = nullptr, int lds = 0) {
   PGA_KERNEL_VGPR_FLOOR();
   PGA_URGENT();
   // U's strictly upper triangle, packed by columns: u_ki (k < i) at i * (i - 1) / 2 + k
@@ -505,7 +509,15 @@ __global__ __launch_bounds__(WG) void k_trsm_lutn_fp64(int m, int n, const uint3
     const int k = e % m, i = e / m;
     if (k < i) sU[i * (i - 1) / 2 + k] = to_double(A[k + (size_t)i * lda]);
   }
-  for (int e = tid; e < m * C; e += WG) sB[e % m][e / m] = to_double(b[e % m + (size_t)(e / m) * ldb]);
+  for (int e = tid; e < m * C; e += WG) {
+    const int r = e % m, cl = e / m;
+    uint32_t v = b[r + (size_t)cl * ldb];
+    if (St) {
+      v = add(mul(P_MONE, St[c0 + cl + (size_t)r * lds]), mul(P_ONE, v));
+      b[r + (size_t)cl * ldb] = v;  // the pattern-level sweep below starts from it too
+    }
+    sB[r][cl] = to_double(v);
+  }
   if (tid < m) {
     const double d = to_double(A[tid + (size_t)tid * lda]);
     sD[tid] = (d != 0.0) ? d : __builtin_nan("");  // a zero pivot: redo (NaR is NaN already)
@@ -847,23 +859,30 @@ static __device__ int potrf2_upper_decoded(int n, uint32_t* a, int lda, double* 
     if (r <= c) at(r, c) = to_double(a[r + (size_t)c * lda]);
   }
   __syncthreads();
+  __shared__ int ok;
   for (int j = 0; j < n; ++j) {
-    const double v = at(j, j);
-    if (!(v > 0.0)) return -1;  // uniform (zero, negative or NaR)
+    if (tid == 0) {  // the leaf: test and root (Rpotrf2.cpp:90-97)
+      const double v = at(j, j);
+      ok = v > 0.0;  // zero, negative or NaR: the pattern-level form decides
+      if (ok) at(j, j) = fp64_posit_sqrt(v);
+    }
     __syncthreads();
-    if (tid == 0) at(j, j) = fp64_posit_sqrt(v);
-    __syncthreads();
+    if (!ok) return -1;  // uniform (every leaf but the last is followed by its node's
+                         // barriers before lane 0 writes ok again)
     int s, mid, e;
     if (wg::node_after(j, n, s, mid, e)) {
       const int n2 = e - mid;
-      for (int k = s; k < mid; ++k) {  // A12 := U11^-T A12, rows s .. mid-1
-        const double d = at(k, k);
-        for (int c = mid + tid; c < e; c += T) at(k, c) = fp64_posit_div(at(k, c), d);
-        __syncthreads();
+      // A12 := U11^-T A12, rows s .. mid-1: row k is final once it has its terms from rows
+      // s .. k-1 and its quotient, so the lane that gives element (k+1, c) its last term
+      // forms that quotient at once (one barrier a row)
+      for (int c = mid + tid; c < e; c += T) at(s, c) = fp64_posit_div(at(s, c), at(s, s));
+      __syncthreads();
+      for (int k = s; k + 1 < mid; ++k) {
         const int cnt = mid - 1 - k;
         for (int q = tid; q < cnt * n2; q += T) {
           const int i = k + 1 + q % cnt, c = mid + q / cnt;
-          at(i, c) = fp64_mac_arith(at(i, c), -at(k, i), at(k, c));
+          const double x = fp64_mac_arith(at(i, c), -at(k, i), at(k, c));
+          at(i, c) = i == k + 1 ? fp64_posit_div(x, at(i, i)) : x;
         }
         __syncthreads();
       }
@@ -935,11 +954,24 @@ static __device__ int potrf2_upper_plain(int n, uint32_t* a, int lda) {
 // (potrf2_upper_decoded, 17 KB of LDS), and where that gives up the recursion in memory
 // (potrf2_upper_plain), so that the block finds room beside the concurrent trailing
 // update.
-__global__ __launch_bounds__(WG, 2) void k_potrf2_upper_fast(int n, uint32_t* a, int lda, int* info, int col_off) {
+// With St, the block first takes its right-looking sums, c := -t + c with the sum of
+// element (i, c) at St[c + i * lds] (k_apply_sums on the diagonal block).
+__global__ __launch_bounds__(WG, 2) void k_potrf2_upper_fast(int n, uint32_t* a, int lda, int* info, int col_off,
+                                                             const uint32_t* __restrict__ St, int lds) {
   PGA_KERNEL_VGPR_FLOOR();
   PGA_URGENT();
   __shared__ double S[POTRF2_UPPER_PACKED];
   if (*info) return;  // every lane reads the same flag
+  if (St) {
+    for (int e = threadIdx.x; e < n * n; e += WG) {
+      const int i = e % n, c = e / n;
+      if (i <= c) {
+        uint32_t& v = a[i + (size_t)c * lda];
+        v = add(mul(P_MONE, St[c + (size_t)i * lds]), mul(P_ONE, v));
+      }
+    }
+    __syncthreads();
+  }
   if (potrf2_upper_decoded<WG>(n, a, lda, S) == 0) return;
   __syncthreads();
   const int f = potrf2_upper_plain<WG>(n, a, lda);
@@ -1554,7 +1586,8 @@ static hipError_t syrk(const Ctx& c, bool upper, bool trans, int n, int k, uint3
 static hipError_t potrf2(const Ctx& c, bool upper, int n, uint32_t* a, int lda, int* info, int col_off) {
   if (n == 0) return hipSuccess;
   if (n <= POTRF2_ONE_WG_MAX) {
-    if (upper) hipLaunchKernelGGL(k_potrf2_upper_fast, dim3(1), dim3(WG), 0, c.s, n, a, lda, info, col_off);
+    if (upper)
+      hipLaunchKernelGGL(k_potrf2_upper_fast, dim3(1), dim3(WG), 0, c.s, n, a, lda, info, col_off, nullptr, 0);
     else hipLaunchKernelGGL(k_potrf2_lower_fast, dim3(1), dim3(WG), 0, c.s, n, a, lda, info, col_off);
     return hipGetLastError();
   }
@@ -1984,6 +2017,18 @@ static hipError_t potrf_right_upper(const Ctx& c, int n, uint32_t* a, int lda, i
     if ((e = potrf_trail(c.s, n - c0, nb1, nb, Pj, ldp, St(c0, c0), n, nb, true, info, 0, P_ONE, beta, false)) !=
         hipSuccess)
       break;
+    if (nb1 <= POTRF2_ONE_WG_MAX && nb1 <= LUTN_MMAX) {
+      // the sums applied inside the diagonal block's and the row solve's own launches
+      hipLaunchKernelGGL(k_potrf2_upper_fast, dim3(1), dim3(WG), 0, c.s, nb1, A(c0, c0), lda, info, c0,
+                         St(c0, c0), n);
+      if ((e = hipGetLastError()) != hipSuccess) break;
+      if (c1 < n) {
+        hipLaunchKernelGGL(k_trsm_lutn_fp64, dim3((unsigned)((n - c1 + LUTN_COLS - 1) / LUTN_COLS)), dim3(WG), 0, c.s,
+                           nb1, n - c1, A(c0, c0), lda, A(c0, c1), lda, info, St(c1, c0), n);
+        e = hipGetLastError();
+      }
+      continue;
+    }
     if ((e = apply_sums(c.s, nb1, nb1, St(c0, c0), n, A(c0, c0), lda, true, info)) != hipSuccess) break;
     if ((e = potrf2(c, true, nb1, A(c0, c0), lda, info, c0)) != hipSuccess) break;
     if (c1 < n) {

@@ -389,6 +389,18 @@ PGA_HD double fp64_grid_c_arith(double x) {
   return __builtin_ldexp(1.5, in ? e + 25 + ak : 2000);
 }
 
+// Half the grid's quantum for a constant c = 1.5 * 2^E from fp64_grid_c_arith: 2^(E - 53),
+// which is c / 1.5 * 2^-53 exactly, from c's exponent field (for an infinite c, a finite
+// value that no difference compares equal to, as the division would give infinity).
+PGA_HD double fp64_grid_half(double c) {
+  uint64_t b;
+  __builtin_memcpy(&b, &c, sizeof b);
+  const uint64_t hb = (((b >> 52) & 0x7FFu) - 53u) << 52;
+  double h;
+  __builtin_memcpy(&h, &hb, sizeof h);
+  return h;
+}
+
 // The GEMM's FP64 product and multiply-add roundings (rgemm_tile.hpp: fp64_posit_mul,
 // fp64_mac) with the constants computed (fp64_grid_c_arith) instead of read from a table:
 // round(t * a) as fma(t, a, C) - C, and round(c + round(t * a)) as (x + C) - C.  NaN where
@@ -413,7 +425,7 @@ PGA_HD double fp64_posit_recip(double a) {
   const double q = 1.0 / a;
   const double c = fp64_grid_c_arith(q);
   const double r = (q + c) - c;
-  const double half = (c / 1.5) * 0x1p-53;  // C = 1.5 * 2^E: the grid's quantum is 2^(E - 52)
+  const double half = fp64_grid_half(c);  // C = 1.5 * 2^E: the grid's quantum is 2^(E - 52)
   if (__builtin_fabs(q - r) == half) return to_double(div(P_ONE, from_double(a)));
   return r;
 }
@@ -427,7 +439,7 @@ PGA_HD double fp64_posit_div(double b, double a) {
   const double q = b / a;
   const double c = fp64_grid_c_arith(q);
   const double r = (q + c) - c;
-  const double half = (c / 1.5) * 0x1p-53;
+  const double half = fp64_grid_half(c);
   if (__builtin_fabs(q - r) == half) return to_double(div(from_double(b), from_double(a)));
   return r;
 }
@@ -440,7 +452,7 @@ PGA_HD double fp64_posit_sqrt(double a) {
   const double q = __builtin_sqrt(a);
   const double c = fp64_grid_c_arith(q);
   const double r = (q + c) - c;
-  const double half = (c / 1.5) * 0x1p-53;
+  const double half = fp64_grid_half(c);
   const double slack = __builtin_ldexp(q, -49);  // more than 4 ulps of q
   if (__builtin_fabs(__builtin_fabs(q - r) - half) <= slack) return to_double(pga::sqrt(from_double(a)));
   return r;
