@@ -44,7 +44,7 @@ __device__ void run_task(const FactorTask& t, Shared& sh) {
 
 // two workgroups per CU: the tile code then keeps its registers (248) without spills in
 // its loops, and one workgroup's barrier-bound steps overlap the other's tile work
-__global__ __launch_bounds__(NT, 2) void k_factor_tasks(const FactorTask* __restrict__ tasks, int ntasks,
+__global__ __launch_bounds__(NT, 3) void k_factor_tasks(const FactorTask* __restrict__ tasks, int ntasks,
                                                      int* __restrict__ counter) {
   PGA_KERNEL_VGPR_FLOOR();
   __shared__ Shared sh;

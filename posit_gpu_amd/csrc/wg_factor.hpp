@@ -54,16 +54,16 @@ static __device__ __forceinline__ uint32_t* at(uint32_t* a, int lda, int i, int 
 // tri 1 / 2 stores only the lower / upper triangle (Rsyrk).  The per-element order does
 // not depend on the tile shape, so the shape is chosen for the least padding.
 // ---------------------------------------------------------------------------------
-template <int T, bool TA, bool TB, bool SKIP0>
+template <int BM, int BN, bool TA, bool TB, bool SKIP0>
 static __device__ __noinline__ void gemm_t(int m, int n, int k, const uint32_t* A, int lda, const uint32_t* B, int ldb,
                                     uint32_t* C, int ldc, int tri, lds_u32* lds) {
-  const int tm = (m + T - 1) / T, tn = (n + T - 1) / T;
+  const int tm = (m + BM - 1) / BM, tn = (n + BN - 1) / BN;
   for (int t = 0; t < tm * tn; ++t) {
-    const int i0 = (t % tm) * T, j0 = (t / tm) * T;
-    if (tri == 1 && i0 + T - 1 < j0) continue;
-    if (tri == 2 && j0 + T - 1 < i0) continue;
-    gemm::gemm_tile_in<T, T, 16, T / 16, T / 16, TA, TB, SKIP0>(m, n, k, P_MONE, A, lda, B, ldb, P_ONE, C, ldc,
-                                                                i0, j0, tri, (uint32_t*)lds);
+    const int i0 = (t % tm) * BM, j0 = (t / tm) * BN;
+    if (tri == 1 && i0 + BM - 1 < j0) continue;
+    if (tri == 2 && j0 + BN - 1 < i0) continue;
+    gemm::gemm_tile_in<BM, BN, 16, BM / 16, BN / 16, TA, TB, SKIP0>(m, n, k, P_MONE, A, lda, B, ldb, P_ONE, C, ldc,
+                                                                    i0, j0, tri, (uint32_t*)lds);
   }
   __syncthreads();
 }
@@ -105,14 +105,15 @@ __device__ __forceinline__ void gemm(int m, int n, int k, const uint32_t* A, int
     gemm_direct<TA, TB, SKIP0>(m, n, k, A, lda, B, ldb, C, ldc, tri);
     return;
   }
-  auto cover = [&](int T) { return (long)((m + T - 1) / T) * ((n + T - 1) / T) * T * T; };
+  auto cover = [&](int BM, int BN) { return (long)((m + BM - 1) / BM) * ((n + BN - 1) / BN) * BM * BN; };
   // relative cost per covered element: fewer elements per lane amortise the LDS
-  // reads over fewer multiply-adds
-  const long c64 = cover(64) * 8, c32 = cover(32) * 9, c16 = cover(16) * 11;
+  // reads over fewer multiply-adds (64 x 32 rather than 64 x 64: 4 x 2 per lane keeps the
+  // task kernels' registers at a level that lets more workgroups share a CU)
+  const long c64 = cover(64, 32) * 8, c32 = cover(32, 32) * 9, c16 = cover(16, 16) * 11;
   lds_u32* lds = (lds_u32*)sh.tile;
-  if (c64 <= c32 && c64 <= c16) gemm_t<64, TA, TB, SKIP0>(m, n, k, A, lda, B, ldb, C, ldc, tri, lds);
-  else if (c32 <= c16) gemm_t<32, TA, TB, SKIP0>(m, n, k, A, lda, B, ldb, C, ldc, tri, lds);
-  else gemm_t<16, TA, TB, SKIP0>(m, n, k, A, lda, B, ldb, C, ldc, tri, lds);
+  if (c64 <= c32 && c64 <= c16) gemm_t<64, 32, TA, TB, SKIP0>(m, n, k, A, lda, B, ldb, C, ldc, tri, lds);
+  else if (c32 <= c16) gemm_t<32, 32, TA, TB, SKIP0>(m, n, k, A, lda, B, ldb, C, ldc, tri, lds);
+  else gemm_t<16, 16, TA, TB, SKIP0>(m, n, k, A, lda, B, ldb, C, ldc, tri, lds);
 }
 
 // ---------------------------------------------------------------------------------
