@@ -1079,7 +1079,26 @@ __global__ __launch_bounds__(64) void k_trsm_diag(int ib, int n, int nounit, con
     if (r < ib) sa[r][c] = A[r + (size_t)c * lda];
   __syncthreads();
   uint32_t* bj = B + (size_t)j * ldb;
-  uint32_t b = r < ib ? bj[r] : 0;
+  const uint32_t b0 = r < ib ? bj[r] : 0;
+  // FP64 form: each term rounded as the pattern-level operations round it (fp64_mac_arith,
+  // fp64_posit_div) while every value stays in scales -108 .. 107; a NaN anywhere (a scale
+  // outside them, a NaR) makes the wave redo its column at the pattern level
+  {
+    double b = to_double(b0);
+    for (int s = 0; s < ib; ++s) {
+      const int k = UPPER ? ib - 1 - s : s;
+      double bk = __shfl(b, k);
+      if (bk == 0.0) continue;  // the same value in every lane
+      if (nounit) bk = fp64_posit_div(bk, to_double(sa[k][k]));
+      if (UPPER ? r < k : (r > k && r < ib)) b = fp64_mac_arith(b, -bk, to_double(sa[r][k]));
+      if (r == k) b = bk;
+    }
+    if (!__any(b != b)) {
+      if (r < ib) bj[r] = from_double(b);
+      return;
+    }
+  }
+  uint32_t b = b0;
   for (int s = 0; s < ib; ++s) {
     const int k = UPPER ? ib - 1 - s : s;
     uint32_t bk = __shfl(b, k);
@@ -1184,6 +1203,9 @@ static hipError_t trsm(const Ctx& c, bool left, bool upper, bool trans, bool nou
     hipLaunchKernelGGL(k_zero, dim3(blocks_for(m)), dim3(WG), 0, c.s, m, n, B, ldb, stop);
     return hipGetLastError();
   }
+  // a diagonal block's worth of rows: one wave per right-hand side (k_trsm_diag)
+  if (left && !trans && alpha == P_ONE && m <= TRSM_DIAG_MAX && trsm_narrow(n))
+    return trsm_diag(c, upper, nounit, m, n, A, lda, B, ldb, stop);
   if (left && !trans && (long)n >= 8L * runtime().cus) {
     const int cpw = (int)std::min<long>(8, (n + 4L * runtime().cus - 1) / (4L * runtime().cus));
     const unsigned grid = (unsigned)((n + cpw - 1) / cpw);
