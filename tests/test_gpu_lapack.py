@@ -46,6 +46,44 @@ def test_trsm(pga, oracle, side, uplo, transa, diag, alpha):
     _same(got, want)
 
 
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("diag", ["N", "U"])
+@pytest.mark.parametrize("kind", ["huge", "tiny", "nar", "cancel"])
+def test_trsm_diagonal_block_classes(pga, oracle, uplo, diag, kind):
+    """Left non-transposed solves of up to 64 rows run one wave per right-hand side in
+    FP64 and redo a column at the pattern level when a value leaves the fast range or is
+    NaR: scaled far up or down, a NaR inside the triangle, and cancelling sums."""
+    m, n = 48, 40
+    rng = np.random.default_rng({"huge": 81, "tiny": 82, "nar": 83, "cancel": 84}[kind])
+    if kind == "cancel":
+        Ad = rng.choice([-1.0, -0.5, 0.5, 1.0], (m, m))
+        Bd = rng.choice([-1.0, -0.5, 0.5, 1.0, 0.0], (m, n))
+    else:
+        Ad = rng.uniform(-1, 1, (m, m))
+        Bd = rng.uniform(-1, 1, (m, n))
+        if kind == "huge":
+            Ad *= 2.0 ** 50
+            Bd *= 2.0 ** 55
+        elif kind == "tiny":
+            Ad *= 2.0 ** -50
+            Bd *= 2.0 ** -58
+    Ad += np.eye(m) * (2.0 if kind != "huge" else 2.0 ** 51)
+    Ad = np.triu(Ad) if uplo == "U" else np.tril(Ad)
+    A = np.asfortranarray(oracle.from_double_v(Ad.ravel(order="F")).reshape(m, m, order="F"))
+    B = np.asfortranarray(oracle.from_double_v(Bd.ravel(order="F")).reshape(m, n, order="F"))
+    if kind == "nar":
+        if uplo == "L":
+            A[30, 12] = NAR
+        else:
+            A[12, 30] = NAR
+        B[5, 7] = NAR
+    want = B.copy(order="F")
+    assert oracle.rtrsm("L", uplo, "N", diag, m, n, ONE, A, want) == 0
+    got = B.copy(order="F")
+    pga.Rtrsm("L", uplo, "N", diag, m, n, ONE, A, m, got, m)
+    _same(got, want)
+
+
 @pytest.mark.parametrize("diag", ["N", "U"])
 @pytest.mark.parametrize("special", [False, True])
 def test_trsm_left_lower_tall(pga, oracle, diag, special):
