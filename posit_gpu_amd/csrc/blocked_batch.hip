@@ -100,6 +100,7 @@ __global__ __launch_bounds__(NT) void k_bb_trsm(const FactorTask* __restrict__ t
                                                 const Item* __restrict__ items) {
   PGA_KERNEL_VGPR_FLOOR();
   PGA_URGENT();
+  __shared__ uint32_t sA[64 * 64];  // the diagonal block for the one-wave-per-column solves
   const Item it = items[blockIdx.x];
   const FactorTask& t = tasks[it.task];
   const int lda = t.lda, j = it.j, jb = it.jb;
@@ -118,12 +119,19 @@ __global__ __launch_bounds__(NT) void k_bb_trsm(const FactorTask* __restrict__ t
       }
     }
     __syncthreads();
-    wg::trsm_llnu(jb, it.cnt, d, lda, d + (size_t)(jb + it.off) * lda, lda);
+    if (jb <= 64)
+      wg::trsm_waves<wg::WS_LLNU>(jb, it.cnt, d, lda, d + (size_t)(jb + it.off) * lda, lda, (wg::lds_u32*)sA);
+    else wg::trsm_llnu(jb, it.cnt, d, lda, d + (size_t)(jb + it.off) * lda, lda);
     return;
   }
   if (*t.info) return;
-  if (t.uplo == 'U') wg::trsm_lutn(jb, it.cnt, d, lda, d + (size_t)(jb + it.off) * lda, lda);
-  else wg::trsm_rltn(it.cnt, jb, d, lda, d + jb + it.off, lda);
+  if (t.uplo == 'U') {
+    if (jb <= 64)
+      wg::trsm_waves<wg::WS_LUTN>(jb, it.cnt, d, lda, d + (size_t)(jb + it.off) * lda, lda, (wg::lds_u32*)sA);
+    else wg::trsm_lutn(jb, it.cnt, d, lda, d + (size_t)(jb + it.off) * lda, lda);
+  } else {
+    wg::trsm_rltn(it.cnt, jb, d, lda, d + jb + it.off, lda);
+  }
 }
 
 // LU panels of RECUR_W columns, all problems at once, walked node by node in the

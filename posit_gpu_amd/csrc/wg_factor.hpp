@@ -224,6 +224,65 @@ static __device__ __noinline__ void trsm_rltn(int m, int n, const uint32_t* L, i
   }
 }
 
+// Left solves of up to 64 rows, one wave per right-hand side: lane r holds b_r as an
+// FP64 value and each step's b_k reaches the other lanes by a shuffle, so the steps need
+// no barrier.  The operations and order of trsm_llnu, trsm_llnn and trsm_lutn: the lower
+// forms take k ascending, a zero b_k skipping its step (non-unit: b_k divided by l_kk
+// before its terms); the transposed upper form takes k ascending with no skip, b_k
+// divided by u_kk once its terms are in.  Products and sums are rounded as the
+// pattern-level operations round them (fp64_mac_arith, fp64_posit_div) while every value
+// stays in scales -108 .. 107; a NaN in any lane (out of range, NaR) makes the wave redo
+// its column at the pattern level.  A (m x m) is staged in sA (LDS, m * m words); every
+// lane of the block takes part.
+enum WaveSolve { WS_LLNU, WS_LLNN, WS_LUTN };
+
+template <int MODE, int T = NT>
+static __device__ __noinline__ void trsm_waves(int m, int n, const uint32_t* A, int lda, uint32_t* B, int ldb,
+                                               lds_u32* sA) {
+  for (int e = threadIdx.x; e < m * m; e += T) sA[e] = A[e % m + (size_t)(e / m) * lda];
+  __syncthreads();
+  const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+  auto a = [&](int r, int c) -> uint32_t { return sA[r + c * m]; };
+  for (int j = wave; j < n; j += T / 64) {
+    uint32_t* bj = B + (size_t)j * ldb;
+    const uint32_t b0 = lane < m ? bj[lane] : 0u;
+    double b = to_double(b0);
+    for (int k = 0; k < m; ++k) {
+      if (MODE == WS_LUTN) {
+        if (lane == k) b = fp64_posit_div(b, to_double(a(k, k)));
+        const double bk = __shfl(b, k);
+        if (lane > k && lane < m) b = fp64_mac_arith(b, -to_double(a(k, lane)), bk);
+      } else {
+        double bk = __shfl(b, k);
+        if (bk == 0.0) continue;  // the same value in every lane
+        if (MODE == WS_LLNN) bk = fp64_posit_div(bk, to_double(a(k, k)));
+        if (lane > k && lane < m) b = fp64_mac_arith(b, -bk, to_double(a(lane, k)));
+        if (lane == k) b = bk;
+      }
+    }
+    if (!__any(b != b)) {
+      if (lane < m) bj[lane] = from_double(b);
+      continue;
+    }
+    uint32_t p = b0;  // the column again at the pattern level
+    for (int k = 0; k < m; ++k) {
+      if (MODE == WS_LUTN) {
+        if (lane == k) p = div(p, a(k, k));
+        const uint32_t pk = __shfl(p, k);
+        if (lane > k && lane < m) p = sub(p, mul(a(k, lane), pk));
+      } else {
+        uint32_t pk = __shfl(p, k);
+        if (pk == 0) continue;
+        if (MODE == WS_LLNN) pk = div(pk, a(k, k));
+        if (lane > k && lane < m) p = sub(p, mul(pk, a(lane, k)));
+        if (lane == k) p = pk;
+      }
+    }
+    if (lane < m) bj[lane] = p;
+  }
+  __syncthreads();
+}
+
 // ---------------------------------------------------------------------------------
 // LU (Rgetrf_opencl / Rgetrf2)
 // ---------------------------------------------------------------------------------
