@@ -756,160 +756,6 @@ __global__ void k_potrf2_leaf(uint32_t* a, int* info, int col_off) {
 // ---------------------------------------------------------------------------------
 constexpr int POTRF2_ONE_WG_MAX = 64;
 
-// Lower Rpotrf2 of an n x n block (n <= 64) with the lower triangle held in registers,
-// T lanes, packed column-major elements dealt round-robin.  The recursion
-// (Rpotrf2.cpp:102-145) gives element (i, j), i > j, the terms -L(j,l)*L(i,l) for
-// l = 0 .. j-1 in ascending order -- from the Rsyrk 'L','N' of every node that has both
-// columns in its right half (l over that node's left half), then from the Rtrsm
-// right/lower/transposed of the node that splits them (l from the node's start to j) --
-// each skipped when the multiplier L(j,l) is zero (Rsyrk.cpp:169, Rtrsm.cpp:272), and
-// then the scale by 1/L(j,j) (Rtrsm.cpp:266).  The diagonal gets the same terms from the
-// Rsyrk calls alone, then the test and the sqrt of the leaf (Rpotrf2.cpp:90-97).  An
-// Rsyrk term c + (-m)*x and an Rtrsm term c - m*x round alike (posit rounding is
-// symmetric in sign), so the right-looking order below -- column j final, then its terms
-// applied to every later column -- computes the same bits.  The terms run in the GEMM's
-// FP64 multiply-add form with computed constants (fp64_mac_arith), each rounded as the
-// pattern-level operations round it as long as every product and sum stays in scales
-// -108 .. 107; the root and the reciprocal come from FP64 rounded to the posit grid
-// (fp64_posit_sqrt, fp64_posit_recip: the pattern-level operation decides where that could
-// round wrongly).  A result outside that range (NaN), a NaR, or a failing diagonal (the
-// state the recursion leaves after a failing leaf is not this one) makes the call return
-// -1 with the block unchanged; the caller then walks the recursion itself.  Returns 0
-// with the block written back otherwise.
-template <int T>
-static __device__ int potrf2_lower_decoded(int n, uint32_t* a, int lda) {
-  constexpr int SLOTS = (POTRF2_ONE_WG_MAX * (POTRF2_ONE_WG_MAX + 1) / 2 + T - 1) / T;
-  __shared__ double col[POTRF2_ONE_WG_MAX];
-  __shared__ double dg;
-  double c[SLOTS];  // posit values, exact in FP64 (NaR and out-of-range results: NaN)
-  int ri[SLOTS], ci[SLOTS];
-  {
-    int j = 0, off = 0;
-#pragma unroll
-    for (int q = 0; q < SLOTS; ++q) {
-      const int e = threadIdx.x + q * T;
-      while (j < n && off + (n - j) <= e) {
-        off += n - j;
-        ++j;
-      }
-      ri[q] = j + (e - off);
-      ci[q] = j < n ? j : -1;
-      c[q] = to_double(j < n ? a[ri[q] + (size_t)j * lda] : 0u);
-    }
-  }
-  if (threadIdx.x == 0) dg = to_double(a[0]);
-  __syncthreads();
-  for (int j = 0; j < n; ++j) {
-    const double v = dg;
-    if (!(v > 0.0)) return -1;  // uniform (zero, negative or NaR); the pattern-level form decides
-    const double d = fp64_posit_sqrt(v);
-    const double r = j + 1 < n ? fp64_posit_recip(d) : 1.0;
-#pragma unroll
-    for (int q = 0; q < SLOTS; ++q) {
-      if (ci[q] == j) {
-        if (ri[q] == j) {
-          c[q] = d;
-        } else {
-          c[q] = fp64_posit_mul_arith(r, c[q]);
-          col[ri[q]] = c[q];
-        }
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < SLOTS; ++q) {
-      if (ci[q] > j) {
-        const double m = col[ci[q]];
-        // a zero multiplier skips the term (Rsyrk.cpp:169, Rtrsm.cpp:272); a zero factor
-        // makes it zero, which leaves the sum unchanged
-        if (m != 0.0) c[q] = fp64_mac_arith(c[q], -m, col[ri[q]]);
-        if (ri[q] == j + 1 && ci[q] == j + 1) dg = c[q];  // NaN: redo
-      }
-    }
-    __syncthreads();
-  }
-  bool bad = false;
-#pragma unroll
-  for (int q = 0; q < SLOTS; ++q) bad |= c[q] != c[q];  // NaN: a scale left the table, or NaR
-  if (__syncthreads_or(bad)) return -1;
-#pragma unroll
-  for (int q = 0; q < SLOTS; ++q)
-    if (ci[q] >= 0) a[ri[q] + (size_t)ci[q] * lda] = from_double(c[q]);
-  return 0;
-}
-
-// The upper Rpotrf2 recursion (Rpotrf2.cpp:102-125) on an n x n block (n <= 64) held in
-// LDS as FP64 values, its upper triangle packed by columns (element (r, c), r <= c, at
-// c * (c + 1) / 2 + r): leaf j takes the root of its diagonal element (fp64_posit_sqrt);
-// the one node whose left half ends at j then solves its A12 (Rtrsm 'L','U','T','N',
-// Rtrsm.cpp:161-173: ascending k, then the quotient by u_ii, fp64_posit_div) and updates
-// the upper triangle of its A22 (Rsyrk 'U','T', Rsyrk.cpp:182-195: t = sum over the left
-// half's rows l of a_li * a_lj, formed from zero in ascending l, then c - t).  Products and
-// sums are rounded as in the GEMM (fp64_mac_arith).  Returns 0 with the block written
-// back, or -1 with the block untouched where a value left scales -108 .. 107, a NaR
-// appeared or a pivot is not positive: the pattern-level recursion then decides.
-constexpr int POTRF2_UPPER_PACKED = POTRF2_ONE_WG_MAX * (POTRF2_ONE_WG_MAX + 1) / 2;
-
-template <int T>
-static __device__ int potrf2_upper_decoded(int n, uint32_t* a, int lda, double* S) {
-  const int tid = threadIdx.x;
-  auto at = [&](int r, int c) -> double& { return S[c * (c + 1) / 2 + r]; };
-  for (int e = tid; e < n * n; e += T) {
-    const int r = e % n, c = e / n;
-    if (r <= c) at(r, c) = to_double(a[r + (size_t)c * lda]);
-  }
-  __syncthreads();
-  __shared__ int ok;
-  for (int j = 0; j < n; ++j) {
-    if (tid == 0) {  // the leaf: test and root (Rpotrf2.cpp:90-97)
-      const double v = at(j, j);
-      ok = v > 0.0;  // zero, negative or NaR: the pattern-level form decides
-      if (ok) at(j, j) = fp64_posit_sqrt(v);
-    }
-    __syncthreads();
-    if (!ok) return -1;  // uniform (every leaf but the last is followed by its node's
-                         // barriers before lane 0 writes ok again)
-    int s, mid, e;
-    if (wg::node_after(j, n, s, mid, e)) {
-      const int n2 = e - mid;
-      // A12 := U11^-T A12, rows s .. mid-1: row k is final once it has its terms from rows
-      // s .. k-1 and its quotient, so the lane that gives element (k+1, c) its last term
-      // forms that quotient at once (one barrier a row)
-      for (int c = mid + tid; c < e; c += T) at(s, c) = fp64_posit_div(at(s, c), at(s, s));
-      __syncthreads();
-      for (int k = s; k + 1 < mid; ++k) {
-        const int cnt = mid - 1 - k;
-        for (int q = tid; q < cnt * n2; q += T) {
-          const int i = k + 1 + q % cnt, c = mid + q / cnt;
-          const double x = fp64_mac_arith(at(i, c), -at(k, i), at(k, c));
-          at(i, c) = i == k + 1 ? fp64_posit_div(x, at(i, i)) : x;
-        }
-        __syncthreads();
-      }
-      for (int q = tid; q < n2 * n2; q += T) {  // A22 -= A12^T A12, upper triangle
-        const int i = mid + q % n2, jj = mid + q / n2;
-        if (i <= jj) {
-          double t = 0.0;
-          for (int l = s; l < mid; ++l) t = fp64_mac_arith(t, at(l, i), at(l, jj));
-          at(i, jj) = fp64_mac_arith(at(i, jj), -1.0, t);
-        }
-      }
-      __syncthreads();
-    }
-  }
-  bool bad = false;
-  for (int e = tid; e < n * n; e += T) {
-    const int r = e % n, c = e / n;
-    if (r <= c) bad |= at(r, c) != at(r, c);  // NaN: a scale left the range, or NaR
-  }
-  if (__syncthreads_or(bad)) return -1;
-  for (int e = tid; e < n * n; e += T) {
-    const int r = e % n, c = e / n;
-    if (r <= c) a[r + (size_t)c * lda] = from_double(at(r, c));
-  }
-  return 0;
-}
-
 // The same recursion on the block in memory with the pattern-level operations, one
 // element per lane: neither LDS nor many registers.  A failing step returns its 1-based
 // column and leaves the block as the reference leaves it.
@@ -960,7 +806,7 @@ __global__ __launch_bounds__(WG, 2) void k_potrf2_upper_fast(int n, uint32_t* a,
                                                              const uint32_t* __restrict__ St, int lds) {
   PGA_KERNEL_VGPR_FLOOR();
   PGA_URGENT();
-  __shared__ double S[POTRF2_UPPER_PACKED];
+  __shared__ double S[wg::UPPER_PACKED];
   if (*info) return;  // every lane reads the same flag
   if (St) {
     for (int e = threadIdx.x; e < n * n; e += WG) {
@@ -972,7 +818,7 @@ __global__ __launch_bounds__(WG, 2) void k_potrf2_upper_fast(int n, uint32_t* a,
     }
     __syncthreads();
   }
-  if (potrf2_upper_decoded<WG>(n, a, lda, S) == 0) return;
+  if (wg::potrf2_upper_decoded<WG>(n, a, lda, S) == 0) return;
   __syncthreads();
   const int f = potrf2_upper_plain<WG>(n, a, lda);
   if (f && threadIdx.x == 0) *info = col_off + f;
@@ -1009,7 +855,7 @@ __global__ __launch_bounds__(WG, 2) void k_potrf2_lower_fast(int n, uint32_t* a,
   PGA_KERNEL_VGPR_FLOOR();
   PGA_URGENT();
   if (*info) return;  // every lane reads the same flag
-  if (potrf2_lower_decoded<WG>(n, a, lda) == 0) return;
+  if (wg::potrf2_lower_decoded<WG>(n, a, lda) == 0) return;
   __syncthreads();
   const int f = potrf2_lower_plain(n, a, lda);
   if (f && threadIdx.x == 0) *info = col_off + f;
