@@ -266,15 +266,28 @@ __global__ __launch_bounds__(NT, (gemm::square_tile_wgs_per_cu<T>())) void k_bb_
 }
 
 // Solves, one workgroup per right-hand side, with the column staged in LDS:
-// Rgetrs 'N' (Rlaswp, L, U), Rpotrs upper (U^T, U) or lower (L, L^T).
-// 1024 lanes: each step of a sweep updates up to n rows of the column in LDS at once.
+// Rgetrs 'N' (Rlaswp, L, U), Rpotrs upper (U^T, U) or lower (L, L^T).  The L, U and U^T
+// sweeps run in FP64 with the column in registers (wg::col_solve_fp64, 64 rows of the
+// sweep per barrier), falling back to the pattern-level sweeps (1024 lanes: each step
+// updates up to n rows of the column in LDS at once) when a value leaves the FP64 range.
 constexpr int SOLVE_NT = 1024;
+static_assert(PGA_BLOCKED_TASK_MAX_N <= wg::COL_SOLVE_SLOTS * SOLVE_NT, "a column fits the solve's registers");
+
+template <int MODE>
+__device__ __forceinline__ void bb_sweep(int n, const uint32_t* a, int lda, uint32_t* col, wg::lds_f64* sb) {
+  if (wg::col_solve_fp64<MODE, SOLVE_NT>(n, a, lda, (wg::lds_u32*)col, sb)) return;
+  if (MODE == wg::CS_LLNU) wg::trsm_llnu<SOLVE_NT>(n, 1, a, lda, col, n);
+  else if (MODE == wg::CS_LLNN) wg::trsm_llnn<SOLVE_NT>(n, 1, a, lda, col, n);
+  else if (MODE == wg::CS_LUNN) wg::trsm_lunn<SOLVE_NT>(n, 1, a, lda, col, n);
+  else wg::trsm_lutn<SOLVE_NT>(n, 1, a, lda, col, n);
+}
 
 __global__ __launch_bounds__(SOLVE_NT) void k_bb_solve(const FactorTask* __restrict__ tasks,
                                                        const Item* __restrict__ items) {
   PGA_KERNEL_VGPR_FLOOR();
   PGA_URGENT();
   extern __shared__ uint32_t col[];
+  __shared__ double sb[128];
   const Item it = items[blockIdx.x];
   const FactorTask& t = tasks[it.task];
   const bool factor = (t.kind == PGA_TASK_GESV || t.kind == PGA_TASK_POSV);
@@ -283,25 +296,32 @@ __global__ __launch_bounds__(SOLVE_NT) void k_bb_solve(const FactorTask* __restr
   uint32_t* b = t.b + (size_t)it.off * t.ldb;
   for (int i = threadIdx.x; i < n; i += SOLVE_NT) col[i] = b[i];
   __syncthreads();
+  wg::lds_f64* sbl = (wg::lds_f64*)sb;
   if (is_lu(t.kind)) {
-    if (threadIdx.x == 0) {  // Rlaswp(1, n, ipiv, 1): the interchanges in order
-      for (int i = 0; i < n; ++i) {
-        const int p = t.ipiv[i] - 1;
-        if (p != i) {
-          const uint32_t x = col[i];
-          col[i] = col[p];
-          col[p] = x;
+    // Rlaswp(1, n, ipiv, 1): the interchanges in order by lane 0, wave 0 reading the
+    // pivots 64 at a time
+    if (threadIdx.x < 64) {
+      for (int i0 = 0; i0 < n; i0 += 64) {
+        const int p = i0 + (int)threadIdx.x < n ? t.ipiv[i0 + threadIdx.x] - 1 : 0;
+#pragma unroll
+        for (int j = 0; j < 64; ++j) {
+          const int i = i0 + j, pj = __shfl(p, j);
+          if (threadIdx.x == 0 && i < n && pj != i) {
+            const uint32_t x = col[i];
+            col[i] = col[pj];
+            col[pj] = x;
+          }
         }
       }
     }
     __syncthreads();
-    wg::trsm_llnu<SOLVE_NT>(n, 1, t.a, t.lda, col, n);
-    wg::trsm_lunn<SOLVE_NT>(n, 1, t.a, t.lda, col, n);
+    bb_sweep<wg::CS_LLNU>(n, t.a, t.lda, col, sbl);
+    bb_sweep<wg::CS_LUNN>(n, t.a, t.lda, col, sbl);
   } else if (t.uplo == 'U') {
-    wg::trsm_lutn<SOLVE_NT>(n, 1, t.a, t.lda, col, n);
-    wg::trsm_lunn<SOLVE_NT>(n, 1, t.a, t.lda, col, n);
+    bb_sweep<wg::CS_LUTN>(n, t.a, t.lda, col, sbl);
+    bb_sweep<wg::CS_LUNN>(n, t.a, t.lda, col, sbl);
   } else {
-    wg::trsm_llnn<SOLVE_NT>(n, 1, t.a, t.lda, col, n);
+    bb_sweep<wg::CS_LLNN>(n, t.a, t.lda, col, sbl);
     wg::trsm_lltn<SOLVE_NT>(n, 1, t.a, t.lda, col, n);
   }
   for (int i = threadIdx.x; i < n; i += SOLVE_NT) b[i] = col[i];

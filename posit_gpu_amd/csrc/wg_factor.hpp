@@ -283,6 +283,140 @@ static __device__ __noinline__ void trsm_waves(int m, int n, const uint32_t* A, 
   __syncthreads();
 }
 
+// One right-hand side of n rows solved by all T lanes of the block in FP64, the forms
+// of trsm_llnu, trsm_llnn, trsm_lunn and trsm_lutn with one column: row i is held by
+// lane i % T in register slot i / T.  The sweep goes 64 rows at a time: the wave holding
+// a block's rows runs the block's steps with shuffles (as trsm_waves does) and leaves
+// the block's final values in LDS (sb: 2 x 64 values, alternating); after one barrier
+// every lane applies the block's terms to its rows beyond the block.  Every row so
+// receives its terms in the pattern-level sweep's k order (ascending; descending for
+// the upper form), zero values of b_k skipping their terms in the same places, each
+// term rounded as the pattern-level operations round it (fp64_mac_arith,
+// fp64_posit_div).  col holds the column as patterns (LDS) and receives the solution;
+// when some value leaves scales -108 .. 107 or meets a NaR or a zero divisor (a NaN in
+// some lane) the routine returns false with col untouched, and the caller runs the
+// pattern-level sweep.  n <= COL_SOLVE_SLOTS * T.
+enum ColSolve { CS_LLNU, CS_LLNN, CS_LUNN, CS_LUTN };
+constexpr int COL_SOLVE_SLOTS = 16;
+typedef __attribute__((address_space(3))) double lds_f64;
+// the slots as one register vector: a slot chosen by a wave-uniform index is a register
+// move, not a memory access
+typedef double col_slots __attribute__((ext_vector_type(COL_SOLVE_SLOTS)));
+
+// x of lane k, k the same in every lane
+static __device__ __forceinline__ double lane_value(double x, int k) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, x);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, k);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), k);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+// b / a at the pattern level, for quotients that land on a posit midpoint in FP64
+static __device__ __noinline__ double col_div_exact(double b, double a) {
+  return to_double(div(from_double(b), from_double(a)));
+}
+
+// fp64_posit_div with its rare exact case out of line
+static __device__ __forceinline__ double col_div(double b, double a) {
+  const double q = b / a;
+  const double c = fp64_grid_c_arith(q);
+  const double r = (q + c) - c;
+  if (__builtin_fabs(q - r) == fp64_grid_half(c)) return col_div_exact(b, a);
+  return r;
+}
+
+template <int MODE, int T>
+static __device__ __noinline__ bool col_solve_fp64(int n, const uint32_t* __restrict__ A, int lda, lds_u32* col,
+                                                   lds_f64* sb) {
+  constexpr int S = COL_SOLVE_SLOTS;
+  constexpr bool BACK = MODE == CS_LUNN;
+  static_assert(T % 64 == 0, "whole waves");
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  col_slots b;
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const int i = s * T + tid;
+    b[s] = i < n ? to_double(col[i]) : 0.0;
+  }
+  // a(r, c) of the sweep: LUTN reads u_kr (row k of U), the others l_rk / u_rk
+  auto entry = [&](int r, int c) -> uint32_t {
+    return MODE == CS_LUTN ? A[c + (size_t)r * lda] : A[r + (size_t)c * lda];
+  };
+  const int nblk = (n + 63) / 64;
+  for (int q = 0; q < nblk; ++q) {
+    const int kb = BACK ? nblk - 1 - q : q;
+    const int k0 = kb * 64, m = min(64, n - k0);
+    lds_f64* sk = sb + (q & 1) * 64;
+    if (wave == (k0 % T) / 64) {  // the block's own steps
+      const int hs = k0 / T;
+      double x = b[hs];
+      const bool row_in = lane < m;
+      for (int h = 0; h < 4; ++h) {
+        const int kc = BACK ? 48 - 16 * h : 16 * h;  // 16 steps' entries at a time
+        uint32_t av[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) av[j] = (row_in && kc + j < m) ? entry(k0 + lane, k0 + kc + j) : 0u;
+#pragma unroll
+        for (int jj = 0; jj < 16; ++jj) {
+          const int j = BACK ? 15 - jj : jj, k = kc + j;
+          if (k >= m) continue;  // uniform
+          if (MODE == CS_LUTN) {
+            if (lane == k) x = col_div(x, to_double(av[j]));
+            const double bk = lane_value(x, k);
+            if (lane > k && row_in) x = fp64_mac_arith(x, -to_double(av[j]), bk);
+          } else {
+            double bk = lane_value(x, k);
+            if (bk == 0.0) continue;  // the same value in every lane
+            if (MODE != CS_LLNU) bk = col_div(bk, lane_value(to_double(av[j]), k));
+            if ((BACK ? lane < k : lane > k) && row_in) x = fp64_mac_arith(x, -bk, to_double(av[j]));
+            if (lane == k) x = bk;
+          }
+        }
+      }
+      b[hs] = x;
+      if (row_in) sk[lane] = x;
+    }
+    __syncthreads();
+    // the block's terms on the rows beyond it: [k0 + m, n) forward, [0, k0) upward
+    const int lo = BACK ? 0 : k0 + m, hi = BACK ? k0 : n;
+    if (lo >= hi) continue;
+    for (int s = lo / T; s <= (hi - 1) / T; ++s) {
+      const int i = s * T + tid;
+      if (i < lo || i >= hi) continue;
+      double x = b[s];
+      for (int kc = 0; kc < m; kc += 16) {
+        uint32_t av[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+          const int t = kc + j, k = BACK ? m - 1 - t : t;
+          av[j] = t < m ? entry(i, k0 + k) : 0u;
+        }
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+          const int t = kc + j;
+          if (t >= m) continue;  // uniform
+          const double bk = sk[BACK ? m - 1 - t : t];
+          if (MODE == CS_LUTN) x = fp64_mac_arith(x, -to_double(av[j]), bk);
+          else if (bk != 0.0) x = fp64_mac_arith(x, -bk, to_double(av[j]));
+        }
+      }
+      b[s] = x;
+    }
+  }
+  bool bad = false;
+#pragma unroll
+  for (int s = 0; s < S; ++s)
+    if (s * T + tid < n) bad |= b[s] != b[s];
+  if (__syncthreads_or(bad)) return false;
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const int i = s * T + tid;
+    if (i < n) col[i] = from_double(b[s]);
+  }
+  __syncthreads();
+  return true;
+}
+
 // ---------------------------------------------------------------------------------
 // LU (Rgetrf_opencl / Rgetrf2)
 // ---------------------------------------------------------------------------------

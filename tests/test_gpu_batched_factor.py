@@ -265,6 +265,32 @@ def test_blocked_solve_only_and_failures(pga, oracle):
     _check_plan(pga, cases)
 
 
+def test_blocked_solves_registers_and_fallback(pga, oracle):
+    """The blocked path's solves hold a right-hand side in FP64 registers, 1024 rows per
+    register slot, and redo a sweep at the pattern level when a value leaves the FP64
+    range: orders over several slots, ends that are not whole 64-row blocks, and columns
+    with a NaR, a huge value (2^100: products out of range) and all zeros."""
+    cases = []
+    for i, (kind, n, uplo) in enumerate((("getrs", 2100, "L"), ("potrs", 1500, "U"), ("potrs", 1100, "L"),
+                                         ("getrs", 1025, "L"), ("potrs", 650, "U"))):
+        A = posit_matrix(oracle, 1400 + i, n, n) if kind == "getrs" else _spd(oracle, 1400 + i, n)
+        F = pga.DeviceArray.from_host(A.copy(order="F"))
+        inf = pga.DeviceArray(1, np.int32, "C")
+        ip = pga.DeviceArray(n, np.int32, "C")
+        if kind == "getrs":
+            pga.getrf_device(n, n, F, n, ip, inf, 64)
+        else:
+            pga.potrf_device(uplo, n, F, n, inf, 64)
+        pga.synchronize()
+        assert int(inf.to_host()[0]) == 0
+        B = posit_matrix(oracle, 1450 + i, n, 4)
+        B[n // 3, 1] = NAR
+        B[n - 2, 2] = oracle.from_double(2.0 ** 100)
+        B[:, 3] = 0
+        cases.append(Case(pga, kind, A, uplo, B=B, factored=(F.to_host(), ip.to_host() if kind == "getrs" else None)))
+    _check_plan(pga, cases)
+
+
 def test_blocked_graph_replays(pga, oracle):
     cases = [Case(pga, "gesv", posit_matrix(oracle, 1300, 640, 640), nb=0, B=posit_matrix(oracle, 1301, 640, 2)),
              Case(pga, "posv", _spd(oracle, 1302, 560), "U", 0, B=posit_matrix(oracle, 1303, 560, 1))]
