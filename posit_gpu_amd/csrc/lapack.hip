@@ -542,6 +542,35 @@ __global__ __launch_bounds__(WG) void k_trsm_lutn_fp64(int m, int n, const uint3
   for (int e = tid; e < m * C; e += WG) b[e % m + (size_t)(e / m) * ldb] = from_double(sB[e % m][e / m]);
 }
 
+// Left solves with alpha = 1 of up to TRSM_COL_MAX rows, one 1024-lane workgroup per
+// column of B: B := L^-1 B (unit or not), U^-1 B or U^-T B (non-unit), the column staged
+// in LDS and solved in FP64 registers 64 rows of the sweep per barrier
+// (wg::col_solve_fp64: the Rtrsm.cpp:120-173 loop orders and zero skips, term for term);
+// a column that leaves the FP64 range is solved again at the pattern level.  One column's
+// work stays on one CU, so the per-call solves use it for U^T only (whose pattern-level
+// sweep, k_trsm_left<true, true>, has the same one-workgroup-per-column shape).
+constexpr int TRSM_COL_NT = 1024, TRSM_COL_MAX = wg::COL_SOLVE_SLOTS * TRSM_COL_NT;
+
+template <int MODE>
+__global__ __launch_bounds__(TRSM_COL_NT) void k_trsm_col(int m, const uint32_t* __restrict__ A, int lda,
+                                                          uint32_t* __restrict__ B, int ldb, const int* stop) {
+  PGA_KERNEL_VGPR_FLOOR();
+  PGA_URGENT();
+  extern __shared__ uint32_t col[];
+  __shared__ double sb[128];
+  if (stopped(stop)) return;
+  uint32_t* b = B + (size_t)blockIdx.x * ldb;
+  for (int i = threadIdx.x; i < m; i += TRSM_COL_NT) col[i] = b[i];
+  __syncthreads();
+  if (!wg::col_solve_fp64<MODE, TRSM_COL_NT>(m, A, lda, (wg::lds_u32*)col, (wg::lds_f64*)sb)) {
+    if (MODE == wg::CS_LLNU) wg::trsm_llnu<TRSM_COL_NT>(m, 1, A, lda, col, m);
+    else if (MODE == wg::CS_LLNN) wg::trsm_llnn<TRSM_COL_NT>(m, 1, A, lda, col, m);
+    else if (MODE == wg::CS_LUNN) wg::trsm_lunn<TRSM_COL_NT>(m, 1, A, lda, col, m);
+    else wg::trsm_lutn<TRSM_COL_NT>(m, 1, A, lda, col, m);
+  }
+  for (int i = threadIdx.x; i < m; i += TRSM_COL_NT) b[i] = col[i];
+}
+
 // B = 0 (alpha == 0, Rtrsm.cpp:101-108)
 __global__ __launch_bounds__(WG) void k_zero(int m, int n, uint32_t* B, int ldb, const int* stop) {
   PGA_KERNEL_VGPR_FLOOR();
@@ -1041,6 +1070,14 @@ static hipError_t trsm(const Ctx& c, bool left, bool upper, bool trans, bool nou
                        uint32_t alpha, const uint32_t* A, int lda, uint32_t* B, int ldb,
                        const int* stop) {
   if (m == 0 || n == 0) return hipSuccess;
+  // U^T B with more rows than a diagonal block: one workgroup per right-hand side in
+  // FP64 (k_trsm_col).  The other left forms keep their row-block paths, whose updates
+  // spread one column's work over the whole device.
+  if (left && upper && trans && nounit && alpha == P_ONE && m > TRSM_DIAG_MAX && m <= TRSM_COL_MAX) {
+    hipLaunchKernelGGL(k_trsm_col<wg::CS_LUTN>, dim3(n), dim3(TRSM_COL_NT), (size_t)m * sizeof(uint32_t), c.s, m, A,
+                       lda, B, ldb, stop);
+    return hipGetLastError();
+  }
   if (left && !upper && !trans && alpha != 0 && m > 2 * TRSM_BLOCK)
     return trsm_lln_blocked(c, nounit, m, n, alpha, A, lda, B, ldb, stop);
   if (left && upper && !trans && alpha != 0 && m > TRSM_DIAG_MAX && trsm_narrow(n))

@@ -326,11 +326,13 @@ static __device__ __forceinline__ double col_div(double b, double a) {
 }
 
 template <int MODE, int T>
-static __device__ __noinline__ bool col_solve_fp64(int n, const uint32_t* __restrict__ A, int lda, lds_u32* col,
+static __device__ __noinline__ bool col_solve_fp64(int n, const uint32_t* __restrict__ Ag, int lda, lds_u32* col,
                                                    lds_f64* sb) {
   constexpr int S = COL_SOLVE_SLOTS;
   constexpr bool BACK = MODE == CS_LUNN;
   static_assert(T % 64 == 0, "whole waves");
+  typedef const __attribute__((address_space(1))) uint32_t gbl_u32;  // global loads, counted apart from LDS
+  gbl_u32* A = (gbl_u32*)Ag;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   col_slots b;
 #pragma unroll

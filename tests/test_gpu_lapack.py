@@ -192,6 +192,37 @@ def test_trsm_left_lower_trans_chain_classes(pga, oracle, kind, diag):
     _same(got, want)
 
 
+@pytest.mark.parametrize("form", [("L", "N", "U"), ("L", "N", "N"), ("U", "N", "N"), ("U", "T", "N")])
+@pytest.mark.parametrize("kind", ["plain", "special", "range"])
+def test_trsm_left_columns_fp64(pga, oracle, form, kind):
+    """Left solves with alpha = 1, more rows than a diagonal block and few right-hand
+    sides: one workgroup per column in FP64 registers (1024 rows per register slot, 64
+    rows of the sweep per barrier), a column redone at the pattern level when a value
+    leaves the FP64 range.  1100 rows: two slots and a last block of 12 rows; zero
+    entries of B (skipped steps), NaR in the triangle and in B, values near the ends of
+    the posit range."""
+    uplo, trans, diag = form
+    m, n = 1100, 5
+    A = _tri(oracle, 90, m, uplo == "U")
+    B = posit_matrix(oracle, 91, m, n)
+    B[40, :] = 0
+    B[1090, ::2] = 0
+    B[:, 4] = 0
+    B[700, 4] = posit_matrix(oracle, 92, 1, 1)[0, 0]
+    if kind == "special":
+        r, c = (900, 130) if uplo == "L" else (130, 900)
+        A[r, c] = NAR
+        B[555, 1] = NAR
+    elif kind == "range":
+        B[1000, 2] = oracle.from_double(2.0 ** 100)
+        B[10, 3] = oracle.from_double(2.0 ** -100)
+    want = B.copy(order="F")
+    assert oracle.rtrsm("L", uplo, trans, diag, m, n, ONE, A, want) == 0
+    got = B.copy(order="F")
+    pga.Rtrsm("L", uplo, trans, diag, m, n, ONE, A, m, got, m)
+    _same(got, want)
+
+
 @pytest.mark.parametrize("uplo", ["U", "L"])
 @pytest.mark.parametrize("diag", ["N", "U"])
 def test_trsm_left_wide(pga, oracle, uplo, diag):
