@@ -194,24 +194,53 @@ __global__ __launch_bounds__(NT, (gemm::square_tile_wgs_per_cu<T>())) void k_bb_
   }
 }
 
-// The interchanges of every later block on LU columns [off, off + cnt), one lane per
-// column (Rgetrf_opencl.cpp:123 applies them block by block to the columns on the left;
-// item.j is the task's block size).
+// The interchanges of every later block on the LU columns of one block, [off, off + cnt)
+// (Rgetrf_opencl.cpp:123 applies them block by block to the columns on the left; item.j
+// is the task's block size).  The columns of a block all take the same interchanges,
+// rows k0 = (off / j + 1) * j .. n - 1 in order: lane 0 composes them once into a
+// permutation of those rows (LDS), and every column is then permuted by a gather.
+constexpr int LASWP_LEFT_REGS = PGA_BLOCKED_TASK_MAX_N / NT;  // gathered rows per lane
+static_assert(PGA_BLOCKED_TASK_MAX_N <= 65536 && PGA_BLOCKED_TASK_MAX_N % NT == 0, "16-bit row indices");
+
 __global__ __launch_bounds__(NT) void k_bb_laswp_left(const FactorTask* __restrict__ tasks,
                                                       const Item* __restrict__ items) {
   PGA_KERNEL_VGPR_FLOOR();
   PGA_URGENT();
+  extern __shared__ uint16_t perm[];  // row k0 + r of the result is row k0 + perm[r] before
   const Item it = items[blockIdx.x];
   const FactorTask& t = tasks[it.task];
-  if (threadIdx.x >= it.cnt) return;
-  const int c = it.off + threadIdx.x, nb = it.j;
-  uint32_t* col = t.a + (size_t)c * t.lda;
-  for (int k = (c / nb + 1) * nb; k < t.n; ++k) {
-    const int p = t.ipiv[k] - 1;
-    if (p != k) {
-      const uint32_t x = col[k];
-      col[k] = col[p];
-      col[p] = x;
+  const int tid = threadIdx.x, nb = it.j, k0 = (it.off / nb + 1) * nb, R = t.n - k0;
+  if (R <= 0) return;
+  for (int r = tid; r < R; r += NT) perm[r] = (uint16_t)r;
+  __syncthreads();
+  if (tid < 64) {  // wave 0 reads the pivots 64 at a time, lane 0 applies them in order
+    for (int i0 = k0; i0 < t.n; i0 += 64) {
+      const int p = i0 + tid < t.n ? t.ipiv[i0 + tid] - 1 : 0;
+#pragma unroll
+      for (int j = 0; j < 64; ++j) {
+        const int k = i0 + j, pk = __shfl(p, j);
+        if (tid == 0 && k < t.n && pk != k && pk >= k0 && pk < t.n) {
+          const uint16_t x = perm[k - k0];
+          perm[k - k0] = perm[pk - k0];
+          perm[pk - k0] = x;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int c = it.off; c < it.off + it.cnt; ++c) {
+    uint32_t* col = t.a + (size_t)c * t.lda + k0;
+    uint32_t v[LASWP_LEFT_REGS];
+#pragma unroll
+    for (int q = 0; q < LASWP_LEFT_REGS; ++q) {
+      const int r = q * NT + tid;
+      v[q] = r < R ? col[perm[r]] : 0u;
+    }
+    __syncthreads();  // the column's reads are done before any lane writes it
+#pragma unroll
+    for (int q = 0; q < LASWP_LEFT_REGS; ++q) {
+      const int r = q * NT + tid;
+      if (r < R) col[r] = v[q];
     }
   }
 }
@@ -347,7 +376,7 @@ struct BlockedGroup {
   std::vector<unsigned char> image;
   std::vector<Step> steps;
   size_t tasks = 0, solve = 0, counters = 0, spare_info = 0, swaps = 0;
-  int ntasks = 0, n_solve = 0, ncounters = 0, max_solve_n = 0, n_swaps = 0, node_counter_base = 0;
+  int ntasks = 0, n_solve = 0, ncounters = 0, max_solve_n = 0, n_swaps = 0, swap_rows = 0, node_counter_base = 0;
   unsigned char* dev = nullptr;  // device copy of the image
 };
 
@@ -545,7 +574,8 @@ BlockedGroup* group_build(const FactorTask* tasks, int count) {
     const FactorTask& t = tasks[i];
     if (!lu_kind(t.kind) || steps_of[i] < 2) continue;
     const int last = (steps_of[i] - 1) * nbt[i];  // columns of the last block take nothing
-    for (int c = 0; c < last; c += bb::NT) swaps.push_back({i, nbt[i], 0, c, std::min(bb::NT, last - c)});
+    for (int c = 0; c < last; c += nbt[i]) swaps.push_back({i, nbt[i], 0, c, nbt[i]});
+    p->swap_rows = std::max(p->swap_rows, t.n - nbt[i]);
   }
   p->swaps = put(p->image, swaps);
   p->n_swaps = (int)swaps.size();
@@ -625,7 +655,7 @@ hipError_t group_launch(const BlockedGroup* p, hipStream_t s) {
                          reinterpret_cast<const Item*>(d + st.trsm_ch));
   }
   if (p->n_swaps > 0)
-    hipLaunchKernelGGL(k_bb_laswp_left, dim3(p->n_swaps), dim3(NT), 0, s, tasks,
+    hipLaunchKernelGGL(k_bb_laswp_left, dim3(p->n_swaps), dim3(NT), (size_t)p->swap_rows * sizeof(uint16_t), s, tasks,
                        reinterpret_cast<const Item*>(d + p->swaps));
   if (p->n_solve > 0)
     hipLaunchKernelGGL(k_bb_solve, dim3(p->n_solve), dim3(SOLVE_NT), (size_t)p->max_solve_n * 4, s, tasks,
