@@ -427,7 +427,8 @@ pga_status run_blocked(pga_server* S, const std::vector<Request*>& batch, hipStr
 
 // Larger requests.  Plain mode: one request at a time through the per-call drivers.
 // Batched mode: every ready large request (up to max_batch) runs as one blocked batch;
-// requests beyond the blocked path's largest order run alone.
+// requests beyond the blocked path's largest order, and a request that is ready alone,
+// run through the per-call drivers.
 void worker_main(pga_server* S, hipStream_t s) {
   (void)hipSetDevice(runtime().device);
   PlanBuf buf;
@@ -468,6 +469,16 @@ void worker_main(pga_server* S, hipStream_t s) {
       } else {
         ++it;
       }
+    }
+    if (batch.size() == 1) {  // alone: the per-call drivers (same bits, no plan to build)
+      Request* q = batch[0];
+      lk.unlock();
+      int info = 0;
+      pga_status st = S->inject_failure() ? PGA_ERROR_HIP : run_alone(S, q, s, &info);
+      if (st != PGA_SUCCESS) (void)hipStreamSynchronize(s);
+      lk.lock();
+      finish(S, q, st == PGA_SUCCESS ? 0 : PGA_REQ_FAILED, info);
+      continue;
     }
     S->stats.batches++;
     S->stats.batched_requests += (int64_t)batch.size();
