@@ -324,6 +324,7 @@ __global__ __launch_bounds__(SOLVE_NT) void k_bb_solve(const FactorTask* __restr
   const int n = t.n;
   uint32_t* b = t.b + (size_t)it.off * t.ldb;
   for (int i = threadIdx.x; i < n; i += SOLVE_NT) col[i] = b[i];
+  gemm::round_table_fill(threadIdx.x, SOLVE_NT);
   __syncthreads();
   wg::lds_f64* sbl = (wg::lds_f64*)sb;
   if (is_lu(t.kind)) {

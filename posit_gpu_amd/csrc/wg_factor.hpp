@@ -291,8 +291,9 @@ static __device__ __noinline__ void trsm_waves(int m, int n, const uint32_t* A, 
 // every lane applies the block's terms to its rows beyond the block.  Every row so
 // receives its terms in the pattern-level sweep's k order (ascending; descending for
 // the upper form), zero values of b_k skipping their terms in the same places, each
-// term rounded as the pattern-level operations round it (fp64_mac_arith,
-// fp64_posit_div).  col holds the column as patterns (LDS) and receives the solution;
+// term rounded as the pattern-level operations round it (gemm::fp64_mac, its constants
+// read from gemm::round_table, which the calling kernel fills first; fp64_posit_div).
+// col holds the column as patterns (LDS) and receives the solution;
 // when some value leaves scales -108 .. 107 or meets a NaR or a zero divisor (a NaN in
 // some lane) the routine returns false with col untouched, and the caller runs the
 // pattern-level sweep.  n <= COL_SOLVE_SLOTS * T.
@@ -365,12 +366,12 @@ static __device__ __noinline__ bool col_solve_fp64(int n, const uint32_t* __rest
           if (MODE == CS_LUTN) {
             if (lane == k) x = col_div(x, to_double(av[j]));
             const double bk = lane_value(x, k);
-            if (lane > k && row_in) x = fp64_mac_arith(x, -to_double(av[j]), bk);
+            if (lane > k && row_in) x = gemm::fp64_mac(x, -to_double(av[j]), bk);
           } else {
             double bk = lane_value(x, k);
             if (bk == 0.0) continue;  // the same value in every lane
             if (MODE != CS_LLNU) bk = col_div(bk, lane_value(to_double(av[j]), k));
-            if ((BACK ? lane < k : lane > k) && row_in) x = fp64_mac_arith(x, -bk, to_double(av[j]));
+            if ((BACK ? lane < k : lane > k) && row_in) x = gemm::fp64_mac(x, -bk, to_double(av[j]));
             if (lane == k) x = bk;
           }
         }
@@ -398,8 +399,8 @@ static __device__ __noinline__ bool col_solve_fp64(int n, const uint32_t* __rest
           const int t = kc + j;
           if (t >= m) continue;  // uniform
           const double bk = sk[BACK ? m - 1 - t : t];
-          if (MODE == CS_LUTN) x = fp64_mac_arith(x, -to_double(av[j]), bk);
-          else if (bk != 0.0) x = fp64_mac_arith(x, -bk, to_double(av[j]));
+          if (MODE == CS_LUTN) x = gemm::fp64_mac(x, -to_double(av[j]), bk);
+          else if (bk != 0.0) x = gemm::fp64_mac(x, -bk, to_double(av[j]));
         }
       }
       b[s] = x;
