@@ -294,7 +294,7 @@ __global__ __launch_bounds__(NT, (gemm::square_tile_wgs_per_cu<T>())) void k_bb_
   }
 }
 
-// Solves, one workgroup per right-hand side, with the column staged in LDS:
+// Solves, one workgroup per right-hand side or pair of them, the columns staged in LDS:
 // Rgetrs 'N' (Rlaswp, L, U), Rpotrs upper (U^T, U) or lower (L, L^T).  The L, U and U^T
 // sweeps run in FP64 with the column in registers (wg::col_solve_fp64, 64 rows of the
 // sweep per barrier), falling back to the pattern-level sweeps (1024 lanes: each step
@@ -302,31 +302,18 @@ __global__ __launch_bounds__(NT, (gemm::square_tile_wgs_per_cu<T>())) void k_bb_
 constexpr int SOLVE_NT = 1024;
 static_assert(PGA_BLOCKED_TASK_MAX_N <= wg::COL_SOLVE_SLOTS * SOLVE_NT, "a column fits the solve's registers");
 
-template <int MODE>
+template <int MODE, int C>
 __device__ __forceinline__ void bb_sweep(int n, const uint32_t* a, int lda, uint32_t* col, wg::lds_f64* sb) {
-  if (wg::col_solve_fp64<MODE, SOLVE_NT>(n, a, lda, (wg::lds_u32*)col, sb)) return;
-  if (MODE == wg::CS_LLNU) wg::trsm_llnu<SOLVE_NT>(n, 1, a, lda, col, n);
-  else if (MODE == wg::CS_LLNN) wg::trsm_llnn<SOLVE_NT>(n, 1, a, lda, col, n);
-  else if (MODE == wg::CS_LUNN) wg::trsm_lunn<SOLVE_NT>(n, 1, a, lda, col, n);
-  else wg::trsm_lutn<SOLVE_NT>(n, 1, a, lda, col, n);
+  if (wg::col_solve_fp64<MODE, SOLVE_NT, C>(n, a, lda, (wg::lds_u32*)col, n, sb)) return;
+  if (MODE == wg::CS_LLNU) wg::trsm_llnu<SOLVE_NT>(n, C, a, lda, col, n);
+  else if (MODE == wg::CS_LLNN) wg::trsm_llnn<SOLVE_NT>(n, C, a, lda, col, n);
+  else if (MODE == wg::CS_LUNN) wg::trsm_lunn<SOLVE_NT>(n, C, a, lda, col, n);
+  else wg::trsm_lutn<SOLVE_NT>(n, C, a, lda, col, n);
 }
 
-__global__ __launch_bounds__(SOLVE_NT) void k_bb_solve(const FactorTask* __restrict__ tasks,
-                                                       const Item* __restrict__ items) {
-  PGA_KERNEL_VGPR_FLOOR();
-  PGA_URGENT();
-  extern __shared__ uint32_t col[];
-  __shared__ double sb[128];
-  const Item it = items[blockIdx.x];
-  const FactorTask& t = tasks[it.task];
-  const bool factor = (t.kind == PGA_TASK_GESV || t.kind == PGA_TASK_POSV);
-  if (factor && *t.info) return;  // factor-and-solve tasks solve only after success
-  const int n = t.n;
-  uint32_t* b = t.b + (size_t)it.off * t.ldb;
-  for (int i = threadIdx.x; i < n; i += SOLVE_NT) col[i] = b[i];
-  gemm::round_table_fill(threadIdx.x, SOLVE_NT);
-  __syncthreads();
-  wg::lds_f64* sbl = (wg::lds_f64*)sb;
+// the solves of C columns held in col (LDS, n apart)
+template <int C>
+__device__ __forceinline__ void bb_solve_cols(const FactorTask& t, int n, uint32_t* col, wg::lds_f64* sb) {
   if (is_lu(t.kind)) {
     // Rlaswp(1, n, ipiv, 1): the interchanges in order by lane 0, wave 0 reading the
     // pivots 64 at a time
@@ -337,24 +324,51 @@ __global__ __launch_bounds__(SOLVE_NT) void k_bb_solve(const FactorTask* __restr
         for (int j = 0; j < 64; ++j) {
           const int i = i0 + j, pj = __shfl(p, j);
           if (threadIdx.x == 0 && i < n && pj != i) {
-            const uint32_t x = col[i];
-            col[i] = col[pj];
-            col[pj] = x;
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+              const uint32_t x = col[c * n + i];
+              col[c * n + i] = col[c * n + pj];
+              col[c * n + pj] = x;
+            }
           }
         }
       }
     }
     __syncthreads();
-    bb_sweep<wg::CS_LLNU>(n, t.a, t.lda, col, sbl);
-    bb_sweep<wg::CS_LUNN>(n, t.a, t.lda, col, sbl);
+    bb_sweep<wg::CS_LLNU, C>(n, t.a, t.lda, col, sb);
+    bb_sweep<wg::CS_LUNN, C>(n, t.a, t.lda, col, sb);
   } else if (t.uplo == 'U') {
-    bb_sweep<wg::CS_LUTN>(n, t.a, t.lda, col, sbl);
-    bb_sweep<wg::CS_LUNN>(n, t.a, t.lda, col, sbl);
+    bb_sweep<wg::CS_LUTN, C>(n, t.a, t.lda, col, sb);
+    bb_sweep<wg::CS_LUNN, C>(n, t.a, t.lda, col, sb);
   } else {
-    bb_sweep<wg::CS_LLNN>(n, t.a, t.lda, col, sbl);
-    wg::trsm_lltn<SOLVE_NT>(n, 1, t.a, t.lda, col, n);
+    bb_sweep<wg::CS_LLNN, C>(n, t.a, t.lda, col, sb);
+    wg::trsm_lltn<SOLVE_NT>(n, C, t.a, t.lda, col, n);
   }
-  for (int i = threadIdx.x; i < n; i += SOLVE_NT) b[i] = col[i];
+}
+
+// Items: (task, first column off, cnt = 1 or SOLVE_PAIR columns); a pair's columns share
+// each factor entry's decoding (orders up to SOLVE_PAIR_MAX_N)
+constexpr int SOLVE_PAIR = 2, SOLVE_PAIR_MAX_N = wg::COL_SOLVE_SLOTS / SOLVE_PAIR * SOLVE_NT;
+
+__global__ __launch_bounds__(SOLVE_NT) void k_bb_solve(const FactorTask* __restrict__ tasks,
+                                                       const Item* __restrict__ items) {
+  PGA_KERNEL_VGPR_FLOOR();
+  PGA_URGENT();
+  extern __shared__ uint32_t col[];  // cnt columns, n apart
+  __shared__ double sb[SOLVE_PAIR * 128];
+  const Item it = items[blockIdx.x];
+  const FactorTask& t = tasks[it.task];
+  const bool factor = (t.kind == PGA_TASK_GESV || t.kind == PGA_TASK_POSV);
+  if (factor && *t.info) return;  // factor-and-solve tasks solve only after success
+  const int n = t.n, cnt = it.cnt;
+  uint32_t* b = t.b + (size_t)it.off * t.ldb;
+  for (int e = threadIdx.x; e < n * cnt; e += SOLVE_NT) col[e] = b[e % n + (size_t)(e / n) * t.ldb];
+  gemm::round_table_fill(threadIdx.x, SOLVE_NT);
+  __syncthreads();
+  wg::lds_f64* sbl = (wg::lds_f64*)sb;
+  if (cnt == SOLVE_PAIR) bb_solve_cols<SOLVE_PAIR>(t, n, col, sbl);
+  else bb_solve_cols<1>(t, n, col, sbl);
+  for (int e = threadIdx.x; e < n * cnt; e += SOLVE_NT) b[e % n + (size_t)(e / n) * t.ldb] = col[e];
 }
 
 }  // namespace bb
@@ -377,7 +391,8 @@ struct BlockedGroup {
   std::vector<unsigned char> image;
   std::vector<Step> steps;
   size_t tasks = 0, solve = 0, counters = 0, spare_info = 0, swaps = 0;
-  int ntasks = 0, n_solve = 0, ncounters = 0, max_solve_n = 0, n_swaps = 0, swap_rows = 0, node_counter_base = 0;
+  int ntasks = 0, n_solve = 0, ncounters = 0, n_swaps = 0, swap_rows = 0, node_counter_base = 0;
+  size_t max_solve_words = 0;  // LDS words of the largest solve item
   unsigned char* dev = nullptr;  // device copy of the image
 };
 
@@ -581,13 +596,22 @@ BlockedGroup* group_build(const FactorTask* tasks, int count) {
   p->swaps = put(p->image, swaps);
   p->n_swaps = (int)swaps.size();
   std::vector<bb::Item> solve;
+  auto solves = [&](const FactorTask& t) { return !(t.kind == PGA_TASK_GETRF || t.kind == PGA_TASK_POTRF) && t.n > 0; };
+  int solve_n_max = 0;
+  for (int i = 0; i < count; ++i)
+    if (solves(tasks[i])) solve_n_max = std::max(solve_n_max, tasks[i].n);
   for (int i = 0; i < count; ++i) {
     const FactorTask& t = tasks[i];
-    const bool solves = !(t.kind == PGA_TASK_GETRF || t.kind == PGA_TASK_POTRF);
-    if (!solves || t.n == 0) continue;
-    for (int c = 0; c < t.nrhs; ++c) solve.push_back({i, 0, 0, c, 1});
-    p->max_solve_n = std::max(p->max_solve_n, t.n);
+    if (!solves(t)) continue;
+    // pairs for orders up to half the largest: a pair's workgroup then stays shorter than
+    // the largest order's single columns, which end the launch
+    const int per = (t.n <= bb::SOLVE_PAIR_MAX_N && 2 * t.n <= solve_n_max) ? bb::SOLVE_PAIR : 1;
+    for (int c = 0; c < t.nrhs; c += per) solve.push_back({i, 0, 0, c, std::min(per, t.nrhs - c)});
+    p->max_solve_words = std::max(p->max_solve_words, (size_t)t.n * std::min(per, t.nrhs));
   }
+  // the largest orders first: the launch ends with its longest workgroups otherwise
+  std::stable_sort(solve.begin(), solve.end(),
+                   [&](const bb::Item& x, const bb::Item& y) { return tasks[x.task].n > tasks[y.task].n; });
   p->solve = put(p->image, solve);
   p->n_solve = (int)solve.size();
   p->node_counter_base = 2 * (int)p->steps.size();
@@ -659,7 +683,7 @@ hipError_t group_launch(const BlockedGroup* p, hipStream_t s) {
     hipLaunchKernelGGL(k_bb_laswp_left, dim3(p->n_swaps), dim3(NT), (size_t)p->swap_rows * sizeof(uint16_t), s, tasks,
                        reinterpret_cast<const Item*>(d + p->swaps));
   if (p->n_solve > 0)
-    hipLaunchKernelGGL(k_bb_solve, dim3(p->n_solve), dim3(SOLVE_NT), (size_t)p->max_solve_n * 4, s, tasks,
+    hipLaunchKernelGGL(k_bb_solve, dim3(p->n_solve), dim3(SOLVE_NT), p->max_solve_words * sizeof(uint32_t), s, tasks,
                        reinterpret_cast<const Item*>(d + p->solve));
   return hipGetLastError();
 }

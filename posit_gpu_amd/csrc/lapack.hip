@@ -563,7 +563,7 @@ __global__ __launch_bounds__(TRSM_COL_NT) void k_trsm_col(int m, const uint32_t*
   for (int i = threadIdx.x; i < m; i += TRSM_COL_NT) col[i] = b[i];
   gemm::round_table_fill(threadIdx.x, TRSM_COL_NT);
   __syncthreads();
-  if (!wg::col_solve_fp64<MODE, TRSM_COL_NT>(m, A, lda, (wg::lds_u32*)col, (wg::lds_f64*)sb)) {
+  if (!wg::col_solve_fp64<MODE, TRSM_COL_NT>(m, A, lda, (wg::lds_u32*)col, m, (wg::lds_f64*)sb)) {
     if (MODE == wg::CS_LLNU) wg::trsm_llnu<TRSM_COL_NT>(m, 1, A, lda, col, m);
     else if (MODE == wg::CS_LLNN) wg::trsm_llnn<TRSM_COL_NT>(m, 1, A, lda, col, m);
     else if (MODE == wg::CS_LUNN) wg::trsm_lunn<TRSM_COL_NT>(m, 1, A, lda, col, m);

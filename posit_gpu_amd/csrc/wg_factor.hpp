@@ -283,20 +283,22 @@ static __device__ __noinline__ void trsm_waves(int m, int n, const uint32_t* A, 
   __syncthreads();
 }
 
-// One right-hand side of n rows solved by all T lanes of the block in FP64, the forms
-// of trsm_llnu, trsm_llnn, trsm_lunn and trsm_lutn with one column: row i is held by
-// lane i % T in register slot i / T.  The sweep goes 64 rows at a time: the wave holding
-// a block's rows runs the block's steps with shuffles (as trsm_waves does) and leaves
-// the block's final values in LDS (sb: 2 x 64 values, alternating); after one barrier
-// every lane applies the block's terms to its rows beyond the block.  Every row so
+// C right-hand sides of n rows solved together by all T lanes of the block in FP64, the
+// forms of trsm_llnu, trsm_llnn, trsm_lunn and trsm_lutn: row i of column c is held by
+// lane i % T in register slot c * (COL_SOLVE_SLOTS / C) + i / T.  The sweep goes 64 rows
+// at a time: the wave holding a block's rows runs the block's steps with shuffles (as
+// trsm_waves does) and leaves the block's final values in LDS (sb: C x 128 values, two
+// blocks alternating); after one barrier every lane applies the block's terms to its rows
+// beyond the block.  The C columns share each factor entry's decoding, the block's wave
+// and the barriers.  Every row so
 // receives its terms in the pattern-level sweep's k order (ascending; descending for
 // the upper form), zero values of b_k skipping their terms in the same places, each
 // term rounded as the pattern-level operations round it (gemm::fp64_mac, its constants
 // read from gemm::round_table, which the calling kernel fills first; fp64_posit_div).
-// col holds the column as patterns (LDS) and receives the solution;
-// when some value leaves scales -108 .. 107 or meets a NaR or a zero divisor (a NaN in
-// some lane) the routine returns false with col untouched, and the caller runs the
-// pattern-level sweep.  n <= COL_SOLVE_SLOTS * T.
+// col holds the C columns as patterns (LDS, column c at col + c * ldc) and receives the
+// solutions; when some value leaves scales -108 .. 107 or meets a NaR or a zero divisor
+// (a NaN in some lane) the routine returns false with col untouched, and the caller runs
+// the pattern-level sweep.  n <= COL_SOLVE_SLOTS / C * T.
 enum ColSolve { CS_LLNU, CS_LLNN, CS_LUNN, CS_LUTN };
 constexpr int COL_SOLVE_SLOTS = 16;
 typedef __attribute__((address_space(3))) double lds_f64;
@@ -326,21 +328,23 @@ static __device__ __forceinline__ double col_div(double b, double a) {
   return r;
 }
 
-template <int MODE, int T>
+template <int MODE, int T, int C = 1>
 static __device__ __noinline__ bool col_solve_fp64(int n, const uint32_t* __restrict__ Ag, int lda, lds_u32* col,
-                                                   lds_f64* sb) {
-  constexpr int S = COL_SOLVE_SLOTS;
+                                                   int ldc, lds_f64* sb) {
+  constexpr int S = COL_SOLVE_SLOTS / C;  // slots of one column; column c holds b[c * S + s]
   constexpr bool BACK = MODE == CS_LUNN;
-  static_assert(T % 64 == 0, "whole waves");
+  static_assert(T % 64 == 0 && COL_SOLVE_SLOTS % C == 0, "whole waves, whole slots");
   typedef const __attribute__((address_space(1))) uint32_t gbl_u32;  // global loads, counted apart from LDS
   gbl_u32* A = (gbl_u32*)Ag;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   col_slots b;
 #pragma unroll
-  for (int s = 0; s < S; ++s) {
-    const int i = s * T + tid;
-    b[s] = i < n ? to_double(col[i]) : 0.0;
-  }
+  for (int c = 0; c < C; ++c)
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      const int i = s * T + tid;
+      b[c * S + s] = i < n ? to_double(col[c * ldc + i]) : 0.0;
+    }
   // a(r, c) of the sweep: LUTN reads u_kr (row k of U), the others l_rk / u_rk
   auto entry = [&](int r, int c) -> uint32_t {
     return MODE == CS_LUTN ? A[c + (size_t)r * lda] : A[r + (size_t)c * lda];
@@ -349,10 +353,12 @@ static __device__ __noinline__ bool col_solve_fp64(int n, const uint32_t* __rest
   for (int q = 0; q < nblk; ++q) {
     const int kb = BACK ? nblk - 1 - q : q;
     const int k0 = kb * 64, m = min(64, n - k0);
-    lds_f64* sk = sb + (q & 1) * 64;
+    lds_f64* sk = sb + (q & 1) * 64;  // column c's values at sk[c * 128 + r]
     if (wave == (k0 % T) / 64) {  // the block's own steps
       const int hs = k0 / T;
-      double x = b[hs];
+      double x[C];
+#pragma unroll
+      for (int c = 0; c < C; ++c) x[c] = b[c * S + hs];
       const bool row_in = lane < m;
       for (int h = 0; h < 4; ++h) {
         const int kc = BACK ? 48 - 16 * h : 16 * h;  // 16 steps' entries at a time
@@ -363,21 +369,30 @@ static __device__ __noinline__ bool col_solve_fp64(int n, const uint32_t* __rest
         for (int jj = 0; jj < 16; ++jj) {
           const int j = BACK ? 15 - jj : jj, k = kc + j;
           if (k >= m) continue;  // uniform
-          if (MODE == CS_LUTN) {
-            if (lane == k) x = col_div(x, to_double(av[j]));
-            const double bk = lane_value(x, k);
-            if (lane > k && row_in) x = gemm::fp64_mac(x, -to_double(av[j]), bk);
-          } else {
-            double bk = lane_value(x, k);
-            if (bk == 0.0) continue;  // the same value in every lane
-            if (MODE != CS_LLNU) bk = col_div(bk, lane_value(to_double(av[j]), k));
-            if ((BACK ? lane < k : lane > k) && row_in) x = gemm::fp64_mac(x, -bk, to_double(av[j]));
-            if (lane == k) x = bk;
+          const double a = to_double(av[j]);  // decoded once for the C columns
+          const double akk = MODE == CS_LLNU ? 1.0 : lane_value(a, k);
+#pragma unroll
+          for (int c = 0; c < C; ++c) {
+            if (MODE == CS_LUTN) {
+              if (lane == k) x[c] = col_div(x[c], a);
+              const double bk = lane_value(x[c], k);
+              if (lane > k && row_in) x[c] = gemm::fp64_mac(x[c], -a, bk);
+            } else {
+              double bk = lane_value(x[c], k);
+              if (bk != 0.0) {  // the same value in every lane; zero skips the step
+                if (MODE != CS_LLNU) bk = col_div(bk, akk);
+                if ((BACK ? lane < k : lane > k) && row_in) x[c] = gemm::fp64_mac(x[c], -bk, a);
+                if (lane == k) x[c] = bk;
+              }
+            }
           }
         }
       }
-      b[hs] = x;
-      if (row_in) sk[lane] = x;
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        b[c * S + hs] = x[c];
+        if (row_in) sk[c * 128 + lane] = x[c];
+      }
     }
     __syncthreads();
     // the block's terms on the rows beyond it: [k0 + m, n) forward, [0, k0) upward
@@ -386,7 +401,9 @@ static __device__ __noinline__ bool col_solve_fp64(int n, const uint32_t* __rest
     for (int s = lo / T; s <= (hi - 1) / T; ++s) {
       const int i = s * T + tid;
       if (i < lo || i >= hi) continue;
-      double x = b[s];
+      double x[C];
+#pragma unroll
+      for (int c = 0; c < C; ++c) x[c] = b[c * S + s];
       for (int kc = 0; kc < m; kc += 16) {
         uint32_t av[16];
 #pragma unroll
@@ -398,24 +415,34 @@ static __device__ __noinline__ bool col_solve_fp64(int n, const uint32_t* __rest
         for (int j = 0; j < 16; ++j) {
           const int t = kc + j;
           if (t >= m) continue;  // uniform
-          const double bk = sk[BACK ? m - 1 - t : t];
-          if (MODE == CS_LUTN) x = gemm::fp64_mac(x, -to_double(av[j]), bk);
-          else if (bk != 0.0) x = gemm::fp64_mac(x, -bk, to_double(av[j]));
+          const double a = to_double(av[j]);
+          const int r = BACK ? m - 1 - t : t;
+#pragma unroll
+          for (int c = 0; c < C; ++c) {
+            const double bk = sk[c * 128 + r];
+            if (MODE == CS_LUTN) x[c] = gemm::fp64_mac(x[c], -a, bk);
+            else if (bk != 0.0) x[c] = gemm::fp64_mac(x[c], -bk, a);
+          }
         }
       }
-      b[s] = x;
+#pragma unroll
+      for (int c = 0; c < C; ++c) b[c * S + s] = x[c];
     }
   }
   bool bad = false;
 #pragma unroll
-  for (int s = 0; s < S; ++s)
-    if (s * T + tid < n) bad |= b[s] != b[s];
+  for (int c = 0; c < C; ++c)
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+      if (s * T + tid < n) bad |= b[c * S + s] != b[c * S + s];
   if (__syncthreads_or(bad)) return false;
 #pragma unroll
-  for (int s = 0; s < S; ++s) {
-    const int i = s * T + tid;
-    if (i < n) col[i] = from_double(b[s]);
-  }
+  for (int c = 0; c < C; ++c)
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      const int i = s * T + tid;
+      if (i < n) col[c * ldc + i] = from_double(b[c * S + s]);
+    }
   __syncthreads();
   return true;
 }
